@@ -1,0 +1,321 @@
+// Context, resident graphs, parameters, errors and timing helpers.
+//
+// Replaces the per-app prepare_gpu()/clean_gpu() pairs (e.g.
+// apps/sssp/src/sssp_wrapper.cu:54-145): device buffers live in a resident
+// graph and are reused across calls instead of being allocated, copied and
+// freed (and the device reset) on every call.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+
+#include "smdp_internal.h"
+
+namespace smdp {
+
+static thread_local std::string g_last_error;
+
+int hip_fail(hipError_t e, const char* what, int line) {
+  char buf[512];
+  snprintf(buf, sizeof buf, "HIP error %d (%s) at line %d: %s", (int)e, hipGetErrorString(e), line, what);
+  g_last_error = buf;
+  return SMDP_ERR_DEVICE;
+}
+
+double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+static smdp_params default_params(int app) {
+  smdp_params p;
+  memset(&p, 0, sizeof p);
+  p.heavy_threshold = 64;  // THRESHOLD, apps/sssp/src/sssp_kernel.cu:7
+  p.split_threshold = 2048;
+  p.chunk = 2048;
+  p.wg_per_cu = 4;
+  p.batch_rounds = 4;
+  p.max_rounds = 1 << 22;
+  switch (app) {
+    case SMDP_APP_SPMV:
+    case SMDP_APP_PAGERANK:
+      p.split_threshold = 4096;
+      p.chunk = 4096;
+      break;
+    case SMDP_APP_MSSSP:
+      p.split_threshold = 512;
+      p.chunk = 256;
+      break;
+    default: break;
+  }
+  return p;
+}
+
+const smdp_params& params_of(smdp_ctx* ctx, int app) { return ctx->params[app]; }
+
+int grid_size(smdp_ctx* ctx, const smdp_params& p) {
+  int g = ctx->cus * (p.wg_per_cu > 0 ? p.wg_per_cu : 4);
+  return g > 0 ? g : 1;
+}
+
+void prof_reset(smdp_ctx* ctx) {
+  for (auto e : ctx->prof_ev) (void)hipEventDestroy(e);
+  ctx->prof_ev.clear();
+}
+
+int prof_mark(smdp_ctx* ctx) {
+  if (!ctx->profiling) return SMDP_OK;
+  hipEvent_t e;
+  SMDP_HIP(hipEventCreate(&e));
+  ctx->prof_ev.push_back(e);
+  SMDP_HIP(hipEventRecord(e, ctx->stream));
+  return SMDP_OK;
+}
+
+int prof_collect(smdp_ctx* ctx, smdp_stats* st) {
+  if (!ctx->profiling || !st) return SMDP_OK;
+  double ms = 0;
+  uint64_t n = 0;
+  for (size_t i = 0; i + 1 < ctx->prof_ev.size(); i += 2) {
+    float t = 0;
+    SMDP_HIP(hipEventElapsedTime(&t, ctx->prof_ev[i], ctx->prof_ev[i + 1]));
+    ms += t;
+    ++n;
+  }
+  st->ms_dominant = ms;
+  st->dominant_launches = n;
+  prof_reset(ctx);
+  return SMDP_OK;
+}
+
+int fill_stats(smdp_graph* g, smdp_stats* st, double ms_kernel, uint64_t launches) {
+  if (!st) return SMDP_OK;
+  Counters c;
+  SMDP_HIP(hipMemcpyAsync(&c, g->cnt, sizeof c, hipMemcpyDeviceToHost, g->ctx->stream));
+  SMDP_HIP(hipStreamSynchronize(g->ctx->stream));
+  st->ms_kernel = ms_kernel;
+  st->edges_relaxed = c.edges;
+  st->vertices_expanded = c.verts;
+  for (int i = 0; i < 4; ++i) st->q_push[i] = c.push[i];
+  st->rounds = c.rounds;
+  st->m_reached = c.reached;
+  st->kernel_launches = launches;
+  return SMDP_OK;
+}
+
+}  // namespace smdp
+
+using namespace smdp;
+
+extern "C" {
+
+const char* smdp_last_hip_error(void) { return g_last_error.c_str(); }
+
+const char* smdp_strerror(int code) {
+  switch (code) {
+    case SMDP_OK: return "ok";
+    case SMDP_ERR_ARG: return "bad argument";
+    case SMDP_ERR_NOMEM: return "out of memory";
+    case SMDP_ERR_DEVICE: return "device or HIP runtime error";
+    case SMDP_ERR_SOLUTION: return "unsupported solution";
+    case SMDP_ERR_OVERFLOW: return "queue or round capacity exceeded";
+    case SMDP_ERR_IO: return "input file missing or malformed";
+    case SMDP_ERR_FORMAT: return "wrong code for dataset";
+    case SMDP_ERR_NOT_BUILT: return "library built without device code";
+    default: return "unknown error";
+  }
+}
+
+int smdp_create(int device, smdp_ctx** out) {
+  if (!out) return SMDP_ERR_ARG;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return SMDP_ERR_DEVICE;
+  if (device < 0 || device >= count) return SMDP_ERR_ARG;
+  SMDP_HIP(hipSetDevice(device));
+  smdp_ctx* c = new smdp_ctx();
+  c->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+    delete c;
+    return SMDP_ERR_DEVICE;
+  }
+  c->cus = prop.multiProcessorCount;
+  strncpy(c->name, prop.name, sizeof c->name - 1);
+  strncpy(c->arch, prop.gcnArchName, sizeof c->arch - 1);
+  for (int a = 0; a < SMDP_APP_COUNT; ++a) c->params[a] = default_params(a);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev_begin) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess ||
+      hipHostMalloc((void**)&c->pinned, 64 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
+    smdp_destroy(c);
+    return SMDP_ERR_DEVICE;
+  }
+  *out = c;
+  return SMDP_OK;
+}
+
+void smdp_destroy(smdp_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  prof_reset(c);
+  if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->ev_begin) (void)hipEventDestroy(c->ev_begin);
+  if (c->ev_end) (void)hipEventDestroy(c->ev_end);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int smdp_get_params(smdp_ctx* ctx, int app, smdp_params* out) {
+  if (!ctx || !out || app < 0 || app >= SMDP_APP_COUNT) return SMDP_ERR_ARG;
+  *out = ctx->params[app];
+  return SMDP_OK;
+}
+
+int smdp_set_params(smdp_ctx* ctx, int app, const smdp_params* p) {
+  if (!ctx || !p || app < 0 || app >= SMDP_APP_COUNT) return SMDP_ERR_ARG;
+  smdp_params d = default_params(app);
+  smdp_params q = *p;
+  if (q.heavy_threshold <= 0) q.heavy_threshold = d.heavy_threshold;
+  if (q.split_threshold <= 0) q.split_threshold = d.split_threshold;
+  if (q.chunk <= 0) q.chunk = d.chunk;
+  if (q.wg_per_cu <= 0) q.wg_per_cu = d.wg_per_cu;
+  if (q.batch_rounds <= 0) q.batch_rounds = d.batch_rounds;
+  if (q.max_rounds <= 0) q.max_rounds = d.max_rounds;
+  if (q.wg_per_cu > 64 || q.chunk < 64) return SMDP_ERR_ARG;
+  ctx->params[app] = q;
+  return SMDP_OK;
+}
+
+int smdp_device_info(smdp_ctx* ctx, char* name, int name_len, char* arch, int arch_len, int32_t* cus) {
+  if (!ctx) return SMDP_ERR_ARG;
+  if (name && name_len > 0) {
+    strncpy(name, ctx->name, name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  if (arch && arch_len > 0) {
+    strncpy(arch, ctx->arch, arch_len - 1);
+    arch[arch_len - 1] = 0;
+  }
+  if (cus) *cus = ctx->cus;
+  return SMDP_OK;
+}
+
+int smdp_set_profiling(smdp_ctx* ctx, int on) {
+  if (!ctx) return SMDP_ERR_ARG;
+  ctx->profiling = on ? 1 : 0;
+  prof_reset(ctx);
+  return SMDP_OK;
+}
+
+int smdp_graph_create(smdp_ctx* ctx, const smdp_csr* h, smdp_graph** out) {
+  if (!ctx || !h || !out || h->n < 0 || h->m < 0 || !h->row || (h->m > 0 && !h->col)) return SMDP_ERR_ARG;
+  if (h->row[h->n] != h->m || h->row[0] != 0) return SMDP_ERR_ARG;
+  *out = nullptr;
+  SMDP_HIP(hipSetDevice(ctx->device));
+  smdp_graph* g = new smdp_graph();
+  g->ctx = ctx;
+  g->n = h->n;
+  g->m = h->m;
+  int rc = SMDP_OK;
+  auto up = [&](int32_t** d, const int32_t* s, int64_t cnt) -> int {
+    int r = alloc_once(d, cnt);
+    if (r != SMDP_OK) return r;
+    if (cnt > 0 && hipMemcpy(*d, s, sizeof(int32_t) * cnt, hipMemcpyHostToDevice) != hipSuccess)
+      return SMDP_ERR_DEVICE;
+    return SMDP_OK;
+  };
+  rc = up(&g->row, h->row, (int64_t)h->n + 1);
+  if (rc == SMDP_OK) rc = up(&g->col, h->col, h->m);
+  if (rc == SMDP_OK && h->w) rc = up(&g->w, h->w, h->m);
+  if (rc == SMDP_OK) rc = alloc_once(&g->ctl, 4);
+  if (rc == SMDP_OK) rc = alloc_once(&g->cnt, 1);
+  if (rc != SMDP_OK) {
+    smdp_graph_destroy(g);
+    return rc;
+  }
+  *out = g;
+  return SMDP_OK;
+}
+
+void smdp_graph_destroy(smdp_graph* g) {
+  if (!g) return;
+  (void)hipSetDevice(g->ctx->device);
+  void* bufs[] = {g->row,    g->col,      g->w,       g->dist,     g->stamp,  g->q[0],  g->q[1],
+                  g->hq,     g->ctl,      g->cnt,     g->val,      g->x,      g->y,     g->outdeg,
+                  g->rank[0], g->rank[1], g->contrib, g->partials, g->chunk_part, g->prio, g->resume,
+                  g->rmax,   g->mdist};
+  for (void* p : bufs)
+    if (p) (void)hipFree(p);
+  delete g;
+}
+
+static int upload_f32(float** d, const float* s, int64_t cnt) {
+  if (!s) return SMDP_ERR_ARG;
+  SMDP_CHECK(alloc_once(d, cnt));
+  if (cnt > 0) SMDP_HIP(hipMemcpy(*d, s, sizeof(float) * cnt, hipMemcpyHostToDevice));
+  return SMDP_OK;
+}
+
+int smdp_graph_set_values(smdp_graph* g, const float* val) {
+  if (!g) return SMDP_ERR_ARG;
+  SMDP_HIP(hipSetDevice(g->ctx->device));
+  return upload_f32(&g->val, val, g->m);
+}
+
+int smdp_graph_set_x(smdp_graph* g, const float* x) {
+  if (!g) return SMDP_ERR_ARG;
+  SMDP_HIP(hipSetDevice(g->ctx->device));
+  return upload_f32(&g->x, x, g->n);
+}
+
+int smdp_graph_set_outdeg(smdp_graph* g, const int32_t* outdeg) {
+  if (!g || !outdeg) return SMDP_ERR_ARG;
+  SMDP_HIP(hipSetDevice(g->ctx->device));
+  SMDP_CHECK(alloc_once(&g->outdeg, g->n));
+  if (g->n > 0) SMDP_HIP(hipMemcpy(g->outdeg, outdeg, sizeof(int32_t) * g->n, hipMemcpyHostToDevice));
+  return SMDP_OK;
+}
+
+int smdp_graph_set_prio(smdp_graph* g, const uint32_t* prio) {
+  if (!g || !prio) return SMDP_ERR_ARG;
+  SMDP_HIP(hipSetDevice(g->ctx->device));
+  SMDP_CHECK(alloc_once(&g->prio, g->n));
+  if (g->n > 0) SMDP_HIP(hipMemcpy(g->prio, prio, sizeof(uint32_t) * g->n, hipMemcpyHostToDevice));
+  return SMDP_OK;
+}
+
+int smdp_graph_read(smdp_graph* g, int what, void* out, int64_t count) {
+  if (!g || !out || count < 0) return SMDP_ERR_ARG;
+  SMDP_HIP(hipSetDevice(g->ctx->device));
+  const void* src = nullptr;
+  int64_t avail = 0;
+  size_t esz = 4;
+  switch (what) {
+    case SMDP_RES_DIST:
+    case SMDP_RES_LEVEL:
+    case SMDP_RES_COLOR:
+      src = g->dist;
+      avail = g->n;
+      break;
+    case SMDP_RES_Y:
+      src = g->y;
+      avail = g->n;
+      break;
+    case SMDP_RES_RANK:
+      src = g->rank[g->rank_cur];
+      avail = g->n;
+      break;
+    case SMDP_RES_MDIST:
+      return SMDP_ERR_ARG;  // multi-source results are read by the multi-source call
+    default: return SMDP_ERR_ARG;
+  }
+  // Only the result of the last run is defined (dist/level/colour share a buffer).
+  if (what != g->last_res) return SMDP_ERR_ARG;
+  if (!src || count > avail) return SMDP_ERR_ARG;
+  SMDP_HIP(hipMemcpyAsync(out, src, esz * (size_t)count, hipMemcpyDeviceToHost, g->ctx->stream));
+  SMDP_HIP(hipStreamSynchronize(g->ctx->stream));
+  return SMDP_OK;
+}
+
+}  // extern "C"

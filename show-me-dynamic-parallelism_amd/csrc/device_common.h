@@ -1,0 +1,129 @@
+// Wave64 building blocks shared by every kernel family (gfx950 / CDNA4).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SMDP_WAVE 64
+#define SMDP_WG 256
+#define SMDP_WAVES_PER_WG (SMDP_WG / SMDP_WAVE)
+
+namespace smdp {
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+// Number of set bits of `mask` below this lane (v_mbcnt_lo/hi).
+__device__ __forceinline__ int mbcnt(uint64_t mask) {
+  return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return (uint64_t)__ballot(p ? 1 : 0); }
+
+__device__ __forceinline__ int first_lane(uint64_t mask) { return __ffsll((unsigned long long)mask) - 1; }
+
+// Wave-aggregated append: every active lane with `pred` gets a distinct slot
+// in q[], at the cost of one atomic per wave.  Must be reached by all active
+// lanes of the wave together.  Returns the number of items the wave appended.
+__device__ __forceinline__ int wave_append(int32_t* __restrict__ q, int32_t* counter, int32_t item,
+                                           bool pred) {
+  uint64_t m = ballot(pred);
+  if (m == 0) return 0;
+  int leader = first_lane(m);
+  int cnt = __popcll(m);
+  int base = 0;
+  if (lane_id() == leader) base = atomicAdd(counter, cnt);
+  base = __shfl(base, leader);
+  if (pred) q[base + mbcnt(m)] = item;
+  return cnt;
+}
+
+// Inclusive prefix sum across the active wave (all 64 lanes must be active).
+__device__ __forceinline__ int wave_inclusive_scan(int x) {
+  const int l = lane_id();
+#pragma unroll
+  for (int off = 1; off < SMDP_WAVE; off <<= 1) {
+    int y = __shfl_up(x, off);
+    if (l >= off) x += y;
+  }
+  return x;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T x) {
+#pragma unroll
+  for (int off = SMDP_WAVE / 2; off > 0; off >>= 1) x += __shfl_xor(x, off);
+  return x;
+}
+
+__device__ __forceinline__ int wave_max(int x) {
+#pragma unroll
+  for (int off = SMDP_WAVE / 2; off > 0; off >>= 1) x = max(x, __shfl_xor(x, off));
+  return x;
+}
+
+// Exclusive scan over a 256-thread workgroup; returns the exclusive prefix and
+// writes the total to *total.  `lds` needs SMDP_WAVES_PER_WG ints.  Contains
+// barriers: every thread of the workgroup must call it.
+__device__ __forceinline__ int wg_exclusive_scan(int x, int* lds, int* total) {
+  const int wave = threadIdx.x / SMDP_WAVE;
+  int inc = wave_inclusive_scan(x);
+  if (lane_id() == SMDP_WAVE - 1) lds[wave] = inc;
+  __syncthreads();
+  int before = 0, sum = 0;
+#pragma unroll
+  for (int k = 0; k < SMDP_WAVES_PER_WG; ++k) {
+    int v = lds[k];
+    before += (k < wave) ? v : 0;
+    sum += v;
+  }
+  __syncthreads();
+  *total = sum;
+  return before + inc - x;
+}
+
+// ---- compensated FP32 arithmetic --------------------------------------------
+// TwoSum (Knuth): s + e == a + b exactly.
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+  s = a + b;
+  float bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+// A running sum with a second word that collects the rounding errors
+// (Ogita-Rump-Oishi Sum2 / Dot2).  The error word absorbs the exact error of
+// every addition and, via fma, of every product.
+struct CSum {
+  float s, c;
+  __device__ __forceinline__ CSum() : s(0.f), c(0.f) {}
+  __device__ __forceinline__ void add(float x) {
+    float t, e;
+    two_sum(s, x, t, e);
+    s = t;
+    c += e;
+  }
+  // Adds the exact product a*b: p = fl(a*b), fma(a,b,-p) is its rounding error.
+  __device__ __forceinline__ void add_prod(float a, float b) {
+    float p = a * b;
+    float pe = __builtin_fmaf(a, b, -p);
+    add(p);
+    c += pe;
+  }
+  __device__ __forceinline__ void merge(const CSum& o) {
+    add(o.s);
+    c += o.c;
+  }
+  __device__ __forceinline__ float value() const { return s + c; }
+};
+
+__device__ __forceinline__ CSum wave_csum(CSum x) {
+#pragma unroll
+  for (int off = SMDP_WAVE / 2; off > 0; off >>= 1) {
+    CSum o;
+    o.s = __shfl_xor(x.s, off);
+    o.c = __shfl_xor(x.c, off);
+    x.merge(o);
+  }
+  return x;
+}
+
+}  // namespace smdp

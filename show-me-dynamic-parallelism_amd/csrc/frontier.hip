@@ -1,0 +1,528 @@
+// Frontier expansion for SSSP and BFS on gfx950.
+//
+// Reference algorithm (apps/sssp/src/sssp_kernel.cu:40-61, :496-506 and
+// apps/sssp/src/sssp_wrapper.cu:147-302): unordered worklist Bellman-Ford.
+// Every round takes the queue of vertices whose cost dropped, and for each
+// arc (u,v,w) of a queued u does "alt = cost[u] + w; if (cost[v] > alt)
+// atomicMin(cost[v], alt) and mark v".  Marked vertices form the next queue.
+// BFS levels (apps/bfs-rec/src/bfs_rec_kernel.cu:42-141) are the same
+// relaxation with w = 1 and the 0x7FFFFFFF sentinel.
+//
+// What is different here:
+//  * The next queue is built while relaxing: the lane whose atomicMin lowers
+//    cost[v] claims v for the next round through stamp[v] (one atomicExch per
+//    improvement), and claims are compacted with one atomic per wave
+//    (ballot + mbcnt).  The reference instead rescans all N update flags every
+//    round (unorder_generateQueue_kernel).
+//  * Heavy vertices (degree >= threshold) are never handed to a child grid.
+//    They go to a per-wave LDS queue, a per-workgroup LDS queue or a grid-wide
+//    HBM queue, according to the solution, and are drained by waves,
+//    workgroups or the whole grid (phase 2 kernel).
+//  * Rounds are launched back to back with the round number as an argument;
+//    each kernel reads its queue length from device memory and returns at
+//    once when it is zero, so the host only reads the queue length back once
+//    per batch of rounds instead of once per round.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "device_common.h"
+#include "smdp_internal.h"
+
+namespace smdp {
+
+constexpr int kWaveQ = 256;   // per-wave LDS queue capacity (SMDP_SOL_WAVE)
+constexpr int kBlockQ = 1024; // per-workgroup LDS queue capacity (SMDP_SOL_BLOCK)
+
+struct FArgs {
+  const int32_t* __restrict__ row;
+  const int32_t* __restrict__ col;
+  const int32_t* __restrict__ w;
+  int32_t* dist;
+  int32_t* stamp;
+  int32_t* q0;
+  int32_t* q1;
+  Chunk* hq;
+  Ctl* ctl;
+  Counters* cnt;
+  int32_t heavy;  // lane solutions: heavy degree; tuned: split degree
+  int32_t chunk;  // grid-queue chunk in edges; 0 = whole vertex
+};
+
+template <bool BFS>
+__device__ __forceinline__ int arc_weight(const FArgs& a, int e) {
+  return BFS ? 1 : a.w[e];
+}
+
+// Relax one arc with candidate `alt`.  Returns true when this lane lowered
+// dist[v] and is the first in this round to claim v for the next queue.
+// A plain read of dist[v] may be stale (larger) when another XCD lowered it;
+// the atomic decides, so a stale read only costs an extra atomic.
+__device__ __forceinline__ bool relax(const FArgs& a, int v, int alt, int label) {
+  if (alt >= a.dist[v]) return false;
+  int old = atomicMin(&a.dist[v], alt);
+  if (alt >= old) return false;
+  if (a.stamp[v] == label) return false;
+  return atomicExch(&a.stamp[v], label) != label;
+}
+
+__device__ __forceinline__ void clear_slot(Ctl* ctl, int r) {
+  if (blockIdx.x == 0 && threadIdx.x < 32) reinterpret_cast<int32_t*>(ctl + ((r + 2) & 3))[threadIdx.x] = 0;
+}
+
+__device__ __forceinline__ void flush_counters(Counters* cnt, unsigned long long edges, unsigned long long verts,
+                                               unsigned long long p0, unsigned long long p1,
+                                               unsigned long long p2, unsigned long long p3) {
+  edges = wave_sum(edges);
+  verts = wave_sum(verts);
+  p0 = wave_sum(p0);
+  p1 = wave_sum(p1);
+  p2 = wave_sum(p2);
+  p3 = wave_sum(p3);
+  if (lane_id() == 0) {
+    if (edges) atomicAdd(&cnt->edges, edges);
+    if (verts) atomicAdd(&cnt->verts, verts);
+    if (p0) atomicAdd(&cnt->push[0], p0);
+    if (p1) atomicAdd(&cnt->push[1], p1);
+    if (p2) atomicAdd(&cnt->push[2], p2);
+    if (p3) atomicAdd(&cnt->push[3], p3);
+  }
+}
+
+// Wave-cooperative expansion of arcs [s, e) of one vertex with cost du.
+template <bool BFS>
+__device__ __forceinline__ void wave_expand(const FArgs& a, int s, int e, int du, int32_t* qout,
+                                            int32_t* qlen, int label, unsigned long long& edges,
+                                            unsigned long long& pushes) {
+  const int lane = lane_id();
+  for (int b = s; b < e; b += SMDP_WAVE) {
+    int k = b + lane;
+    int v = 0;
+    bool p = false;
+    if (k < e) {
+      v = a.col[k];
+      p = relax(a, v, du + arc_weight<BFS>(a, k), label);
+      ++edges;
+    }
+    pushes += wave_append(qout, qlen, v, p);
+  }
+}
+
+// Workgroup-cooperative expansion (all SMDP_WG threads call it).
+template <bool BFS>
+__device__ __forceinline__ void wg_expand(const FArgs& a, int s, int e, int du, int32_t* qout,
+                                          int32_t* qlen, int label, unsigned long long& edges,
+                                          unsigned long long& pushes) {
+  for (int b = s; b < e; b += SMDP_WG) {
+    int k = b + (int)threadIdx.x;
+    int v = 0;
+    bool p = false;
+    if (k < e) {
+      v = a.col[k];
+      p = relax(a, v, du + arc_weight<BFS>(a, k), label);
+      ++edges;
+    }
+    pushes += wave_append(qout, qlen, v, p);
+  }
+}
+
+__global__ void k_init_frontier(int32_t* dist, int32_t* stamp, int32_t n, int32_t inf, int32_t src,
+                                int32_t* q0, Ctl* ctl, Counters* cnt) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    dist[i] = (i == src) ? 0 : inf;
+    stamp[i] = (i == src) ? 0 : -1;
+  }
+  if (blockIdx.x == 0) {
+    int32_t* c = reinterpret_cast<int32_t*>(ctl);
+    for (int i = threadIdx.x; i < 4 * 32; i += blockDim.x) c[i] = 0;
+    unsigned long long* k = reinterpret_cast<unsigned long long*>(cnt);
+    for (int i = threadIdx.x; i < (int)(sizeof(Counters) / 8); i += blockDim.x) k[i] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      q0[0] = src;
+      ctl[0].qlen = 1;
+    }
+  }
+}
+
+// One round with one lane per queued vertex (solutions FLAT..GRID).
+template <int SOL, bool BFS>
+__global__ __launch_bounds__(SMDP_WG) void k_expand_lane(FArgs a, int r) {
+  clear_slot(a.ctl, r);
+  Ctl* cin = a.ctl + (r & 3);
+  Ctl* cout = a.ctl + ((r + 1) & 3);
+  const int L = cin->qlen;
+  if (L == 0) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&a.cnt->rounds, (unsigned long long)(r + 1));
+  const int32_t* qin = (r & 1) ? a.q1 : a.q0;
+  int32_t* qout = (r & 1) ? a.q0 : a.q1;
+  const int label = r + 1;
+  const int lane = lane_id();
+  const int wave = threadIdx.x / SMDP_WAVE;
+
+  constexpr int kLdsQ = SOL == SMDP_SOL_BLOCK ? kBlockQ : (SOL == SMDP_SOL_WAVE ? SMDP_WAVES_PER_WG * kWaveQ : 1);
+  __shared__ int32_t s_q[kLdsQ];
+  __shared__ int32_t s_qn;
+  int wq_n = 0;  // SOL_WAVE: entries in this wave's LDS queue (wave-uniform)
+  if (SOL == SMDP_SOL_BLOCK) {
+    if (threadIdx.x == 0) s_qn = 0;
+    __syncthreads();
+  }
+
+  unsigned long long edges = 0, verts = 0, pf = 0, pw = 0, pb = 0, pg = 0;
+  const int stride = gridDim.x * SMDP_WG;
+  for (int base = blockIdx.x * SMDP_WG; base < L; base += stride) {
+    const int i = base + (int)threadIdx.x;
+    const bool valid = i < L;
+    int u = 0, s = 0, e = 0, du = 0;
+    if (valid) {
+      u = qin[i];
+      s = a.row[u];
+      e = a.row[u + 1];
+      du = a.dist[u];
+      ++verts;
+    }
+    const bool heavy = SOL != SMDP_SOL_FLAT && valid && (e - s) >= a.heavy;
+
+    if (SOL == SMDP_SOL_NAIVE) {
+      // Analogue of the per-thread child launch: each heavy vertex is
+      // expanded right away by the wave that found it.
+      uint64_t hm = ballot(heavy);
+      while (hm) {
+        int l = first_lane(hm);
+        hm &= hm - 1;
+        ++pw;
+        wave_expand<BFS>(a, __shfl(s, l), __shfl(e, l), __shfl(du, l), qout, &cout->qlen, label, edges, pf);
+      }
+    }
+
+    if (!heavy) {
+      for (int k = s; k < e; ++k) {
+        int v = a.col[k];
+        bool p = relax(a, v, du + arc_weight<BFS>(a, k), label);
+        ++edges;
+        pf += wave_append(qout, &cout->qlen, v, p);
+      }
+    }
+
+    if (SOL == SMDP_SOL_WAVE) {
+      uint64_t hm = ballot(heavy);
+      if (heavy) s_q[wave * kWaveQ + wq_n + mbcnt(hm)] = u;
+      // Lanes read entries other lanes wrote: order the LDS traffic.
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      wq_n += __popcll(hm);
+      pw += (lane == 0) ? __popcll(hm) : 0;
+      // Drain when another iteration could overflow, and at the end.
+      bool last = base + stride >= L;
+      if (wq_n > kWaveQ - SMDP_WAVE || (last && wq_n > 0)) {
+        for (int j = 0; j < wq_n; ++j) {
+          int hu = s_q[wave * kWaveQ + j];
+          wave_expand<BFS>(a, a.row[hu], a.row[hu + 1], a.dist[hu], qout, &cout->qlen, label, edges, pf);
+        }
+        wq_n = 0;
+      }
+    }
+
+    if (SOL == SMDP_SOL_BLOCK) {
+      uint64_t hm = ballot(heavy);
+      if (hm) {
+        int b0 = 0;
+        if (lane == first_lane(hm)) b0 = atomicAdd(&s_qn, __popcll(hm));
+        b0 = __shfl(b0, first_lane(hm));
+        if (heavy) s_q[b0 + mbcnt(hm)] = u;
+        pb += (lane == first_lane(hm)) ? __popcll(hm) : 0;
+      }
+      __syncthreads();
+      const int qn = s_qn;
+      // Every wave must read the count before any wave appends again.
+      __syncthreads();
+      const bool last = base + stride >= L;
+      if (qn > kBlockQ - SMDP_WG || (last && qn > 0)) {
+        for (int j = 0; j < qn; ++j) {
+          int hu = s_q[j];
+          wg_expand<BFS>(a, a.row[hu], a.row[hu + 1], a.dist[hu], qout, &cout->qlen, label, edges, pf);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) s_qn = 0;
+        __syncthreads();
+      }
+    }
+
+    if (SOL == SMDP_SOL_GRID) {
+      uint64_t hm = ballot(heavy);
+      if (hm) {
+        int leader = first_lane(hm);
+        int b0 = 0;
+        if (lane == leader) b0 = atomicAdd(&cin->hlen, __popcll(hm));
+        b0 = __shfl(b0, leader);
+        if (heavy) a.hq[b0 + mbcnt(hm)] = Chunk{u, s};
+        pg += (lane == leader) ? __popcll(hm) : 0;
+      }
+    }
+  }
+  flush_counters(a.cnt, edges, verts, pf, pw, pb, pg);
+}
+
+// One round of the tuned solution: the workgroup takes tiles of SMDP_WG queue
+// entries from a shared counter, stages (first arc, degree, cost) in LDS,
+// scans the degrees and lets every thread take consecutive arcs of the tile,
+// so loads of col/w are coalesced and lanes stay balanced whatever the degree
+// mix.  Vertices of degree >= split are cut into chunks on the grid queue.
+template <bool BFS>
+__global__ __launch_bounds__(SMDP_WG) void k_expand_tile(FArgs a, int r) {
+  clear_slot(a.ctl, r);
+  Ctl* cin = a.ctl + (r & 3);
+  Ctl* cout = a.ctl + ((r + 1) & 3);
+  const int L = cin->qlen;
+  if (L == 0) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&a.cnt->rounds, (unsigned long long)(r + 1));
+  const int32_t* qin = (r & 1) ? a.q1 : a.q0;
+  int32_t* qout = (r & 1) ? a.q0 : a.q1;
+  const int label = r + 1;
+  const int lane = lane_id();
+
+  __shared__ int s_tile;
+  __shared__ int s_start[SMDP_WG];
+  __shared__ int s_du[SMDP_WG];
+  __shared__ int s_off[SMDP_WG];
+  __shared__ int s_scan[SMDP_WAVES_PER_WG];
+
+  unsigned long long edges = 0, verts = 0, pf = 0, pg = 0;
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = atomicAdd(&cin->tile_next, 1);
+    __syncthreads();
+    const int base = s_tile * SMDP_WG;
+    if (base >= L) break;
+    const int i = base + (int)threadIdx.x;
+    const bool valid = i < L;
+    int u = 0, s = 0, deg = 0, du = 0;
+    if (valid) {
+      u = qin[i];
+      s = a.row[u];
+      deg = a.row[u + 1] - s;
+      du = a.dist[u];
+      ++verts;
+    }
+    const bool split = deg >= a.heavy;
+    const int nch = split ? (deg + a.chunk - 1) / a.chunk : 0;
+    const int incl = wave_inclusive_scan(nch);
+    const int wtot = __shfl(incl, SMDP_WAVE - 1);
+    if (wtot > 0) {
+      int b0 = 0;
+      if (lane == 0) b0 = atomicAdd(&cin->hlen, wtot);
+      b0 = __shfl(b0, 0);
+      int off = b0 + incl - nch;
+      for (int c = 0; c < nch; ++c) a.hq[off + c] = Chunk{u, s + c * a.chunk};
+      pg += nch;
+    }
+    s_start[threadIdx.x] = s;
+    s_du[threadIdx.x] = du;
+    int tot;
+    const int ex = wg_exclusive_scan(split ? 0 : deg, s_scan, &tot);
+    s_off[threadIdx.x] = ex;
+    __syncthreads();
+    for (int jb = 0; jb < tot; jb += SMDP_WG) {
+      const int j = jb + (int)threadIdx.x;
+      int v = 0;
+      bool p = false;
+      if (j < tot) {
+        // owner: the last tile slot whose arc range starts at or before j
+        int lo = 0, hi = SMDP_WG - 1;
+        while (lo < hi) {
+          int mid = (lo + hi + 1) >> 1;
+          if (s_off[mid] <= j) lo = mid;
+          else hi = mid - 1;
+        }
+        const int k = s_start[lo] + (j - s_off[lo]);
+        v = a.col[k];
+        p = relax(a, v, s_du[lo] + arc_weight<BFS>(a, k), label);
+        ++edges;
+      }
+      pf += wave_append(qout, &cout->qlen, v, p);
+    }
+    __syncthreads();
+  }
+  flush_counters(a.cnt, edges, verts, pf, 0, 0, pg);
+}
+
+// Phase 2 of the grid solutions: workgroups take grid-queue entries from a
+// shared counter and expand them with all SMDP_WG threads.
+template <bool BFS>
+__global__ __launch_bounds__(SMDP_WG) void k_drain(FArgs a, int r) {
+  Ctl* cin = a.ctl + (r & 3);
+  Ctl* cout = a.ctl + ((r + 1) & 3);
+  const int H = cin->hlen;
+  if (H == 0) return;
+  int32_t* qout = (r & 1) ? a.q0 : a.q1;
+  const int label = r + 1;
+  __shared__ int s_idx;
+  unsigned long long edges = 0, pf = 0;
+  for (;;) {
+    if (threadIdx.x == 0) s_idx = atomicAdd(&cin->chunk_next, 1);
+    __syncthreads();
+    const int idx = s_idx;
+    __syncthreads();
+    if (idx >= H) break;
+    const Chunk c = a.hq[idx];
+    const int vend = a.row[c.v + 1];
+    const int end = (a.chunk > 0 && vend - c.first > a.chunk) ? c.first + a.chunk : vend;
+    wg_expand<BFS>(a, c.first, end, a.dist[c.v], qout, &cout->qlen, label, edges, pf);
+  }
+  flush_counters(a.cnt, edges, 0, pf, 0, 0, 0);
+}
+
+// Out-arcs of reached vertices (the GTEPS numerator).
+__global__ void k_reached(const int32_t* __restrict__ row, const int32_t* __restrict__ dist, int32_t n,
+                          int32_t inf, Counters* cnt) {
+  unsigned long long acc = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    if (dist[i] != inf) acc += (unsigned long long)(row[i + 1] - row[i]);
+  acc = wave_sum(acc);
+  if (lane_id() == 0 && acc) atomicAdd(&cnt->reached, acc);
+}
+
+template <bool BFS>
+static void launch_round(int sol, int grid, hipStream_t st, const FArgs& a, int r, int* launches) {
+  dim3 g(grid), b(SMDP_WG);
+  switch (sol) {
+    case SMDP_SOL_FLAT: hipLaunchKernelGGL((k_expand_lane<SMDP_SOL_FLAT, BFS>), g, b, 0, st, a, r); break;
+    case SMDP_SOL_NAIVE:
+    case SMDP_SOL_HIER: hipLaunchKernelGGL((k_expand_lane<SMDP_SOL_NAIVE, BFS>), g, b, 0, st, a, r); break;
+    case SMDP_SOL_WAVE: hipLaunchKernelGGL((k_expand_lane<SMDP_SOL_WAVE, BFS>), g, b, 0, st, a, r); break;
+    case SMDP_SOL_BLOCK: hipLaunchKernelGGL((k_expand_lane<SMDP_SOL_BLOCK, BFS>), g, b, 0, st, a, r); break;
+    case SMDP_SOL_GRID: hipLaunchKernelGGL((k_expand_lane<SMDP_SOL_GRID, BFS>), g, b, 0, st, a, r); break;
+    default: hipLaunchKernelGGL((k_expand_tile<BFS>), g, b, 0, st, a, r); break;
+  }
+  ++*launches;
+}
+
+template <bool BFS>
+static void launch_drain(int grid, hipStream_t st, const FArgs& a, int r, int* launches) {
+  hipLaunchKernelGGL((k_drain<BFS>), dim3(grid), dim3(SMDP_WG), 0, st, a, r);
+  ++*launches;
+}
+
+static int ensure_frontier(smdp_graph* g, int32_t chunk) {
+  SMDP_CHECK(alloc_once(&g->dist, g->n));
+  SMDP_CHECK(alloc_once(&g->stamp, g->n));
+  SMDP_CHECK(alloc_once(&g->q[0], g->n));
+  SMDP_CHECK(alloc_once(&g->q[1], g->n));
+  // Each vertex enters a round's queue at most once (stamp), so a round pushes
+  // at most n whole-vertex entries or n + m/chunk chunk entries.
+  int64_t need = (int64_t)g->n + (chunk > 0 ? (int64_t)g->m / chunk : 0) + 64;
+  return ensure(&g->hq, &g->hq_cap, need);
+}
+
+// Runs SSSP (bfs = false) or BFS from src with the given solution; results stay
+// in g->dist.
+int run_frontier(smdp_graph* g, int32_t src, int sol, bool bfs, smdp_stats* st) {
+  if (!g) return SMDP_ERR_ARG;
+  if (g->n <= 0 || src < 0 || src >= g->n) return SMDP_ERR_ARG;
+  if (sol < 0 || sol >= SMDP_SOL_COUNT) return SMDP_ERR_SOLUTION;
+  if (sol == SMDP_SOL_HIER && !bfs) return SMDP_ERR_SOLUTION;
+  if (!bfs && g->m > 0 && !g->w) return SMDP_ERR_ARG;
+  smdp_ctx* ctx = g->ctx;
+  SMDP_HIP(hipSetDevice(ctx->device));
+  const smdp_params& p = params_of(ctx, bfs ? SMDP_APP_BFS : SMDP_APP_SSSP);
+  const bool tuned = sol == SMDP_SOL_TUNED;
+  SMDP_CHECK(ensure_frontier(g, tuned ? p.chunk : 0));
+
+  FArgs a;
+  a.row = g->row;
+  a.col = g->col;
+  a.w = g->w;
+  a.dist = g->dist;
+  a.stamp = g->stamp;
+  a.q0 = g->q[0];
+  a.q1 = g->q[1];
+  a.hq = g->hq;
+  a.ctl = g->ctl;
+  a.cnt = g->cnt;
+  a.heavy = tuned ? p.split_threshold : (sol == SMDP_SOL_HIER ? 1 : p.heavy_threshold);
+  a.chunk = tuned ? p.chunk : 0;
+  const int grid = grid_size(ctx, p);
+  const int32_t inf = bfs ? SMDP_BFS_UNDEF : SMDP_SSSP_INF;
+  const bool two_phase = sol == SMDP_SOL_GRID || tuned;
+  hipStream_t s = ctx->stream;
+  prof_reset(ctx);
+
+  int launches = 0;
+  SMDP_HIP(hipEventRecord(ctx->ev_begin, s));
+  hipLaunchKernelGGL(k_init_frontier, dim3(std::min<int64_t>((g->n + 255) / 256, 4096)), dim3(256), 0, s,
+                     g->dist, g->stamp, g->n, inf, src, g->q[0], g->ctl, g->cnt);
+  ++launches;
+  int r = 0;
+  int batch = std::max(1, p.batch_rounds);
+  for (;;) {
+    for (int j = 0; j < batch; ++j, ++r) {
+      SMDP_CHECK(prof_mark(ctx));
+      if (bfs) launch_round<true>(sol, grid, s, a, r, &launches);
+      else launch_round<false>(sol, grid, s, a, r, &launches);
+      SMDP_CHECK(prof_mark(ctx));
+      if (two_phase) {
+        if (bfs) launch_drain<true>(grid, s, a, r, &launches);
+        else launch_drain<false>(grid, s, a, r, &launches);
+      }
+    }
+    SMDP_HIP(hipGetLastError());
+    SMDP_HIP(hipMemcpyAsync(ctx->pinned, &g->ctl[r & 3].qlen, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    SMDP_HIP(hipStreamSynchronize(s));
+    if (ctx->pinned[0] == 0) break;
+    if (r >= p.max_rounds) return SMDP_ERR_OVERFLOW;
+    batch = std::min(batch * 2, 16);
+  }
+  SMDP_HIP(hipEventRecord(ctx->ev_end, s));
+  SMDP_HIP(hipEventSynchronize(ctx->ev_end));
+  float ms = 0;
+  SMDP_HIP(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end));
+  hipLaunchKernelGGL(k_reached, dim3(std::min<int64_t>((g->n + 255) / 256, 2048)), dim3(256), 0, s, g->row, g->dist,
+                     g->n, inf, g->cnt);
+  SMDP_HIP(hipGetLastError());
+  g->last_res = bfs ? SMDP_RES_LEVEL : SMDP_RES_DIST;
+  if (st) {
+    SMDP_CHECK(fill_stats(g, st, ms, launches));
+    SMDP_CHECK(prof_collect(ctx, st));
+  } else {
+    prof_reset(ctx);
+  }
+  return SMDP_OK;
+}
+
+// Host-array drop-in: upload, run, copy back (SSSP_GPU / BFS_REC_GPU shape).
+int frontier_dropin(smdp_ctx* ctx, const smdp_csr* h, int32_t src, int sol, bool bfs, int32_t* out,
+                    smdp_stats* st) {
+  if (!ctx || !h || !out) return SMDP_ERR_ARG;
+  double t0 = now_ms();
+  smdp_graph* g = nullptr;
+  SMDP_CHECK(smdp_graph_create(ctx, h, &g));
+  int rc = run_frontier(g, src, sol, bfs, st);
+  if (rc == SMDP_OK) rc = smdp_graph_read(g, bfs ? SMDP_RES_LEVEL : SMDP_RES_DIST, out, g->n);
+  smdp_graph_destroy(g);
+  if (st) st->ms_total = now_ms() - t0;
+  return rc;
+}
+
+}  // namespace smdp
+
+using namespace smdp;
+
+extern "C" {
+
+int smdp_sssp_run(smdp_graph* g, int32_t src, int solution, smdp_stats* st) {
+  return run_frontier(g, src, solution, false, st);
+}
+
+int smdp_bfs_run(smdp_graph* g, int32_t src, int solution, smdp_stats* st) {
+  return run_frontier(g, src, solution, true, st);
+}
+
+int smdp_sssp(smdp_ctx* ctx, const smdp_csr* g, int32_t src, int solution, int32_t* dist, smdp_stats* st) {
+  return frontier_dropin(ctx, g, src, solution, false, dist, st);
+}
+
+int smdp_bfs(smdp_ctx* ctx, const smdp_csr* g, int32_t src, int solution, int32_t* level, smdp_stats* st) {
+  return frontier_dropin(ctx, g, src, solution, true, level, st);
+}
+
+}  // extern "C"

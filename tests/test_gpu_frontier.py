@@ -1,0 +1,142 @@
+"""SSSP and BFS on the device against the oracle, for every solution.
+
+Integer outputs must match bit for bit.  Inputs cover the reference's own
+fixture, the uniform-degree configuration (16,384 vertices, 8 arcs each),
+RMAT graphs whose hubs exceed the heavy threshold, a Zipf tree, and the edge
+cases the reference handles (unreachable vertices, self loops, duplicate arcs,
+a source without arcs, a single vertex).
+"""
+import numpy as np
+import pytest
+
+import orc
+import smdp
+from conftest import random_graph
+
+pytestmark = pytest.mark.gpu
+
+SSSP_SOLS = [smdp.SOL_FLAT, smdp.SOL_NAIVE, smdp.SOL_WAVE, smdp.SOL_BLOCK, smdp.SOL_GRID, smdp.SOL_TUNED]
+BFS_SOLS = SSSP_SOLS + [smdp.SOL_HIER]
+
+
+def _graphs():
+    out = {}
+    out["uniform16k"] = smdp.gen_uniform(16384, 8, 1, 100, 1)
+    out["rmat12"] = smdp.gen_rmat(12, 16, wlo=1, whi=255, seed=3)
+    out["rmat12_noperm"] = smdp.gen_rmat(12, 16, wlo=1, whi=255, permute=False, seed=6)
+    row, col, w = random_graph(3000, 9000, 21)
+    out["random_sparse"] = smdp.HostGraph(row, col, w, 0)
+    return out
+
+
+GRAPHS = _graphs()
+
+
+@pytest.mark.parametrize("name", sorted(GRAPHS))
+@pytest.mark.parametrize("sol", SSSP_SOLS)
+def test_sssp_parity(ctx, name, sol):
+    g = GRAPHS[name]
+    print(f"sssp {name} sol={sol}", flush=True)
+    ref = orc.dijkstra(g.row, g.col, g.w, g.source)
+    d, st = ctx.sssp(g, g.source, sol)
+    assert np.array_equal(d, ref)
+    assert st.rounds >= 1 and st.edges_relaxed >= np.diff(g.row)[g.source]
+    reached = ref != orc.INF
+    assert st.m_reached == int(np.diff(g.row)[reached].sum())
+
+
+@pytest.mark.parametrize("sol", SSSP_SOLS)
+def test_sssp_known_answer(ctx, golden, data_test_path, sol):
+    g = smdp.read_graph(data_test_path, smdp.FMT_DIMACS9)
+    d, _ = ctx.sssp(g, g.source, sol)
+    assert d.tolist() == golden["dist"]
+
+
+@pytest.mark.parametrize("name", sorted(GRAPHS))
+@pytest.mark.parametrize("sol", BFS_SOLS)
+def test_bfs_parity(ctx, name, sol):
+    g = GRAPHS[name]
+    print(f"bfs {name} sol={sol}", flush=True)
+    ref = orc.bfs(g.row, g.col, g.source)
+    lv, _ = ctx.bfs(g, g.source, sol)
+    assert np.array_equal(lv, ref)
+
+
+@pytest.mark.parametrize("sol", BFS_SOLS)
+def test_bfs_known_answer(ctx, golden, data_test_path, sol):
+    g = smdp.read_graph(data_test_path, smdp.FMT_DIMACS9)
+    lv, _ = ctx.bfs(g, g.source, sol)
+    assert lv.tolist() == golden["level"]
+
+
+@pytest.mark.parametrize("sol", BFS_SOLS)
+def test_bfs_zipf_tree_depth(ctx, sol):
+    t = smdp.gen_zipf_tree(200000, 1.5, 4096, 5)
+    lv, _ = ctx.bfs(t, 0, sol)
+    # in a tree the level is the depth: level[child] = level[parent] + 1
+    parent = np.repeat(np.arange(t.n), np.diff(t.row))
+    assert lv[0] == 0 and np.array_equal(lv[t.col], lv[parent] + 1)
+    assert np.array_equal(lv, orc.bfs(t.row, t.col, 0))
+
+
+def test_edge_cases(ctx):
+    cases = {
+        "single": smdp.HostGraph([0, 0], [], [], 0),
+        "no_edges": smdp.HostGraph([0, 0, 0, 0], [], [], 1),
+        "self_loops_dups": smdp.HostGraph([0, 4, 5, 6], [0, 1, 1, 2, 1, 0], [3, 5, 2, 9, 0, 1], 0),
+        "src_without_arcs": smdp.HostGraph([0, 1, 1, 2], [1, 0], [4, 4], 1),
+        "unreachable": smdp.HostGraph([0, 1, 1, 1], [1], [7], 0),
+    }
+    for name, g in cases.items():
+        ref = orc.dijkstra(g.row, g.col, g.w, g.source)
+        lref = orc.bfs(g.row, g.col, g.source)
+        for sol in SSSP_SOLS:
+            d, _ = ctx.sssp(g, g.source, sol)
+            assert np.array_equal(d, ref), (name, sol)
+            lv, _ = ctx.bfs(g, g.source, sol)
+            assert np.array_equal(lv, lref), (name, sol)
+
+
+def test_bad_arguments(ctx):
+    g = GRAPHS["uniform16k"]
+    with pytest.raises(smdp.SmdpError) as e:
+        ctx.sssp(g, g.n, smdp.SOL_FLAT)
+    assert e.value.code == smdp.ERR_ARG
+    with pytest.raises(smdp.SmdpError) as e:
+        ctx.sssp(g, 0, 99)
+    assert e.value.code == smdp.ERR_SOLUTION
+    with pytest.raises(smdp.SmdpError) as e:
+        ctx.sssp(g, 0, smdp.SOL_HIER)  # hierarchical is a BFS-only solution
+    assert e.value.code == smdp.ERR_SOLUTION
+
+
+def test_tiny_thresholds_force_every_queue(ctx):
+    """Thresholds of 1-2 push almost every vertex through the LDS and grid queues."""
+    g = GRAPHS["rmat12"]
+    ref = orc.dijkstra(g.row, g.col, g.w, g.source)
+    old = ctx.get_params(smdp.APP_SSSP)
+    try:
+        ctx.set_params(smdp.APP_SSSP, heavy_threshold=1, split_threshold=2, chunk=64, wg_per_cu=1)
+        for sol in SSSP_SOLS:
+            d, st = ctx.sssp(g, g.source, sol)
+            assert np.array_equal(d, ref), sol
+            if sol in (smdp.SOL_WAVE, smdp.SOL_BLOCK, smdp.SOL_GRID, smdp.SOL_NAIVE):
+                assert sum(st.q_push[1:]) > 0
+    finally:
+        ctx.set_params(smdp.APP_SSSP, **old.as_dict())
+
+
+def test_resident_graph_repeat_determinism(ctx):
+    g = GRAPHS["rmat12_noperm"]
+    ref = orc.dijkstra(g.row, g.col, g.w, g.source)
+    srcs = smdp.gen_sources(g, 4, 9)
+    with smdp.Graph(ctx, g) as dg:
+        for s in srcs:
+            r = orc.dijkstra(g.row, g.col, g.w, int(s))
+            for _ in range(2):
+                dg.sssp(int(s), smdp.SOL_TUNED)
+                assert np.array_equal(dg.read(smdp.RES_DIST), r)
+        dg.sssp(g.source, smdp.SOL_GRID)
+        assert np.array_equal(dg.read(smdp.RES_DIST), ref)
+        with pytest.raises(smdp.SmdpError):
+            dg.read(smdp.RES_LEVEL)  # the last run was SSSP

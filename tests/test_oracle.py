@@ -1,0 +1,215 @@
+"""Pins the CPU oracle (oracle/oracle.c) before it is trusted as the checker.
+
+Every oracle function is checked against something it shares no code with:
+known answers computed by library routines (tests/golden), scipy/networkx,
+a second algorithm, a closed form, or brute force on tiny inputs.  Each check
+is chosen so that a plausible slip (wrong sign, dropped term, off-by-one,
+transposed operand) fails it.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from scipy.sparse.csgraph import dijkstra as sp_dijkstra, shortest_path
+
+import orc
+from conftest import random_graph, symmetrize
+
+
+def _scipy_dist(row, col, w, src):
+    n = len(row) - 1
+    srcs = np.repeat(np.arange(n), np.diff(row))
+    best = {}
+    for u, v, ww in zip(srcs.tolist(), col.tolist(), w.tolist()):
+        if u == v:
+            continue
+        best[(u, v)] = min(ww, best.get((u, v), ww))
+    if not best:
+        d = np.full(n, np.inf)
+        d[src] = 0
+    else:
+        k = list(best)
+        A = sp.csr_matrix(([best[x] for x in k], ([x[0] for x in k], [x[1] for x in k])), shape=(n, n))
+        d = sp_dijkstra(A, directed=True, indices=src)
+    return np.where(np.isfinite(d), d, orc.INF).astype(np.int64)
+
+
+def _golden_graph(data_test_path):
+    import smdp
+    return smdp.read_graph(data_test_path, smdp.FMT_DIMACS9)
+
+
+# ---- SSSP ------------------------------------------------------------------
+
+def test_dijkstra_known_answer(golden, data_test_path):
+    g = _golden_graph(data_test_path)
+    assert orc.dijkstra(g.row, g.col, g.w, golden["source"]).tolist() == golden["dist"]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_dijkstra_vs_scipy(seed):
+    row, col, w = random_graph(300, 1500 + 200 * seed, seed, 1, 50 + 100 * seed)
+    d = orc.dijkstra(row, col, w, seed)
+    assert np.array_equal(d, _scipy_dist(row, col, w, seed))
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_dijkstra_vs_brute_force(seed):
+    row, col, w = random_graph(12, 30, 100 + seed, 1, 9)
+    assert np.array_equal(orc.dijkstra(row, col, w, 0), orc.bellman_ford(row, col, w, 0))
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_worklist_matches_dijkstra(seed):
+    row, col, w = random_graph(2000, 16000, 7 + seed, 1, 255)
+    d, rounds, edges = orc.sssp_worklist(row, col, w, 3)
+    assert np.array_equal(d, orc.dijkstra(row, col, w, 3))
+    assert rounds >= 1 and edges >= np.diff(row)[3]
+
+
+def test_sssp_unreachable_sentinel():
+    row = np.array([0, 1, 1, 1], dtype=np.int32)
+    col = np.array([1], dtype=np.int32)
+    w = np.array([5], dtype=np.int32)
+    assert orc.dijkstra(row, col, w, 0).tolist() == [0, 5, orc.INF]
+    assert orc.sssp_worklist(row, col, w, 0)[0].tolist() == [0, 5, orc.INF]
+
+
+# ---- BFS -------------------------------------------------------------------
+
+def test_bfs_known_answer(golden, data_test_path):
+    g = _golden_graph(data_test_path)
+    assert orc.bfs(g.row, g.col, golden["source"]).tolist() == golden["level"]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_bfs_vs_scipy(seed):
+    row, col, _ = random_graph(500, 1200 + 300 * seed, 50 + seed)
+    n = len(row) - 1
+    A = sp.csr_matrix((np.ones(len(col)), col, row), shape=(n, n))
+    ref = shortest_path(A, directed=True, unweighted=True, indices=seed)
+    ref = np.where(np.isfinite(ref), ref, orc.UNDEF).astype(np.int64)
+    assert np.array_equal(orc.bfs(row, col, seed), ref)
+
+
+# ---- coloring --------------------------------------------------------------
+
+def test_color_known_answer(golden, data_test_path):
+    g = _golden_graph(data_test_path)
+    c1, k1 = orc.color_rounds(g.row, g.col)
+    c2, k2 = orc.color_closed(g.row, g.col)
+    assert c1.tolist() == golden["color"] and c2.tolist() == golden["color"]
+    assert k1 == k2 == max(golden["color"])
+
+
+@pytest.mark.parametrize("seed", range(8))
+@pytest.mark.parametrize("use_prio", [False, True])
+def test_color_rounds_equals_closed_form(seed, use_prio):
+    row, col, _ = random_graph(400, 2500, 200 + seed)
+    prio = np.random.default_rng(seed).integers(0, 2**32, 400, dtype=np.uint64).astype(np.uint32) if use_prio else None
+    if use_prio and seed % 2:
+        prio = (prio % 7).astype(np.uint32)  # many ties: broken by vertex id
+    a, ka = orc.color_rounds(row, col, prio)
+    b, kb = orc.color_closed(row, col, prio)
+    assert np.array_equal(a, b) and ka == kb
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_color_proper_on_symmetric(seed):
+    row, col, _ = random_graph(300, 1500, 300 + seed, self_loops=False)
+    row, col = symmetrize(row, col)
+    prio = np.random.default_rng(seed).integers(0, 2**32, 300, dtype=np.uint64).astype(np.uint32)
+    c, _ = orc.color_closed(row, col, prio)
+    src = np.repeat(np.arange(300), np.diff(row))
+    assert np.all(c > 0)
+    assert not np.any(c[src] == c[col])
+
+
+def test_color_chain_brute_force():
+    # path 0-1-2-...-9 with identity priority: colours alternate 1,2
+    n = 10
+    src = list(range(n - 1)) + list(range(1, n))
+    dst = list(range(1, n)) + list(range(n - 1))
+    order = np.argsort(src, kind="stable")
+    src = np.array(src)[order]
+    dst = np.array(dst)[order]
+    row = np.searchsorted(src, np.arange(n + 1)).astype(np.int32)
+    # identity priority: the path in id order is one precedence chain
+    c, _ = orc.color_rounds(row, dst)
+    assert c.tolist() == list(range(1, n + 1))
+    # even vertices first: two colours alternate
+    prio = np.array([i % 2 for i in range(n)], dtype=np.uint32)
+    c, k = orc.color_rounds(row, dst, prio)
+    assert c.tolist() == [1, 2] * 5 and k == 2
+    # directed chain i -> i-1 only: colour(i) = i + 1
+    row2 = np.array([0] + list(range(0, n)), dtype=np.int32)
+    col2 = np.arange(0, n - 1, dtype=np.int32)
+    c2, k2 = orc.color_closed(row2, col2)
+    assert c2.tolist() == list(range(1, n + 1)) and k2 == n
+
+
+# ---- SpMV ------------------------------------------------------------------
+
+@pytest.mark.parametrize("seed", range(4))
+def test_spmv_vs_scipy(seed):
+    row, col, _ = random_graph(700, 9000, 400 + seed)
+    rng = np.random.default_rng(seed)
+    val = rng.uniform(-1, 1, len(col)).astype(np.float32)
+    x = rng.uniform(0, 1, 700).astype(np.float32)
+    A = sp.csr_matrix((val.astype(np.float64), col, row), shape=(700, 700))
+    ref = A @ x.astype(np.float64)
+    y = orc.spmv(row, col, val, x)
+    assert np.max(np.abs(y - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
+
+
+def test_spmv_reference_inputs_exact():
+    # data[i] = i%13+0.5, x[i] = i%17+0.5 (SpMV_wrapper.cu:105-110): 4*y is an
+    # integer, computed exactly with Python ints.
+    row, col, _ = random_graph(500, 20000, 9)
+    m = len(col)
+    data = (np.arange(m) % 13 + 0.5).astype(np.float32)
+    x = (np.arange(500) % 17 + 0.5).astype(np.float32)
+    y = orc.spmv(row, col, data, x)
+    for r in range(500):
+        s4 = sum((2 * (i % 13) + 1) * (2 * (col[i] % 17) + 1) for i in range(row[r], row[r + 1]))
+        assert y[r] * 4 == s4
+
+
+# ---- PageRank --------------------------------------------------------------
+
+def _pr_direct(row, col, d, T):
+    """Fixed point of the reference update by a dense linear solve."""
+    n = len(row) - 1
+    outdeg = np.diff(row)
+    M = np.zeros((n, n))
+    src = np.repeat(np.arange(n), outdeg)
+    for u, v in zip(src, col):
+        M[v, u] += 1.0 / outdeg[u]
+    dang = (outdeg == 0).astype(float)
+    A = np.eye(n) - d * M - d * np.outer(np.ones(n), dang) / n
+    b = np.full(n, (1 - d) * T / n)
+    return np.linalg.solve(A, b)
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("d,T", [(0.85, 1.0), (0.5, 1000.0)])
+def test_pagerank_fixed_point(seed, d, T):
+    import smdp
+    row, col, _ = random_graph(60, 200, 500 + seed)  # has dangling vertices
+    g = smdp.HostGraph(row, col)
+    ig = smdp.invert(g)
+    r, it, res = orc.pagerank(ig.row, ig.col, np.diff(row), d=d, T=T, tol=1e-15, max_iter=2000)
+    ref = _pr_direct(row, col, d, T)
+    assert np.max(np.abs(r - ref)) <= 1e-10 * T
+    assert abs(r.sum() - T) <= 1e-9 * T
+
+
+def test_pagerank_iteration_matched_and_init():
+    import smdp
+    row, col, _ = random_graph(50, 150, 77)
+    ig = smdp.invert(smdp.HostGraph(row, col))
+    od = np.diff(row)
+    r1, it1, _ = orc.pagerank(ig.row, ig.col, od, fixed_iters=1, init_zero=True, d=0.5, T=1000.0)
+    # one step from r = 0: every rank is (1-d)T/N (no dangling mass yet)
+    assert np.allclose(r1, 0.5 * 1000.0 / 50, rtol=0, atol=1e-12) and it1 == 1
+    r2, it2, res2 = orc.pagerank(ig.row, ig.col, od, tol=1e-6, max_iter=100)
+    assert it2 < 100 and res2 < 1e-6
