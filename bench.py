@@ -1,0 +1,263 @@
+#!/usr/bin/env python3
+"""Benchmark: GTEPS of SSSP and PageRank on RMAT-22 on one MI355X, fraction of
+the HBM roofline of the dominant kernel, and batched vs plain SSSP.
+
+`value` is SSSP GTEPS on RMAT scale 22, edge factor 16 (a/b/c/d =
+.57/.19/.19/.05, weights U[1,255], seed 3): reached out-arcs / wall time of one
+full single-source solve with the graph already resident in HBM.  A "step" is
+one such solve.  The PageRank (same graph, pull, d = 0.85, FP64 L1 < 1e-6 or
+100 iterations) and batched-SSSP figures ride along in the same JSON line.
+
+The library call is synchronous (it returns after the device finished and the
+round counter was read back), so the wall clock around K calls is the device
+time plus host launch overhead.
+"""
+import argparse
+import glob
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "show-me-dynamic-parallelism_amd"), os.path.join(ROOT, "oracle")]
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "GTEPS (SSSP, PageRank) on RMAT-22, 1 MI355X; % HBM roofline; batched vs plain"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def traffic_for(kernel):
+    """HBM bytes per launch of `kernel` from committed rocprofv3 --pmc summaries."""
+    for p in glob.glob(os.path.join(ROOT, "profiles", "traffic_*.json")):
+        try:
+            d = json.load(open(p))
+        except (OSError, ValueError):
+            continue
+        if d.get("kernel") == kernel and "bytes_per_launch" in d:
+            return d["bytes_per_launch"]
+    return None
+
+
+def sssp_section(ctx, g, dg, args, smdp, orc, np):
+    src = g.source
+    sol = smdp.SOL_TUNED
+    t0 = time.time()
+    params = dg.autotune(smdp.APP_SSSP, src, os.path.join(ROOT, "gpurun_out", "tune_cache.json")) \
+        if args.tune else ctx.get_params(smdp.APP_SSSP)
+    log(f"sssp: params {params.as_dict()} (tuning {time.time() - t0:.1f}s)")
+    for _ in range(args.warmup):
+        dg.sssp(src, sol)
+    t0 = time.perf_counter()
+    sts = [dg.sssp(src, sol) for _ in range(args.steps)]
+    wall = time.perf_counter() - t0
+    ms_step = wall * 1e3 / args.steps
+    st = sts[-1]
+    dist = dg.read(smdp.RES_DIST)
+    # parity on the full-size graph: exact against Dijkstra
+    ok = None
+    if not args.skip_check:
+        ref = orc.dijkstra(g.row, g.col, g.w, src)
+        ok = bool(np.array_equal(dist, ref))
+    gteps = st.m_reached / (ms_step * 1e6)
+    # roofline pass: per-launch HIP events on the library stream
+    ctx.set_profiling(True)
+    pst = dg.sssp(src, sol)
+    ctx.set_profiling(False)
+    e1 = pst.edges_relaxed - pst.edges_phase2
+    bytes1 = 12 * e1 + 20 * pst.vertices_expanded  # col + w + dist[v]; 2 offsets + dist[u] + queue in/out
+    bytes2 = 12 * pst.edges_phase2
+    k1 = {"kernel": "k_expand_tile<false>", "bytes": bytes1, "ms": pst.ms_dominant,
+          "launches": pst.dominant_launches}
+    k2 = {"kernel": "k_drain<false>", "bytes": bytes2, "ms": pst.ms_phase2, "launches": pst.phase2_launches}
+    dom = k1 if k1["ms"] >= k2["ms"] else k2
+    ach = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9 if dom["ms"] > 0 else 0.0
+    traffic = traffic_for(dom["kernel"])
+    roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": dom["kernel"], "launches": dom["launches"],
+            "avg_launch_us": round(dom["ms"] * 1e3 / max(1, dom["launches"]), 3),
+            "algorithmic_bytes_per_launch": int(dom["bytes"] / max(1, dom["launches"]))}
+    kernels = []
+    for k in (k1, k2):
+        a = k["bytes"] / (k["ms"] * 1e-3) / 1e9 if k["ms"] > 0 else 0.0
+        kernels.append({"kernel": k["kernel"], "ms_total": round(k["ms"], 4), "launches": k["launches"],
+                        "achieved_GBs": round(a, 1), "frac": round(a / HBM_PEAK_GBS, 4)})
+    whole = (bytes1 + bytes2) / (ms_step * 1e-3) / 1e9
+    info = {"source": int(src), "ms_per_solve": round(ms_step, 4), "ms_device": round(st.ms_kernel, 4),
+            "rounds": st.rounds, "edges_relaxed": st.edges_relaxed, "m_reached": st.m_reached,
+            "inflation": round(st.edges_relaxed / max(1, st.m_reached), 3),
+            "vertices_expanded": st.vertices_expanded, "kernel_launches": st.kernel_launches,
+            "exact_vs_dijkstra": ok, "atomics_tried": st.atomics_tried, "atomics_won": st.atomics_won, "whole_solve_GBs": round(whole, 1), "kernels": kernels,
+            "params": params.as_dict()}
+    return gteps, ms_step, roof, info, st
+
+
+def cpu_baseline(g, src, orc, np, budget_s=12.0):
+    """The oracle's worklist Bellman-Ford (the reference algorithm, one thread)."""
+    runs = 0
+    t0 = time.perf_counter()
+    mr = None
+    while True:
+        d, rounds, edges = orc.sssp_worklist(g.row, g.col, g.w, src)
+        runs += 1
+        if mr is None:
+            mr = int(np.diff(g.row)[d != orc.INF].sum())
+        if time.perf_counter() - t0 >= budget_s or runs >= 20:
+            break
+    t = (time.perf_counter() - t0) / runs
+    return {"value": round(mr / t / 1e9, 5), "unit": "GTEPS", "cores": 1, "kind": "port",
+            "sample": f"{runs} full solves of worklist Bellman-Ford (sssp_kernel.cu:40-61 restated, "
+                      f"oracle/oracle.c orc_sssp_worklist) on the same RMAT-22 graph and source; "
+                      f"{t:.3f} s each, {rounds} rounds, {edges} arcs scanned"}
+
+
+def pagerank_section(ctx, g, args, smdp, orc, np):
+    ig = smdp.invert(g)
+    od = g.outdegree()
+    opts = smdp.pr_opts(0.85, 1.0, 1e-6, 100)
+    with smdp.Graph(ctx, smdp.HostGraph(ig.row, ig.col)) as dg:
+        dg.set_outdeg(od)
+        sol = smdp.SOL_TUNED
+        for _ in range(max(1, args.warmup)):
+            dg.pagerank(opts, sol)
+        t0 = time.perf_counter()
+        sts = [dg.pagerank(opts, sol) for _ in range(args.steps)]
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        st = sts[-1]
+        r = dg.read(smdp.RES_RANK, np.float32)
+        ctx.set_profiling(True)
+        pst = dg.pagerank(opts, sol)
+        ctx.set_profiling(False)
+    iters = int(st.rounds)
+    gteps = g.m * iters / (ms * 1e6)
+    per_iter = 8 * g.m + 20 * g.n
+    ach = per_iter * pst.dominant_launches / (pst.ms_dominant * 1e-3) / 1e9 if pst.ms_dominant > 0 else 0.0
+    out = {"gteps": round(gteps, 3), "ms_per_solve": round(ms, 4), "iterations": iters,
+           "residual": st.residual, "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
+                                                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                                                 "kernel": "k_pr_iter", "traffic": traffic_for("k_pr_iter"),
+                                                 "launches": pst.dominant_launches,
+                                                 "avg_launch_us": round(pst.ms_dominant * 1e3 / max(1, pst.dominant_launches), 3),
+                                                 "algorithmic_bytes_per_launch": per_iter}}
+    if not args.skip_check:
+        ref, _, _ = orc.pagerank(ig.row, ig.col, od, d=0.85, T=1.0, fixed_iters=iters)
+        out["l1_vs_fp64_iteration_matched"] = float(np.abs(r.astype(np.float64) - ref).sum())
+        out["linf_vs_fp64"] = float(np.abs(r.astype(np.float64) - ref).max())
+    return out
+
+
+def batched_section(ctx, args, smdp, orc, np):
+    out = {}
+    g = smdp.gen_rmat(args.batch_scale, 16, wlo=1, whi=255, seed=4)
+    srcs = smdp.gen_sources(g, 64, 4)
+    with smdp.Graph(ctx, g) as dg:
+        dg.sssp_multi_source(srcs)  # warm-up
+        t0 = time.perf_counter()
+        st = dg.sssp_multi_source(srcs)
+        t_batch = time.perf_counter() - t0
+        for s in srcs[:2]:
+            dg.sssp(int(s), smdp.SOL_TUNED)
+        t0 = time.perf_counter()
+        for s in srcs:
+            dg.sssp(int(s), smdp.SOL_TUNED)
+        t_plain = time.perf_counter() - t0
+    out["multi_source"] = {"graph": f"rmat-{args.batch_scale} ef16 w[1,255] seed 4", "sources": 64,
+                           "ms_batched": round(t_batch * 1e3, 3), "ms_64_plain": round(t_plain * 1e3, 3),
+                           "speedup": round(t_plain / t_batch, 2), "rounds": st.rounds,
+                           "edges_relaxed": st.edges_relaxed}
+    graphs = [smdp.gen_small_graph(i, seed=4) for i in range(256)]
+    srcs = np.zeros(256, dtype=np.int32)
+    with smdp.Batch(ctx, graphs, srcs) as b:
+        b.run()
+        t0 = time.perf_counter()
+        reps = 5
+        for _ in range(reps):
+            st = b.run()
+        t_batch = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for gi in graphs:
+        ctx.sssp(gi, 0, smdp.SOL_TUNED)
+    t_plain_upload = time.perf_counter() - t0
+    devs = [smdp.Graph(ctx, gi) for gi in graphs]
+    for d in devs[:4]:
+        d.sssp(0, smdp.SOL_TUNED)
+    t0 = time.perf_counter()
+    for d in devs:
+        d.sssp(0, smdp.SOL_TUNED)
+    t_plain = time.perf_counter() - t0
+    for d in devs:
+        d.close()
+    out["multi_graph"] = {"graphs": 256, "vertices": int(sum(x.n for x in graphs)),
+                          "arcs": int(sum(x.m for x in graphs)), "ms_batched": round(t_batch * 1e3, 3),
+                          "ms_256_plain_resident": round(t_plain * 1e3, 3),
+                          "ms_256_plain_with_upload": round(t_plain_upload * 1e3, 3),
+                          "speedup": round(t_plain / t_batch, 2)}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--batch-scale", type=int, default=21)
+    ap.add_argument("--no-tune", dest="tune", action="store_false")
+    ap.add_argument("--skip-check", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-pagerank", action="store_true")
+    ap.add_argument("--skip-batched", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} is not supported; this build measures one GPU "
+              f"(the multi-GPU sharding is design only).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import orc
+    import smdp
+
+    t0 = time.time()
+    g = smdp.gen_rmat(args.scale, 16, wlo=1, whi=255, seed=3)
+    log(f"graph: rmat-{args.scale} n={g.n} m={g.m} source={g.source} ({time.time() - t0:.1f}s)")
+    ctx = smdp.Context(0)
+    info_dev = ctx.info()
+    dg = smdp.Graph(ctx, g)
+    gteps, ms_step, roof, sssp_info, st = sssp_section(ctx, g, dg, args, smdp, orc, np)
+    log(f"sssp: {gteps:.3f} GTEPS, {ms_step:.3f} ms/solve, roofline {roof['frac']:.3f}")
+    dg.close()
+
+    line = {"metric": METRIC, "value": round(gteps, 4), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+            "data": f"synthetic: seeded RMAT scale {args.scale}, edge factor 16, a/b/c/d=.57/.19/.19/.05, "
+                    f"vertex ids permuted, weights U[1,255], seed 3",
+            "config": {"workload": f"sssp_rmat{args.scale}", "scale": args.scale, "edge_factor": 16,
+                       "vertices": g.n, "arcs": g.m, "solution": "tuned", "device": info_dev},
+            "roofline": roof, "sssp": sssp_info}
+    if not args.skip_cpu:
+        line["cpu_baseline"] = cpu_baseline(g, g.source, orc, np)
+        log(f"cpu baseline: {line['cpu_baseline']['value']} GTEPS")
+    if not args.skip_pagerank:
+        try:
+            line["pagerank"] = pagerank_section(ctx, g, args, smdp, orc, np)
+            log(f"pagerank: {line['pagerank']}")
+        except smdp.SmdpError as e:
+            line["pagerank"] = {"error": str(e)}
+    if not args.skip_batched:
+        try:
+            line["batched"] = batched_section(ctx, args, smdp, orc, np)
+            log(f"batched: {line['batched']}")
+        except smdp.SmdpError as e:
+            line["batched"] = {"error": str(e)}
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

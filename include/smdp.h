@@ -106,8 +106,13 @@ typedef struct {
   double   residual;           /* PageRank: last FP64 L1 residual */
   double   dangling;           /* PageRank: last dangling mass */
   uint64_t kernel_launches;    /* kernels launched by the call */
-  double   ms_dominant;        /* summed device time of the dominant kernel (when profiling is on) */
-  uint64_t dominant_launches;  /* launches of the dominant kernel (when profiling is on) */
+  double   ms_dominant;        /* summed device time of the phase-1 kernel (when profiling is on) */
+  uint64_t dominant_launches;  /* launches of the phase-1 kernel (when profiling is on) */
+  double   ms_phase2;          /* summed device time of the phase-2 (grid queue) kernel (profiling) */
+  uint64_t phase2_launches;    /* launches of the phase-2 kernel (profiling) */
+  uint64_t edges_phase2;       /* arcs of edges_relaxed scanned by the phase-2 kernel */
+  uint64_t atomics_tried;      /* SSSP/BFS: atomicMin issued after a plain read said "improves" */
+  uint64_t atomics_won;        /* SSSP/BFS: atomicMin that lowered the distance */
 } smdp_stats;
 
 /* PageRank options (the reference hard-codes them: DAMPING_FACTOR 0.5 and

@@ -62,12 +62,14 @@ int grid_size(smdp_ctx* ctx, const smdp_params& p) {
 void prof_reset(smdp_ctx* ctx) {
   for (auto e : ctx->prof_ev) (void)hipEventDestroy(e);
   ctx->prof_ev.clear();
+  ctx->prof_kind.clear();
 }
 
-int prof_mark(smdp_ctx* ctx) {
+int prof_mark(smdp_ctx* ctx, int kind) {
   if (!ctx->profiling) return SMDP_OK;
   hipEvent_t e;
   SMDP_HIP(hipEventCreate(&e));
+  if (ctx->prof_ev.size() % 2 == 0) ctx->prof_kind.push_back(kind);
   ctx->prof_ev.push_back(e);
   SMDP_HIP(hipEventRecord(e, ctx->stream));
   return SMDP_OK;
@@ -75,16 +77,19 @@ int prof_mark(smdp_ctx* ctx) {
 
 int prof_collect(smdp_ctx* ctx, smdp_stats* st) {
   if (!ctx->profiling || !st) return SMDP_OK;
-  double ms = 0;
-  uint64_t n = 0;
+  double ms[2] = {0, 0};
+  uint64_t n[2] = {0, 0};
   for (size_t i = 0; i + 1 < ctx->prof_ev.size(); i += 2) {
     float t = 0;
     SMDP_HIP(hipEventElapsedTime(&t, ctx->prof_ev[i], ctx->prof_ev[i + 1]));
-    ms += t;
-    ++n;
+    int k = ctx->prof_kind[i / 2] ? 1 : 0;
+    ms[k] += t;
+    ++n[k];
   }
-  st->ms_dominant = ms;
-  st->dominant_launches = n;
+  st->ms_dominant = ms[0];
+  st->dominant_launches = n[0];
+  st->ms_phase2 = ms[1];
+  st->phase2_launches = n[1];
   prof_reset(ctx);
   return SMDP_OK;
 }
@@ -100,6 +105,9 @@ int fill_stats(smdp_graph* g, smdp_stats* st, double ms_kernel, uint64_t launche
   for (int i = 0; i < 4; ++i) st->q_push[i] = c.push[i];
   st->rounds = c.rounds;
   st->m_reached = c.reached;
+  st->edges_phase2 = c.edges2;
+  st->atomics_tried = c.atom_try;
+  st->atomics_won = c.atom_win;
   st->kernel_launches = launches;
   return SMDP_OK;
 }
@@ -211,6 +219,12 @@ int smdp_set_profiling(smdp_ctx* ctx, int on) {
 int smdp_graph_create(smdp_ctx* ctx, const smdp_csr* h, smdp_graph** out) {
   if (!ctx || !h || !out || h->n < 0 || h->m < 0 || !h->row || (h->m > 0 && !h->col)) return SMDP_ERR_ARG;
   if (h->row[h->n] != h->m || h->row[0] != 0) return SMDP_ERR_ARG;
+  // Every kernel indexes row[u..u+1] and dist[col[e]] without bounds checks,
+  // so the CSR is validated here once.
+  for (int32_t v = 0; v < h->n; ++v)
+    if (h->row[v + 1] < h->row[v]) return SMDP_ERR_ARG;
+  for (int32_t e = 0; e < h->m; ++e)
+    if ((uint32_t)h->col[e] >= (uint32_t)h->n) return SMDP_ERR_ARG;
   *out = nullptr;
   SMDP_HIP(hipSetDevice(ctx->device));
   smdp_graph* g = new smdp_graph();

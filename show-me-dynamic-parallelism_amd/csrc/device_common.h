@@ -37,6 +37,52 @@ __device__ __forceinline__ int wave_append(int32_t* __restrict__ q, int32_t* cou
   return cnt;
 }
 
+// Per-wave output staging in LDS.  Items are appended with ballot + mbcnt
+// (no atomics) and written to the global queue in coalesced runs with one
+// global atomic per flush.  `n` lives in a register and must stay the same in
+// every lane, so push() and flush() must be reached by all 64 lanes together
+// (wave-uniform control flow); callers keep per-lane work predicated.
+template <int CAP>
+struct WaveOut {
+  int32_t* buf;  // this wave's CAP-entry LDS region
+  int n;
+
+  __device__ __forceinline__ void init(int32_t* region) {
+    buf = region;
+    n = 0;
+  }
+
+  __device__ __forceinline__ void flush(int32_t* __restrict__ q, int32_t* counter) {
+    if (n == 0) return;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint64_t act = ballot(true);
+    const int leader = first_lane(act);
+    const int me = mbcnt(act);
+    const int stride = __popcll(act);
+    int base = 0;
+    if (lane_id() == leader) base = atomicAdd(counter, n);
+    base = __shfl(base, leader);
+    for (int j = me; j < n; j += stride) q[base + j] = buf[j];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    n = 0;
+  }
+
+  // Returns the number of items the wave appended.
+  __device__ __forceinline__ int push(bool pred, int32_t item, int32_t* __restrict__ q, int32_t* counter) {
+    const uint64_t m = ballot(pred);
+    if (m == 0) return 0;
+    if (pred) buf[n + mbcnt(m)] = item;
+    const int c = __popcll(m);
+    n += c;
+    if (n > CAP - SMDP_WAVE) flush(q, counter);
+    return c;
+  }
+};
+
 // Inclusive prefix sum across the active wave (all 64 lanes must be active).
 __device__ __forceinline__ int wave_inclusive_scan(int x) {
   const int l = lane_id();

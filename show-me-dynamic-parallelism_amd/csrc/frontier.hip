@@ -33,6 +33,25 @@ namespace smdp {
 
 constexpr int kWaveQ = 256;   // per-wave LDS queue capacity (SMDP_SOL_WAVE)
 constexpr int kBlockQ = 1024; // per-workgroup LDS queue capacity (SMDP_SOL_BLOCK)
+constexpr int kOutCap = 512;  // per-wave LDS staging of next-queue entries
+
+// Next-queue output of one wave: LDS staging + the global queue it flushes to.
+struct Sink {
+  WaveOut<kOutCap> wo;
+  int32_t* q;
+  int32_t* qlen;
+  unsigned long long pushes;
+  unsigned long long tries;  // atomicMin attempts (plain read said "improves")
+  unsigned long long wins;   // atomicMin that lowered the value
+  __device__ __forceinline__ void init(int32_t* lds_region, int32_t* queue, int32_t* counter) {
+    wo.init(lds_region);
+    q = queue;
+    qlen = counter;
+    pushes = tries = wins = 0;
+  }
+  __device__ __forceinline__ void push(bool pred, int32_t v) { pushes += wo.push(pred, v, q, qlen); }
+  __device__ __forceinline__ void flush() { wo.flush(q, qlen); }
+};
 
 struct FArgs {
   const int32_t* __restrict__ row;
@@ -58,10 +77,12 @@ __device__ __forceinline__ int arc_weight(const FArgs& a, int e) {
 // dist[v] and is the first in this round to claim v for the next queue.
 // A plain read of dist[v] may be stale (larger) when another XCD lowered it;
 // the atomic decides, so a stale read only costs an extra atomic.
-__device__ __forceinline__ bool relax(const FArgs& a, int v, int alt, int label) {
+__device__ __forceinline__ bool relax(const FArgs& a, int v, int alt, int label, Sink& out) {
   if (alt >= a.dist[v]) return false;
+  ++out.tries;
   int old = atomicMin(&a.dist[v], alt);
   if (alt >= old) return false;
+  ++out.wins;
   if (a.stamp[v] == label) return false;
   return atomicExch(&a.stamp[v], label) != label;
 }
@@ -72,7 +93,14 @@ __device__ __forceinline__ void clear_slot(Ctl* ctl, int r) {
 
 __device__ __forceinline__ void flush_counters(Counters* cnt, unsigned long long edges, unsigned long long verts,
                                                unsigned long long p0, unsigned long long p1,
-                                               unsigned long long p2, unsigned long long p3) {
+                                               unsigned long long p2, unsigned long long p3,
+                                               Sink& out) {
+  out.flush();
+  unsigned long long tries = wave_sum(out.tries), wins = wave_sum(out.wins);
+  if (lane_id() == 0) {
+    if (tries) atomicAdd(&cnt->atom_try, tries);
+    if (wins) atomicAdd(&cnt->atom_win, wins);
+  }
   edges = wave_sum(edges);
   verts = wave_sum(verts);
   p0 = wave_sum(p0);
@@ -91,9 +119,8 @@ __device__ __forceinline__ void flush_counters(Counters* cnt, unsigned long long
 
 // Wave-cooperative expansion of arcs [s, e) of one vertex with cost du.
 template <bool BFS>
-__device__ __forceinline__ void wave_expand(const FArgs& a, int s, int e, int du, int32_t* qout,
-                                            int32_t* qlen, int label, unsigned long long& edges,
-                                            unsigned long long& pushes) {
+__device__ __forceinline__ void wave_expand(const FArgs& a, int s, int e, int du, int label,
+                                            unsigned long long& edges, Sink& out) {
   const int lane = lane_id();
   for (int b = s; b < e; b += SMDP_WAVE) {
     int k = b + lane;
@@ -101,28 +128,27 @@ __device__ __forceinline__ void wave_expand(const FArgs& a, int s, int e, int du
     bool p = false;
     if (k < e) {
       v = a.col[k];
-      p = relax(a, v, du + arc_weight<BFS>(a, k), label);
+      p = relax(a, v, du + arc_weight<BFS>(a, k), label, out);
       ++edges;
     }
-    pushes += wave_append(qout, qlen, v, p);
+    out.push(p, v);
   }
 }
 
 // Workgroup-cooperative expansion (all SMDP_WG threads call it).
 template <bool BFS>
-__device__ __forceinline__ void wg_expand(const FArgs& a, int s, int e, int du, int32_t* qout,
-                                          int32_t* qlen, int label, unsigned long long& edges,
-                                          unsigned long long& pushes) {
+__device__ __forceinline__ void wg_expand(const FArgs& a, int s, int e, int du, int label,
+                                          unsigned long long& edges, Sink& out) {
   for (int b = s; b < e; b += SMDP_WG) {
     int k = b + (int)threadIdx.x;
     int v = 0;
     bool p = false;
     if (k < e) {
       v = a.col[k];
-      p = relax(a, v, du + arc_weight<BFS>(a, k), label);
+      p = relax(a, v, du + arc_weight<BFS>(a, k), label, out);
       ++edges;
     }
-    pushes += wave_append(qout, qlen, v, p);
+    out.push(p, v);
   }
 }
 
@@ -163,13 +189,16 @@ __global__ __launch_bounds__(SMDP_WG) void k_expand_lane(FArgs a, int r) {
   constexpr int kLdsQ = SOL == SMDP_SOL_BLOCK ? kBlockQ : (SOL == SMDP_SOL_WAVE ? SMDP_WAVES_PER_WG * kWaveQ : 1);
   __shared__ int32_t s_q[kLdsQ];
   __shared__ int32_t s_qn;
+  __shared__ int32_t s_out[SMDP_WAVES_PER_WG * kOutCap];
+  Sink out;
+  out.init(s_out + wave * kOutCap, qout, &cout->qlen);
   int wq_n = 0;  // SOL_WAVE: entries in this wave's LDS queue (wave-uniform)
   if (SOL == SMDP_SOL_BLOCK) {
     if (threadIdx.x == 0) s_qn = 0;
     __syncthreads();
   }
 
-  unsigned long long edges = 0, verts = 0, pf = 0, pw = 0, pb = 0, pg = 0;
+  unsigned long long edges = 0, verts = 0, pw = 0, pb = 0, pg = 0;
   const int stride = gridDim.x * SMDP_WG;
   for (int base = blockIdx.x * SMDP_WG; base < L; base += stride) {
     const int i = base + (int)threadIdx.x;
@@ -192,16 +221,25 @@ __global__ __launch_bounds__(SMDP_WG) void k_expand_lane(FArgs a, int r) {
         int l = first_lane(hm);
         hm &= hm - 1;
         ++pw;
-        wave_expand<BFS>(a, __shfl(s, l), __shfl(e, l), __shfl(du, l), qout, &cout->qlen, label, edges, pf);
+        wave_expand<BFS>(a, __shfl(s, l), __shfl(e, l), __shfl(du, l), label, edges, out);
       }
     }
 
-    if (!heavy) {
-      for (int k = s; k < e; ++k) {
-        int v = a.col[k];
-        bool p = relax(a, v, du + arc_weight<BFS>(a, k), label);
-        ++edges;
-        pf += wave_append(qout, &cout->qlen, v, p);
+    {
+      // Light vertices, one per lane.  The loop runs to the wave's largest
+      // light degree so every lane reaches the wave-wide push together.
+      const int deg = heavy ? 0 : e - s;
+      const int iters = wave_max(deg);
+      for (int j = 0; j < iters; ++j) {
+        int v = 0;
+        bool p = false;
+        if (j < deg) {
+          const int k = s + j;
+          v = a.col[k];
+          p = relax(a, v, du + arc_weight<BFS>(a, k), label, out);
+          ++edges;
+        }
+        out.push(p, v);
       }
     }
 
@@ -218,7 +256,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_expand_lane(FArgs a, int r) {
       if (wq_n > kWaveQ - SMDP_WAVE || (last && wq_n > 0)) {
         for (int j = 0; j < wq_n; ++j) {
           int hu = s_q[wave * kWaveQ + j];
-          wave_expand<BFS>(a, a.row[hu], a.row[hu + 1], a.dist[hu], qout, &cout->qlen, label, edges, pf);
+          wave_expand<BFS>(a, a.row[hu], a.row[hu + 1], a.dist[hu], label, edges, out);
         }
         wq_n = 0;
       }
@@ -241,7 +279,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_expand_lane(FArgs a, int r) {
       if (qn > kBlockQ - SMDP_WG || (last && qn > 0)) {
         for (int j = 0; j < qn; ++j) {
           int hu = s_q[j];
-          wg_expand<BFS>(a, a.row[hu], a.row[hu + 1], a.dist[hu], qout, &cout->qlen, label, edges, pf);
+          wg_expand<BFS>(a, a.row[hu], a.row[hu + 1], a.dist[hu], label, edges, out);
         }
         __syncthreads();
         if (threadIdx.x == 0) s_qn = 0;
@@ -261,7 +299,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_expand_lane(FArgs a, int r) {
       }
     }
   }
-  flush_counters(a.cnt, edges, verts, pf, pw, pb, pg);
+  flush_counters(a.cnt, edges, verts, out.pushes, pw, pb, pg, out);
 }
 
 // One round of the tuned solution: the workgroup takes tiles of SMDP_WG queue
@@ -287,8 +325,11 @@ __global__ __launch_bounds__(SMDP_WG) void k_expand_tile(FArgs a, int r) {
   __shared__ int s_du[SMDP_WG];
   __shared__ int s_off[SMDP_WG];
   __shared__ int s_scan[SMDP_WAVES_PER_WG];
+  __shared__ int32_t s_out[SMDP_WAVES_PER_WG * kOutCap];
+  Sink out;
+  out.init(s_out + (threadIdx.x / SMDP_WAVE) * kOutCap, qout, &cout->qlen);
 
-  unsigned long long edges = 0, verts = 0, pf = 0, pg = 0;
+  unsigned long long edges = 0, verts = 0, pg = 0;
   for (;;) {
     if (threadIdx.x == 0) s_tile = atomicAdd(&cin->tile_next, 1);
     __syncthreads();
@@ -336,14 +377,14 @@ __global__ __launch_bounds__(SMDP_WG) void k_expand_tile(FArgs a, int r) {
         }
         const int k = s_start[lo] + (j - s_off[lo]);
         v = a.col[k];
-        p = relax(a, v, s_du[lo] + arc_weight<BFS>(a, k), label);
+        p = relax(a, v, s_du[lo] + arc_weight<BFS>(a, k), label, out);
         ++edges;
       }
-      pf += wave_append(qout, &cout->qlen, v, p);
+      out.push(p, v);
     }
     __syncthreads();
   }
-  flush_counters(a.cnt, edges, verts, pf, 0, 0, pg);
+  flush_counters(a.cnt, edges, verts, out.pushes, 0, 0, pg, out);
 }
 
 // Phase 2 of the grid solutions: workgroups take grid-queue entries from a
@@ -357,7 +398,10 @@ __global__ __launch_bounds__(SMDP_WG) void k_drain(FArgs a, int r) {
   int32_t* qout = (r & 1) ? a.q0 : a.q1;
   const int label = r + 1;
   __shared__ int s_idx;
-  unsigned long long edges = 0, pf = 0;
+  __shared__ int32_t s_out[SMDP_WAVES_PER_WG * kOutCap];
+  Sink out;
+  out.init(s_out + (threadIdx.x / SMDP_WAVE) * kOutCap, qout, &cout->qlen);
+  unsigned long long edges = 0;
   for (;;) {
     if (threadIdx.x == 0) s_idx = atomicAdd(&cin->chunk_next, 1);
     __syncthreads();
@@ -367,9 +411,11 @@ __global__ __launch_bounds__(SMDP_WG) void k_drain(FArgs a, int r) {
     const Chunk c = a.hq[idx];
     const int vend = a.row[c.v + 1];
     const int end = (a.chunk > 0 && vend - c.first > a.chunk) ? c.first + a.chunk : vend;
-    wg_expand<BFS>(a, c.first, end, a.dist[c.v], qout, &cout->qlen, label, edges, pf);
+    wg_expand<BFS>(a, c.first, end, a.dist[c.v], label, edges, out);
   }
-  flush_counters(a.cnt, edges, 0, pf, 0, 0, 0);
+  flush_counters(a.cnt, edges, 0, out.pushes, 0, 0, 0, out);
+  unsigned long long e2 = wave_sum(edges);
+  if (lane_id() == 0 && e2) atomicAdd(&a.cnt->edges2, e2);
 }
 
 // Out-arcs of reached vertices (the GTEPS numerator).
@@ -456,13 +502,15 @@ int run_frontier(smdp_graph* g, int32_t src, int sol, bool bfs, smdp_stats* st) 
   int batch = std::max(1, p.batch_rounds);
   for (;;) {
     for (int j = 0; j < batch; ++j, ++r) {
-      SMDP_CHECK(prof_mark(ctx));
+      SMDP_CHECK(prof_mark(ctx, 0));
       if (bfs) launch_round<true>(sol, grid, s, a, r, &launches);
       else launch_round<false>(sol, grid, s, a, r, &launches);
-      SMDP_CHECK(prof_mark(ctx));
+      SMDP_CHECK(prof_mark(ctx, 0));
       if (two_phase) {
+        SMDP_CHECK(prof_mark(ctx, 1));
         if (bfs) launch_drain<true>(grid, s, a, r, &launches);
         else launch_drain<false>(grid, s, a, r, &launches);
+        SMDP_CHECK(prof_mark(ctx, 1));
       }
     }
     SMDP_HIP(hipGetLastError());

@@ -31,7 +31,10 @@ struct Counters {
   unsigned long long rounds;   // max round index + 1 that saw work
   unsigned long long reached;  // out-arcs of reached vertices
   unsigned long long done;     // PageRank: iteration at which the stop rule fired (0 = not yet)
-  unsigned long long pad[7];
+  unsigned long long edges2;   // arcs scanned by phase-2 (grid queue) kernels
+  unsigned long long atom_try; // atomicMin attempts after the plain read said "improves"
+  unsigned long long atom_win; // atomicMin calls that lowered the value
+  unsigned long long pad[4];
 };
 
 // Entry of the grid queue: vertex/row and the first edge of its chunk.
@@ -51,7 +54,8 @@ struct smdp_ctx {
   smdp_params params[SMDP_APP_COUNT];
   int profiling = 0;
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
-  std::vector<hipEvent_t> prof_ev;  // pairs around dominant-kernel launches
+  std::vector<hipEvent_t> prof_ev;  // pairs of events around profiled launches
+  std::vector<int> prof_kind;       // kernel kind of each pair
   int32_t* pinned = nullptr;        // small host readback buffer (hipHostMalloc)
 };
 
@@ -143,7 +147,9 @@ double now_ms();
 
 // Profiling helpers around the dominant kernel of a call.
 void prof_reset(smdp_ctx* ctx);
-int prof_mark(smdp_ctx* ctx);  // records one event (call before and after each launch)
+// Records one event; call before and after each launch of kernel `kind`
+// (0: phase-1 / dominant kernel, 1: phase-2 kernel).
+int prof_mark(smdp_ctx* ctx, int kind);
 int prof_collect(smdp_ctx* ctx, smdp_stats* st);
 
 // Copies counters into st and finishes timings.

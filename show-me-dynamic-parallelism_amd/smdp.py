@@ -50,7 +50,8 @@ class Stats(C.Structure):
                 ("vertices_expanded", C.c_uint64), ("rounds", C.c_uint64), ("q_push", C.c_uint64 * 4),
                 ("m_reached", C.c_uint64), ("residual", C.c_double), ("dangling", C.c_double),
                 ("kernel_launches", C.c_uint64), ("ms_dominant", C.c_double),
-                ("dominant_launches", C.c_uint64)]
+                ("dominant_launches", C.c_uint64), ("ms_phase2", C.c_double), ("phase2_launches", C.c_uint64),
+                ("edges_phase2", C.c_uint64), ("atomics_tried", C.c_uint64), ("atomics_won", C.c_uint64)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
