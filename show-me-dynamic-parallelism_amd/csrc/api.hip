@@ -261,6 +261,8 @@ void smdp_graph_destroy(smdp_graph* g) {
                   g->rmax,   g->mdist};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
+  rowbufs_free(g);
+  colorbufs_free(g);
   delete g;
 }
 

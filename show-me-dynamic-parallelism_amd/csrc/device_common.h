@@ -128,8 +128,11 @@ __device__ __forceinline__ int wg_exclusive_scan(int x, int* lds, int* total) {
 }
 
 // ---- compensated FP32 arithmetic --------------------------------------------
+// These error-free transformations need every operation rounded on its own:
+// contraction into fma (HIP's default -ffp-contract=fast) would break them.
 // TwoSum (Knuth): s + e == a + b exactly.
 __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+#pragma clang fp contract(off)
   s = a + b;
   float bb = s - a;
   e = (a - (s - bb)) + (b - bb);
@@ -142,6 +145,7 @@ struct CSum {
   float s, c;
   __device__ __forceinline__ CSum() : s(0.f), c(0.f) {}
   __device__ __forceinline__ void add(float x) {
+#pragma clang fp contract(off)
     float t, e;
     two_sum(s, x, t, e);
     s = t;
@@ -149,16 +153,21 @@ struct CSum {
   }
   // Adds the exact product a*b: p = fl(a*b), fma(a,b,-p) is its rounding error.
   __device__ __forceinline__ void add_prod(float a, float b) {
+#pragma clang fp contract(off)
     float p = a * b;
     float pe = __builtin_fmaf(a, b, -p);
     add(p);
     c += pe;
   }
   __device__ __forceinline__ void merge(const CSum& o) {
+#pragma clang fp contract(off)
     add(o.s);
     c += o.c;
   }
-  __device__ __forceinline__ float value() const { return s + c; }
+  __device__ __forceinline__ float value() const {
+#pragma clang fp contract(off)
+    return s + c;
+  }
 };
 
 __device__ __forceinline__ CSum wave_csum(CSum x) {

@@ -95,6 +95,8 @@ struct smdp_graph {
   int32_t msrc_k = 0;
   int32_t msrc_groups = 0;
   int last_res = -1;
+  void* rowbufs = nullptr;  // row-reduce family buffers (rowreduce.hip)
+  void* colorbufs = nullptr; // coloring buffers (color.hip)
 };
 
 namespace smdp {
@@ -151,6 +153,9 @@ void prof_reset(smdp_ctx* ctx);
 // (0: phase-1 / dominant kernel, 1: phase-2 kernel).
 int prof_mark(smdp_ctx* ctx, int kind);
 int prof_collect(smdp_ctx* ctx, smdp_stats* st);
+
+void rowbufs_free(smdp_graph* g);
+void colorbufs_free(smdp_graph* g);
 
 // Copies counters into st and finishes timings.
 int fill_stats(smdp_graph* g, smdp_stats* st, double ms_kernel, uint64_t launches);

@@ -1,0 +1,159 @@
+"""SpMV and PageRank on the device against the FP64 oracle, every solution.
+
+SpMV is FP32 with compensated (Dot2) accumulation.  Tolerance per row,
+written here once: |y - y64| <= 2u|y64| + 2 (n_r u)^2 S_r with u = 2^-24,
+n_r the row length and S_r = sum |val*x| over the row (the Dot2 error bound
+of Ogita, Rump and Oishi with a factor 2 of margin).  With the reference's own
+inputs (data[i] = i%13+0.5, x[i] = i%17+0.5) every partial sum is a multiple
+of 0.25 below 2^22 here, so y must be exact.
+
+PageRank runs FP32 ranks with compensated row sums; its L1 error against an
+iteration-matched FP64 oracle (same iteration count) must be <= 1e-6 * T, and
+the FP64 residual stop rule must fire as on the device.
+"""
+import numpy as np
+import pytest
+
+import orc
+import smdp
+from conftest import random_graph
+
+pytestmark = pytest.mark.gpu
+
+SOLS = [smdp.SOL_FLAT, smdp.SOL_NAIVE, smdp.SOL_WAVE, smdp.SOL_BLOCK, smdp.SOL_GRID, smdp.SOL_TUNED]
+U = 2.0 ** -24
+
+
+def _mats():
+    out = {}
+    out["rmat12"] = smdp.gen_rmat(12, 16, seed=2)
+    out["rmat10_noperm"] = smdp.gen_rmat(10, 32, permute=False, seed=7)
+    row, col, _ = random_graph(5000, 30000, 3)
+    out["random"] = smdp.HostGraph(row, col)
+    out["uniform"] = smdp.gen_uniform(8192, 8, 1, 1, 5)
+    return out
+
+
+MATS = _mats()
+
+
+def _spmv_bound(g, val, x, y64):
+    prod = np.abs(val.astype(np.float64) * x.astype(np.float64)[g.col])
+    S = np.zeros(g.n)
+    np.add.at(S, np.repeat(np.arange(g.n), np.diff(g.row)), prod)
+    nr = np.diff(g.row)
+    return 2 * U * np.abs(y64) + 2 * (nr * U) ** 2 * S
+
+
+@pytest.mark.parametrize("name", sorted(MATS))
+@pytest.mark.parametrize("sol", SOLS)
+def test_spmv_parity(ctx, name, sol):
+    g = MATS[name]
+    print(f"spmv {name} sol={sol}", flush=True)
+    val = smdp.gen_uniform_f32(g.m, -1.0, 1.0, 2, 6)
+    x = smdp.gen_uniform_f32(g.n, 0.0, 1.0, 2, 9)
+    y, st = ctx.spmv(g, val, x, sol)
+    y64 = orc.spmv(g.row, g.col, val, x)
+    err = np.abs(y.astype(np.float64) - y64)
+    assert np.all(err <= _spmv_bound(g, val, x, y64)), float(err.max())
+    assert st.edges_relaxed == g.m
+
+
+@pytest.mark.parametrize("sol", SOLS)
+def test_spmv_reference_inputs_exact(ctx, sol):
+    g = MATS["rmat12"]
+    data, x = smdp.gen_spmv_reference_inputs(g.n, g.m)
+    y, _ = ctx.spmv(g, data, x, sol)
+    four_y = np.zeros(g.n, dtype=np.int64)
+    terms = (2 * (np.arange(g.m) % 13) + 1) * (2 * (g.col % 17) + 1)
+    src = np.repeat(np.arange(g.n), np.diff(g.row))
+    np.add.at(four_y, src, terms)
+    assert np.all(four_y < 4 * 2 ** 22)
+    assert np.array_equal(y.astype(np.float64) * 4, four_y.astype(np.float64))
+
+
+def test_spmv_tiny_thresholds(ctx):
+    g = MATS["rmat12"]
+    val = smdp.gen_uniform_f32(g.m, -1.0, 1.0, 3, 6)
+    x = smdp.gen_uniform_f32(g.n, 0.0, 1.0, 3, 9)
+    y64 = orc.spmv(g.row, g.col, val, x)
+    old = ctx.get_params(smdp.APP_SPMV)
+    try:
+        ctx.set_params(smdp.APP_SPMV, heavy_threshold=2, split_threshold=64, chunk=64, wg_per_cu=1)
+        for sol in SOLS:
+            y, _ = ctx.spmv(g, val, x, sol)
+            assert np.all(np.abs(y - y64) <= _spmv_bound(g, val, x, y64)), sol
+    finally:
+        ctx.set_params(smdp.APP_SPMV, **old.as_dict())
+
+
+def test_spmv_edge_cases(ctx):
+    for g in [smdp.HostGraph([0, 0], []), smdp.HostGraph([0, 0, 3, 3], [0, 2, 2]),
+              smdp.HostGraph([0, 1], [0])]:
+        val = np.linspace(-1, 1, max(g.m, 1)).astype(np.float32)[:g.m]
+        x = np.linspace(0.1, 1, g.n).astype(np.float32)
+        for sol in SOLS:
+            y, _ = ctx.spmv(g, val, x, sol)
+            assert np.allclose(y, orc.spmv(g.row, g.col, val, x), rtol=1e-6, atol=1e-7)
+
+
+def _pr_graph(name):
+    g = MATS[name]
+    ig = smdp.invert(g)
+    return g, ig, g.outdegree()
+
+
+@pytest.mark.parametrize("name", ["rmat12", "random", "rmat10_noperm"])
+@pytest.mark.parametrize("sol", SOLS)
+def test_pagerank_parity(ctx, name, sol):
+    g, ig, od = _pr_graph(name)
+    print(f"pagerank {name} sol={sol}", flush=True)
+    opts = smdp.pr_opts(0.85, 1.0, 1e-6, 100)
+    r, st = ctx.pagerank(ig, od, opts, sol)
+    it = int(st.rounds)
+    ref, it_ref, res_ref = orc.pagerank(ig.row, ig.col, od, d=0.85, T=1.0, fixed_iters=it)
+    l1 = np.abs(r.astype(np.float64) - ref).sum()
+    assert l1 <= 1e-6, l1
+    assert st.residual < 1e-6 or it == 100
+    # the FP64 oracle with the same stop rule stops within one iteration
+    _, it_conv, _ = orc.pagerank(ig.row, ig.col, od, d=0.85, T=1.0, tol=1e-6, max_iter=100)
+    assert abs(it_conv - it) <= 1
+    assert abs(r.astype(np.float64).sum() - 1.0) < 1e-5
+
+
+@pytest.mark.parametrize("sol", SOLS)
+def test_pagerank_reference_parameters(ctx, sol):
+    # damping 0.5, total rank 1000, ranks starting at 0 (pagerank_wrapper.cu:7,115; pagerank.h:15)
+    g, ig, od = _pr_graph("random")
+    opts = smdp.pr_opts(0.5, 1000.0, 1e-6, 100, init_zero=True)
+    r, st = ctx.pagerank(ig, od, opts, sol)
+    ref, _, _ = orc.pagerank(ig.row, ig.col, od, d=0.5, T=1000.0, init_zero=True, fixed_iters=int(st.rounds))
+    assert np.abs(r.astype(np.float64) - ref).sum() <= 1e-6 * 1000.0
+
+
+def test_pagerank_deterministic_and_tiny_thresholds(ctx):
+    g, ig, od = _pr_graph("rmat12")
+    opts = smdp.pr_opts(0.85, 1.0, 1e-6, 100)
+    a, _ = ctx.pagerank(ig, od, opts, smdp.SOL_TUNED)
+    b, _ = ctx.pagerank(ig, od, opts, smdp.SOL_TUNED)
+    assert np.array_equal(a, b)
+    old = ctx.get_params(smdp.APP_PAGERANK)
+    try:
+        ctx.set_params(smdp.APP_PAGERANK, heavy_threshold=4, split_threshold=64, chunk=64, wg_per_cu=1)
+        for sol in SOLS:
+            r, st = ctx.pagerank(ig, od, opts, sol)
+            ref, _, _ = orc.pagerank(ig.row, ig.col, od, fixed_iters=int(st.rounds))
+            assert np.abs(r.astype(np.float64) - ref).sum() <= 1e-6, sol
+    finally:
+        ctx.set_params(smdp.APP_PAGERANK, **old.as_dict())
+
+
+def test_pagerank_max_iter_and_resident(ctx):
+    g, ig, od = _pr_graph("random")
+    with smdp.Graph(ctx, smdp.HostGraph(ig.row, ig.col)) as dg:
+        dg.set_outdeg(od)
+        st = dg.pagerank(smdp.pr_opts(0.85, 1.0, 0.0, 7), smdp.SOL_TUNED)
+        assert st.rounds == 7
+        r = dg.read(smdp.RES_RANK, np.float32)
+        ref, _, _ = orc.pagerank(ig.row, ig.col, od, fixed_iters=7)
+        assert np.abs(r - ref).sum() <= 1e-6
