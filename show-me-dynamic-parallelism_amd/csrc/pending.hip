@@ -1,13 +1,8 @@
 // Entry points whose kernels are not written yet.
 #include "smdp_internal.h"
-namespace smdp {
-void colorbufs_free(smdp_graph*) {}
-}
 extern "C" {
-int smdp_color(smdp_ctx*, const smdp_csr*, const uint32_t*, int, int32_t*, smdp_stats*) { return SMDP_ERR_SOLUTION; }
 int smdp_sssp_multi_source(smdp_ctx*, const smdp_csr*, const int32_t*, int32_t, int32_t*, smdp_stats*) { return SMDP_ERR_SOLUTION; }
 int smdp_sssp_multi_graph(smdp_ctx*, const smdp_csr*, int32_t, const int32_t*, int32_t* const*, smdp_stats*) { return SMDP_ERR_SOLUTION; }
-int smdp_color_run(smdp_graph*, int, int, smdp_stats*) { return SMDP_ERR_SOLUTION; }
 int smdp_sssp_multi_source_run(smdp_graph*, const int32_t*, int32_t, smdp_stats*) { return SMDP_ERR_SOLUTION; }
 int smdp_batch_create(smdp_ctx*, const smdp_csr*, int32_t, const int32_t*, smdp_batch**) { return SMDP_ERR_SOLUTION; }
 int smdp_batch_run(smdp_batch*, smdp_stats*) { return SMDP_ERR_SOLUTION; }

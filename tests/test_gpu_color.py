@@ -1,0 +1,84 @@
+"""Colouring on the device against the oracle's closed form, every solution.
+
+Colours are integers and must match bit for bit.  The closed form is itself
+pinned against a round-by-round restatement of the reference kernel in
+tests/test_oracle.py.
+"""
+import numpy as np
+import pytest
+
+import orc
+import smdp
+from conftest import random_graph, symmetrize
+
+pytestmark = pytest.mark.gpu
+
+SOLS = [smdp.SOL_FLAT, smdp.SOL_NAIVE, smdp.SOL_WAVE, smdp.SOL_BLOCK, smdp.SOL_GRID, smdp.SOL_TUNED]
+
+
+def _graphs():
+    out = {}
+    out["rmat12_sym"] = smdp.gen_rmat(12, 8, symmetrize=True, seed=5)
+    out["rmat11_sym_noperm"] = smdp.gen_rmat(11, 16, symmetrize=True, permute=False, seed=5)
+    row, col, _ = random_graph(4000, 20000, 8)
+    out["random_directed"] = smdp.HostGraph(row, col)
+    out["uniform"] = smdp.gen_uniform(8192, 8, 1, 1, 3)
+    return out
+
+
+GRAPHS = _graphs()
+
+
+@pytest.mark.parametrize("name", sorted(GRAPHS))
+@pytest.mark.parametrize("use_prio", [False, True])
+@pytest.mark.parametrize("sol", SOLS)
+def test_color_parity(ctx, name, use_prio, sol):
+    g = GRAPHS[name]
+    print(f"color {name} prio={use_prio} sol={sol}", flush=True)
+    prio = smdp.gen_prio(g.n, 5) if use_prio else None
+    ref, k = orc.color_closed(g.row, g.col, prio)
+    c, st = ctx.color(g, prio, sol)
+    assert np.array_equal(c, ref)
+    assert st.rounds == k
+
+
+@pytest.mark.parametrize("sol", SOLS)
+def test_color_known_answer(ctx, golden, data_test_path, sol):
+    g = smdp.read_graph(data_test_path, smdp.FMT_DIMACS9)
+    c, st = ctx.color(g, None, sol)
+    assert c.tolist() == golden["color"]
+
+
+def test_color_proper_and_ties(ctx):
+    g = GRAPHS["rmat12_sym"]
+    prio = (smdp.gen_prio(g.n, 7) % 3).astype(np.uint32)  # heavy ties: broken by id
+    for sol in SOLS:
+        c, _ = ctx.color(g, prio, sol)
+        assert np.array_equal(c, orc.color_closed(g.row, g.col, prio)[0])
+        src = np.repeat(np.arange(g.n), np.diff(g.row))
+        loop = src == g.col
+        assert not np.any(c[src[~loop]] == c[g.col[~loop]])
+
+
+def test_color_tiny_thresholds(ctx):
+    g = GRAPHS["rmat11_sym_noperm"]
+    prio = smdp.gen_prio(g.n, 9)
+    ref, _ = orc.color_closed(g.row, g.col, prio)
+    old = ctx.get_params(smdp.APP_COLOR)
+    try:
+        ctx.set_params(smdp.APP_COLOR, heavy_threshold=2, split_threshold=8, wg_per_cu=1)
+        for sol in SOLS:
+            c, st = ctx.color(g, prio, sol)
+            assert np.array_equal(c, ref), sol
+    finally:
+        ctx.set_params(smdp.APP_COLOR, **old.as_dict())
+
+
+def test_color_edge_cases(ctx):
+    cases = [smdp.HostGraph([0, 0], []), smdp.HostGraph([0, 1, 2], [0, 1]),
+             smdp.HostGraph([0, 0, 1, 2, 3], [0, 1, 2])]
+    for g in cases:
+        ref, _ = orc.color_closed(g.row, g.col)
+        for sol in SOLS:
+            c, _ = ctx.color(g, None, sol)
+            assert np.array_equal(c, ref)
