@@ -1,0 +1,700 @@
+// Batched SSSP on gfx950 (no counterpart in the reference, which solves one
+// source per process run: SSSP_GPU, apps/sssp/src/sssp_wrapper.cu:304-341).
+//
+// Many sources on one graph: distances are laid out [vertex][64] so the 64
+// lanes of a wave hold the 64 sources of one vertex.  A frontier vertex is
+// expanded by one wave: for each arc (u,v,w) the wave reads the 256 bytes of
+// dist[v][*] in one coalesced access and relaxes all sources at once, so one
+// read of the CSR serves every source.  The frontier is the union over
+// sources; a vertex is queued once per round whatever source improved it.
+// Vertices with at least `split` arcs are cut into chunks of `chunk` arcs that
+// any wave can take in a second kernel.  More than 64 sources run as groups of
+// 64.
+//
+// Many small graphs: one workgroup solves one whole graph, every round inside
+// the kernel with workgroup barriers between rounds (no grid-wide
+// synchronisation, no host round trip).  Workgroups take graphs from a
+// counter, largest first.  The launch is captured once in a hipGraph and
+// replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "device_common.h"
+#include "smdp_internal.h"
+
+namespace smdp {
+
+constexpr int kMsOut = 512;
+constexpr int kLanes = SMDP_WAVE;  // sources per group
+
+struct MArgs {
+  const int32_t* __restrict__ row;
+  const int32_t* __restrict__ col;
+  const int32_t* __restrict__ w;
+  int32_t* mdist;  // [n][64]
+  int32_t* stamp;
+  int32_t* q0;
+  int32_t* q1;
+  Chunk* hq;
+  Ctl* ctl;
+  Counters* cnt;
+  int32_t split;
+  int32_t chunk;
+};
+
+// Relax arcs [s, e) of u for all 64 sources held by the lanes.  Wave-uniform.
+__device__ __forceinline__ void ms_arcs(const MArgs& a, int s, int e, int du, int label, WaveOut<kMsOut>& out,
+                                        int32_t* qout, int32_t* qlen, unsigned long long& edges,
+                                        unsigned long long& wins) {
+  const int lane = lane_id();
+  for (int b = s; b < e; b += SMDP_WAVE) {
+    // one coalesced load of up to 64 arcs, then broadcast one at a time
+    const int k = b + lane;
+    const int cv = k < e ? a.col[k] : 0;
+    const int cw = k < e ? a.w[k] : 0;
+    const int cnt = min(SMDP_WAVE, e - b);
+    for (int j0 = 0; j0 < cnt; j0 += 4) {
+      int v[4], dv[4], alt[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int j = min(j0 + t, cnt - 1);
+        v[t] = __builtin_amdgcn_readlane(cv, j);
+        alt[t] = du + __builtin_amdgcn_readlane(cw, j);
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) dv[t] = a.mdist[(int64_t)v[t] * kLanes + lane];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        bool imp = false;
+        if (j0 + t < cnt && du < SMDP_SSSP_INF && alt[t] < dv[t]) {
+          const int old = atomicMin(&a.mdist[(int64_t)v[t] * kLanes + lane], alt[t]);
+          imp = alt[t] < old;
+        }
+        const uint64_t m = ballot(imp);
+        bool push = false;
+        if (m) {
+          wins += lane == 0 ? __popcll(m) : 0;
+          if (lane == 0 && a.stamp[v[t]] != label) push = atomicExch(&a.stamp[v[t]], label) != label;
+        }
+        out.push(push, v[t], qout, qlen);
+      }
+    }
+    edges += lane == 0 ? cnt : 0;
+  }
+}
+
+__global__ void k_ms_init(int32_t* mdist, int32_t* stamp, int32_t n, const int32_t* __restrict__ srcs, int32_t k,
+                          int32_t* q0, Ctl* ctl, Counters* cnt) {
+  const int64_t total = (int64_t)n * kLanes;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x)
+    mdist[i] = SMDP_SSSP_INF;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    stamp[i] = -1;
+  if (blockIdx.x == 0) {
+    int32_t* c = reinterpret_cast<int32_t*>(ctl);
+    for (int i = threadIdx.x; i < 4 * 32; i += blockDim.x) c[i] = 0;
+    unsigned long long* p = reinterpret_cast<unsigned long long*>(cnt);
+    for (int i = threadIdx.x; i < (int)(sizeof(Counters) / 8); i += blockDim.x) p[i] = 0;
+  }
+}
+
+// Second step of the init: sources into the distance table and queue 0.
+// Several sources may share a vertex; only the first entry queues it.
+__global__ void k_ms_seed(int32_t* mdist, int32_t* stamp, const int32_t* __restrict__ srcs, int32_t k, int32_t* q0,
+                          Ctl* ctl) {
+  if (threadIdx.x < k) {
+    const int s = srcs[threadIdx.x];
+    mdist[(int64_t)s * kLanes + threadIdx.x] = 0;
+    if (atomicExch(&stamp[s], 0) != 0) q0[atomicAdd(&ctl[0].qlen, 1)] = s;
+  }
+}
+
+__global__ __launch_bounds__(SMDP_WG) void k_ms_expand(MArgs a, int r) {
+  if (blockIdx.x == 0 && threadIdx.x < 32) reinterpret_cast<int32_t*>(a.ctl + ((r + 2) & 3))[threadIdx.x] = 0;
+  Ctl* cin = a.ctl + (r & 3);
+  Ctl* cout = a.ctl + ((r + 1) & 3);
+  const int L = cin->qlen;
+  if (L == 0) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&a.cnt->rounds, (unsigned long long)(r + 1));
+  const int32_t* qin = (r & 1) ? a.q1 : a.q0;
+  int32_t* qout = (r & 1) ? a.q0 : a.q1;
+  const int label = r + 1;
+  const int lane = lane_id();
+  const int wave = threadIdx.x / SMDP_WAVE;
+  __shared__ int32_t s_out[SMDP_WAVES_PER_WG * kMsOut];
+  WaveOut<kMsOut> out;
+  out.init(s_out + wave * kMsOut);
+  unsigned long long edges = 0, verts = 0, wins = 0, pg = 0;
+  const int gw = blockIdx.x * SMDP_WAVES_PER_WG + wave;
+  const int nw = gridDim.x * SMDP_WAVES_PER_WG;
+  for (int i = gw; i < L; i += nw) {
+    const int u = qin[i];
+    const int s = a.row[u], e = a.row[u + 1];
+    verts += lane == 0;
+    if (e - s >= a.split) {
+      if (lane == 0) {
+        const int nch = (e - s + a.chunk - 1) / a.chunk;
+        const int b0 = atomicAdd(&cin->hlen, nch);
+        for (int c = 0; c < nch; ++c) a.hq[b0 + c] = Chunk{u, s + c * a.chunk};
+        pg += nch;
+      }
+      continue;
+    }
+    const int du = a.mdist[(int64_t)u * kLanes + lane];
+    ms_arcs(a, s, e, du, label, out, qout, &cout->qlen, edges, wins);
+  }
+  out.flush(qout, &cout->qlen);
+  edges = wave_sum(edges);
+  verts = wave_sum(verts);
+  wins = wave_sum(wins);
+  pg = wave_sum(pg);
+  if (lane == 0) {
+    if (edges) atomicAdd(&a.cnt->edges, edges);
+    if (verts) atomicAdd(&a.cnt->verts, verts);
+    if (wins) atomicAdd(&a.cnt->atom_win, wins);
+    if (pg) atomicAdd(&a.cnt->push[3], pg);
+  }
+}
+
+__global__ __launch_bounds__(SMDP_WG) void k_ms_drain(MArgs a, int r) {
+  Ctl* cin = a.ctl + (r & 3);
+  Ctl* cout = a.ctl + ((r + 1) & 3);
+  const int H = cin->hlen;
+  if (H == 0) return;
+  int32_t* qout = (r & 1) ? a.q0 : a.q1;
+  const int label = r + 1;
+  const int lane = lane_id();
+  const int wave = threadIdx.x / SMDP_WAVE;
+  __shared__ int32_t s_out[SMDP_WAVES_PER_WG * kMsOut];
+  WaveOut<kMsOut> out;
+  out.init(s_out + wave * kMsOut);
+  unsigned long long edges = 0, wins = 0;
+  for (;;) {
+    int idx = 0;
+    if (lane == 0) idx = atomicAdd(&cin->chunk_next, 1);
+    idx = __shfl(idx, 0);
+    if (idx >= H) break;
+    const Chunk c = a.hq[idx];
+    const int e = min(a.row[c.v + 1], c.first + a.chunk);
+    const int du = a.mdist[(int64_t)c.v * kLanes + lane];
+    ms_arcs(a, c.first, e, du, label, out, qout, &cout->qlen, edges, wins);
+  }
+  out.flush(qout, &cout->qlen);
+  edges = wave_sum(edges);
+  wins = wave_sum(wins);
+  if (lane == 0) {
+    if (edges) {
+      atomicAdd(&a.cnt->edges, edges);
+      atomicAdd(&a.cnt->edges2, edges);
+    }
+    if (wins) atomicAdd(&a.cnt->atom_win, wins);
+  }
+}
+
+// [n][64] group table -> rows [s0, s0+k) of the caller's [k_total][n] output.
+__global__ void k_ms_transpose(const int32_t* __restrict__ mdist, int32_t n, int32_t k, int32_t* out) {
+  __shared__ int32_t tile[64][65];
+  const int v0 = blockIdx.x * 64;
+  for (int i = threadIdx.y; i < 64; i += blockDim.y) {
+    const int v = v0 + i;
+    tile[i][threadIdx.x] = v < n ? mdist[(int64_t)v * kLanes + threadIdx.x] : 0;
+  }
+  __syncthreads();
+  for (int s = threadIdx.y; s < k; s += blockDim.y) {
+    const int v = v0 + threadIdx.x;
+    if (v < n) out[(int64_t)s * n + v] = tile[threadIdx.x][s];
+  }
+}
+
+// ----------------------------------------------------------- many graphs
+
+struct GArgs {
+  const int32_t* __restrict__ row;   // per graph n_i+1 offsets (already shifted by the graph's arc base)
+  const int32_t* __restrict__ col;   // local vertex ids
+  const int32_t* __restrict__ w;
+  const int64_t* __restrict__ rbase; // per graph: first entry in row
+  const int64_t* __restrict__ vbase; // per graph: first vertex slot in dist/stamp/queues
+  const int32_t* __restrict__ nv;    // per graph: vertices
+  const int32_t* __restrict__ src;   // per graph: source
+  const int32_t* __restrict__ order; // graphs by decreasing arcs
+  int32_t* dist;
+  int32_t* stamp;
+  int32_t* qa;
+  int32_t* qb;
+  int32_t* next;                     // graph counter
+  int32_t* status;                   // per graph: rounds run, or -1 when the round cap was hit
+  Counters* cnt;
+  int32_t ng;
+};
+
+constexpr int kMgOut = 256;
+
+__global__ __launch_bounds__(SMDP_WG) void k_mg_reset(int32_t* next, Counters* cnt) {
+  if (threadIdx.x == 0) *next = 0;
+  unsigned long long* p = reinterpret_cast<unsigned long long*>(cnt);
+  for (int i = threadIdx.x; i < (int)(sizeof(Counters) / 8); i += blockDim.x) p[i] = 0;
+}
+
+// One workgroup solves whole graphs, one after another.
+__global__ __launch_bounds__(SMDP_WG) void k_mg(GArgs a) {
+  __shared__ int s_g;
+  __shared__ int32_t s_qlen[2];
+  __shared__ int s_start[SMDP_WG];
+  __shared__ int s_du[SMDP_WG];
+  __shared__ int s_off[SMDP_WG];
+  __shared__ int s_scan[SMDP_WAVES_PER_WG];
+  __shared__ int32_t s_out[SMDP_WAVES_PER_WG * kMgOut];
+  const int wave = threadIdx.x / SMDP_WAVE;
+  unsigned long long edges = 0, verts = 0;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      const int t = atomicAdd(a.next, 1);
+      s_g = t < a.ng ? a.order[t] : -1;
+    }
+    __syncthreads();
+    const int g = s_g;
+    __syncthreads();
+    if (g < 0) break;
+    const int n = a.nv[g];
+    const int32_t* row = a.row + a.rbase[g];
+    int32_t* dist = a.dist + a.vbase[g];
+    int32_t* stamp = a.stamp + a.vbase[g];
+    int32_t* q[2] = {a.qa + a.vbase[g], a.qb + a.vbase[g]};
+    const int src = a.src[g];
+    for (int i = threadIdx.x; i < n; i += SMDP_WG) {
+      dist[i] = i == src ? 0 : SMDP_SSSP_INF;
+      stamp[i] = i == src ? 0 : -1;
+    }
+    if (threadIdx.x == 0) {
+      q[0][0] = src;
+      s_qlen[0] = 1;
+      s_qlen[1] = 0;
+    }
+    __syncthreads();
+    int r = 0;
+    for (;; ++r) {
+      const int cur = r & 1;
+      const int L = s_qlen[cur];
+      if (L == 0) break;
+      if (r > n) {  // more rounds than vertices: a negative cycle
+        if (threadIdx.x == 0) a.status[g] = -1;
+        break;
+      }
+      const int label = r + 1;
+      WaveOut<kMgOut> out;
+      out.init(s_out + wave * kMgOut);
+      int32_t* qout = q[cur ^ 1];
+      for (int base = 0; base < L; base += SMDP_WG) {
+        const int i = base + (int)threadIdx.x;
+        int s = 0, deg = 0, du = 0;
+        if (i < L) {
+          const int u = q[cur][i];
+          s = row[u];
+          deg = row[u + 1] - s;
+          // Rounds do not cross a kernel boundary here, so this CU's L1 may
+          // hold an older dist[u]; an agent-scope atomic load reads the
+          // value the atomics left in L2.
+          du = __hip_atomic_load(&dist[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ++verts;
+        }
+        s_start[threadIdx.x] = s;
+        s_du[threadIdx.x] = du;
+        int tot;
+        const int ex = wg_exclusive_scan(deg, s_scan, &tot);
+        s_off[threadIdx.x] = ex;
+        __syncthreads();
+        for (int jb = 0; jb < tot; jb += SMDP_WG) {
+          const int j = jb + (int)threadIdx.x;
+          int v = 0;
+          bool p = false;
+          if (j < tot) {
+            int lo = 0, hi = SMDP_WG - 1;
+            while (lo < hi) {
+              int mid = (lo + hi + 1) >> 1;
+              if (s_off[mid] <= j) lo = mid;
+              else hi = mid - 1;
+            }
+            const int k = s_start[lo] + (j - s_off[lo]);
+            v = a.col[k];
+            const int alt = s_du[lo] + a.w[k];
+            if (alt < dist[v]) {
+              const int old = atomicMin(&dist[v], alt);
+              if (alt < old && stamp[v] != label) p = atomicExch(&stamp[v], label) != label;
+            }
+            ++edges;
+          }
+          out.push(p, v, qout, &s_qlen[cur ^ 1]);
+        }
+        __syncthreads();
+      }
+      out.flush(qout, &s_qlen[cur ^ 1]);
+      __syncthreads();
+      if (threadIdx.x == 0) s_qlen[cur] = 0;
+      __syncthreads();
+    }
+    if (threadIdx.x == 0 && a.status[g] != -1) a.status[g] = r;
+    __syncthreads();
+  }
+  edges = wave_sum(edges);
+  verts = wave_sum(verts);
+  if (lane_id() == 0) {
+    if (edges) atomicAdd(&a.cnt->edges, edges);
+    if (verts) atomicAdd(&a.cnt->verts, verts);
+  }
+}
+
+__global__ void k_mg_status_clear(int32_t* status, int32_t ng) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ng; i += gridDim.x * blockDim.x) status[i] = 0;
+}
+
+// ------------------------------------------------------------------ host
+
+int run_multi_source(smdp_graph* g, const int32_t* srcs, int32_t k, int32_t* host_out, smdp_stats* st) {
+  if (!g || !srcs || k <= 0 || g->n <= 0) return SMDP_ERR_ARG;
+  if (g->m > 0 && !g->w) return SMDP_ERR_ARG;
+  for (int32_t i = 0; i < k; ++i)
+    if (srcs[i] < 0 || srcs[i] >= g->n) return SMDP_ERR_ARG;
+  smdp_ctx* ctx = g->ctx;
+  SMDP_HIP(hipSetDevice(ctx->device));
+  const smdp_params& p = params_of(ctx, SMDP_APP_MSSSP);
+  const int64_t n = g->n;
+  SMDP_CHECK(ensure(&g->mdist, &g->mdist_cap, n * kLanes));
+  SMDP_CHECK(alloc_once(&g->stamp, n));
+  SMDP_CHECK(alloc_once(&g->q[0], n));
+  SMDP_CHECK(alloc_once(&g->q[1], n));
+  SMDP_CHECK(ensure(&g->hq, &g->hq_cap, n + g->m / std::max(1, p.chunk) + 64));
+  int32_t* d_out = nullptr;   // [k][n] on the device
+  int32_t* d_srcs = nullptr;
+  if (hipMalloc((void**)&d_out, sizeof(int32_t) * (size_t)k * n) != hipSuccess) return SMDP_ERR_NOMEM;
+  if (hipMalloc((void**)&d_srcs, sizeof(int32_t) * k) != hipSuccess) {
+    (void)hipFree(d_out);
+    return SMDP_ERR_NOMEM;
+  }
+  int rc = SMDP_OK;
+  hipStream_t s = ctx->stream;
+  MArgs a;
+  a.row = g->row;
+  a.col = g->col;
+  a.w = g->w;
+  a.mdist = g->mdist;
+  a.stamp = g->stamp;
+  a.q0 = g->q[0];
+  a.q1 = g->q[1];
+  a.hq = g->hq;
+  a.ctl = g->ctl;
+  a.cnt = g->cnt;
+  a.split = p.split_threshold;
+  a.chunk = p.chunk;
+  const int grid = grid_size(ctx, p);
+  double ms_total = 0;
+  uint64_t edges = 0, verts = 0, rounds = 0, wins = 0, edges2 = 0, launches = 0;
+  prof_reset(ctx);
+  if (hipMemcpy(d_srcs, srcs, sizeof(int32_t) * k, hipMemcpyHostToDevice) != hipSuccess) rc = SMDP_ERR_DEVICE;
+  for (int32_t s0 = 0; s0 < k && rc == SMDP_OK; s0 += kLanes) {
+    const int kk = std::min<int32_t>(kLanes, k - s0);
+    if (hipEventRecord(ctx->ev_begin, s) != hipSuccess) {
+      rc = SMDP_ERR_DEVICE;
+      break;
+    }
+    hipLaunchKernelGGL(k_ms_init, dim3(2048), dim3(256), 0, s, g->mdist, g->stamp, g->n, d_srcs + s0, kk, g->q[0],
+                       g->ctl, g->cnt);
+    hipLaunchKernelGGL(k_ms_seed, dim3(1), dim3(64), 0, s, g->mdist, g->stamp, d_srcs + s0, kk, g->q[0], g->ctl);
+    launches += 2;
+    int r = 0, batch = std::max(1, p.batch_rounds);
+    for (;;) {
+      for (int j = 0; j < batch; ++j, ++r) {
+        prof_mark(ctx, 0);
+        hipLaunchKernelGGL(k_ms_expand, dim3(grid), dim3(SMDP_WG), 0, s, a, r);
+        prof_mark(ctx, 0);
+        prof_mark(ctx, 1);
+        hipLaunchKernelGGL(k_ms_drain, dim3(grid), dim3(SMDP_WG), 0, s, a, r);
+        prof_mark(ctx, 1);
+        launches += 2;
+      }
+      if (hipGetLastError() != hipSuccess ||
+          hipMemcpyAsync(ctx->pinned, &g->ctl[r & 3].qlen, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess) {
+        rc = SMDP_ERR_DEVICE;
+        break;
+      }
+      if (ctx->pinned[0] == 0) break;
+      if (r >= p.max_rounds) {
+        rc = SMDP_ERR_OVERFLOW;
+        break;
+      }
+      batch = std::min(batch * 2, 16);
+    }
+    if (rc != SMDP_OK) break;
+    hipLaunchKernelGGL(k_ms_transpose, dim3((unsigned)((n + 63) / 64)), dim3(64, 16), 0, s, g->mdist, g->n, kk,
+                       d_out + (int64_t)s0 * n);
+    ++launches;
+    float ms = 0;
+    if (hipEventRecord(ctx->ev_end, s) != hipSuccess || hipEventSynchronize(ctx->ev_end) != hipSuccess ||
+        hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end) != hipSuccess) {
+      rc = SMDP_ERR_DEVICE;
+      break;
+    }
+    ms_total += ms;
+    Counters c;
+    if (hipMemcpy(&c, g->cnt, sizeof c, hipMemcpyDeviceToHost) != hipSuccess) {
+      rc = SMDP_ERR_DEVICE;
+      break;
+    }
+    edges += c.edges;
+    edges2 += c.edges2;
+    verts += c.verts;
+    rounds += c.rounds;
+    wins += c.atom_win;
+  }
+  if (rc == SMDP_OK && host_out &&
+      hipMemcpy(host_out, d_out, sizeof(int32_t) * (size_t)k * n, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = SMDP_ERR_DEVICE;
+  (void)hipFree(d_out);
+  (void)hipFree(d_srcs);
+  if (rc != SMDP_OK) return rc;
+  g->msrc_k = k;
+  if (st) {
+    memset(st, 0, sizeof *st);
+    st->ms_kernel = ms_total;
+    st->edges_relaxed = edges;
+    st->edges_phase2 = edges2;
+    st->vertices_expanded = verts;
+    st->rounds = rounds;
+    st->atomics_won = wins;
+    st->kernel_launches = launches;
+    prof_collect(ctx, st);
+  } else {
+    prof_reset(ctx);
+  }
+  return SMDP_OK;
+}
+
+}  // namespace smdp
+
+struct smdp_batch {
+  smdp_ctx* ctx = nullptr;
+  int32_t ng = 0;
+  std::vector<int32_t> nv;
+  std::vector<int64_t> vbase;
+  smdp::GArgs args;
+  int32_t* d_row = nullptr;
+  int32_t* d_col = nullptr;
+  int32_t* d_w = nullptr;
+  int64_t* d_rbase = nullptr;
+  int64_t* d_vbase = nullptr;
+  int32_t* d_nv = nullptr;
+  int32_t* d_src = nullptr;
+  int32_t* d_order = nullptr;
+  int32_t* d_dist = nullptr;
+  int32_t* d_stamp = nullptr;
+  int32_t* d_qa = nullptr;
+  int32_t* d_qb = nullptr;
+  int32_t* d_next = nullptr;
+  int32_t* d_status = nullptr;
+  smdp::Counters* d_cnt = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int grid = 0;
+};
+
+using namespace smdp;
+
+extern "C" {
+
+int smdp_sssp_multi_source_run(smdp_graph* g, const int32_t* srcs, int32_t k, smdp_stats* st) {
+  return run_multi_source(g, srcs, k, nullptr, st);
+}
+
+int smdp_sssp_multi_source(smdp_ctx* ctx, const smdp_csr* h, const int32_t* srcs, int32_t k, int32_t* dist,
+                           smdp_stats* st) {
+  if (!ctx || !h || !srcs || !dist || k <= 0) return SMDP_ERR_ARG;
+  double t0 = now_ms();
+  smdp_graph* g = nullptr;
+  SMDP_CHECK(smdp_graph_create(ctx, h, &g));
+  int rc = run_multi_source(g, srcs, k, dist, st);
+  smdp_graph_destroy(g);
+  if (st) st->ms_total = now_ms() - t0;
+  return rc;
+}
+
+void smdp_batch_destroy(smdp_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->ctx->device);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  void* ptrs[] = {b->d_row,   b->d_col,   b->d_w,  b->d_rbase, b->d_vbase, b->d_nv,   b->d_src,    b->d_order,
+                  b->d_dist,  b->d_stamp, b->d_qa, b->d_qb,    b->d_next,  b->d_status, b->d_cnt};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete b;
+}
+
+int smdp_batch_create(smdp_ctx* ctx, const smdp_csr* graphs, int32_t ng, const int32_t* srcs, smdp_batch** out) {
+  if (!ctx || !graphs || !srcs || !out || ng <= 0) return SMDP_ERR_ARG;
+  *out = nullptr;
+  SMDP_HIP(hipSetDevice(ctx->device));
+  // validate and lay out: rows of graph i at rbase[i] (n_i+1 entries, shifted
+  // by the graph's arc base), vertices at vbase[i]
+  std::vector<int64_t> rbase(ng), vbase(ng), ebase(ng);
+  int64_t rtot = 0, vtot = 0, etot = 0;
+  for (int32_t i = 0; i < ng; ++i) {
+    const smdp_csr& c = graphs[i];
+    if (c.n <= 0 || c.m < 0 || !c.row || (c.m > 0 && (!c.col || !c.w)) || srcs[i] < 0 || srcs[i] >= c.n)
+      return SMDP_ERR_ARG;
+    if (c.row[0] != 0 || c.row[c.n] != c.m) return SMDP_ERR_ARG;
+    for (int32_t v = 0; v < c.n; ++v)
+      if (c.row[v + 1] < c.row[v]) return SMDP_ERR_ARG;
+    for (int32_t e = 0; e < c.m; ++e)
+      if ((uint32_t)c.col[e] >= (uint32_t)c.n) return SMDP_ERR_ARG;
+    rbase[i] = rtot;
+    vbase[i] = vtot;
+    ebase[i] = etot;
+    rtot += c.n + 1;
+    vtot += c.n;
+    etot += c.m;
+  }
+  if (etot > INT32_MAX) return SMDP_ERR_ARG;
+  std::vector<int32_t> row(rtot), col(std::max<int64_t>(1, etot)), w(std::max<int64_t>(1, etot)), nv(ng), order(ng);
+  for (int32_t i = 0; i < ng; ++i) {
+    const smdp_csr& c = graphs[i];
+    for (int32_t v = 0; v <= c.n; ++v) row[rbase[i] + v] = (int32_t)(ebase[i] + c.row[v]);
+    std::copy(c.col, c.col + c.m, col.begin() + ebase[i]);
+    std::copy(c.w, c.w + c.m, w.begin() + ebase[i]);
+    nv[i] = c.n;
+  }
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return graphs[x].m > graphs[y].m; });
+  smdp_batch* b = new smdp_batch();
+  b->ctx = ctx;
+  b->ng = ng;
+  b->nv = nv;
+  b->vbase = vbase;
+  int rc = SMDP_OK;
+  auto up32 = [&](int32_t** d, const int32_t* h, int64_t cnt) {
+    if (rc != SMDP_OK) return;
+    if (hipMalloc((void**)d, sizeof(int32_t) * std::max<int64_t>(1, cnt)) != hipSuccess) rc = SMDP_ERR_NOMEM;
+    else if (h && cnt && hipMemcpy(*d, h, sizeof(int32_t) * cnt, hipMemcpyHostToDevice) != hipSuccess)
+      rc = SMDP_ERR_DEVICE;
+  };
+  auto up64 = [&](int64_t** d, const int64_t* h, int64_t cnt) {
+    if (rc != SMDP_OK) return;
+    if (hipMalloc((void**)d, sizeof(int64_t) * cnt) != hipSuccess) rc = SMDP_ERR_NOMEM;
+    else if (hipMemcpy(*d, h, sizeof(int64_t) * cnt, hipMemcpyHostToDevice) != hipSuccess) rc = SMDP_ERR_DEVICE;
+  };
+  up32(&b->d_row, row.data(), rtot);
+  up32(&b->d_col, col.data(), etot);
+  up32(&b->d_w, w.data(), etot);
+  up64(&b->d_rbase, rbase.data(), ng);
+  up64(&b->d_vbase, vbase.data(), ng);
+  up32(&b->d_nv, nv.data(), ng);
+  up32(&b->d_src, srcs, ng);
+  up32(&b->d_order, order.data(), ng);
+  up32(&b->d_dist, nullptr, vtot);
+  up32(&b->d_stamp, nullptr, vtot);
+  up32(&b->d_qa, nullptr, vtot);
+  up32(&b->d_qb, nullptr, vtot);
+  up32(&b->d_next, nullptr, 1);
+  up32(&b->d_status, nullptr, ng);
+  if (rc == SMDP_OK && hipMalloc((void**)&b->d_cnt, sizeof(Counters)) != hipSuccess) rc = SMDP_ERR_NOMEM;
+  if (rc != SMDP_OK) {
+    smdp_batch_destroy(b);
+    return rc;
+  }
+  GArgs& a = b->args;
+  a.row = b->d_row;
+  a.col = b->d_col;
+  a.w = b->d_w;
+  a.rbase = b->d_rbase;
+  a.vbase = b->d_vbase;
+  a.nv = b->d_nv;
+  a.src = b->d_src;
+  a.order = b->d_order;
+  a.dist = b->d_dist;
+  a.stamp = b->d_stamp;
+  a.qa = b->d_qa;
+  a.qb = b->d_qb;
+  a.next = b->d_next;
+  a.status = b->d_status;
+  a.cnt = b->d_cnt;
+  a.ng = ng;
+  const smdp_params& p = params_of(ctx, SMDP_APP_MSSSP);
+  b->grid = std::max(1, std::min<int>(ng, ctx->cus * std::max(1, p.wg_per_cu)));
+  // Capture reset + solve once; every run replays the graph.  No memset nodes:
+  // counters are cleared by a kernel.
+  hipStream_t s = ctx->stream;
+  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = SMDP_ERR_DEVICE;
+  if (rc == SMDP_OK) {
+    hipLaunchKernelGGL(k_mg_reset, dim3(1), dim3(64), 0, s, b->d_next, b->d_cnt);
+    hipLaunchKernelGGL(k_mg_status_clear, dim3(std::max(1, (ng + 255) / 256)), dim3(256), 0, s, b->d_status, ng);
+    hipLaunchKernelGGL(k_mg, dim3(b->grid), dim3(SMDP_WG), 0, s, b->args);
+    if (hipStreamEndCapture(s, &b->graph) != hipSuccess) rc = SMDP_ERR_DEVICE;
+  }
+  if (rc == SMDP_OK && hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess)
+    rc = SMDP_ERR_DEVICE;
+  if (rc != SMDP_OK) {
+    smdp_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return SMDP_OK;
+}
+
+int smdp_batch_run(smdp_batch* b, smdp_stats* st) {
+  if (!b) return SMDP_ERR_ARG;
+  smdp_ctx* ctx = b->ctx;
+  SMDP_HIP(hipSetDevice(ctx->device));
+  double t0 = now_ms();
+  hipStream_t s = ctx->stream;
+  SMDP_HIP(hipEventRecord(ctx->ev_begin, s));
+  SMDP_HIP(hipGraphLaunch(b->exec, s));
+  SMDP_HIP(hipEventRecord(ctx->ev_end, s));
+  SMDP_HIP(hipEventSynchronize(ctx->ev_end));
+  float ms = 0;
+  SMDP_HIP(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end));
+  std::vector<int32_t> status(b->ng);
+  SMDP_HIP(hipMemcpy(status.data(), b->d_status, sizeof(int32_t) * b->ng, hipMemcpyDeviceToHost));
+  Counters c;
+  SMDP_HIP(hipMemcpy(&c, b->d_cnt, sizeof c, hipMemcpyDeviceToHost));
+  int rc = SMDP_OK;
+  uint64_t rounds = 0;
+  for (int32_t v : status) {
+    if (v < 0) rc = SMDP_ERR_OVERFLOW;
+    else rounds = std::max<uint64_t>(rounds, (uint64_t)v);
+  }
+  if (st) {
+    memset(st, 0, sizeof *st);
+    st->ms_kernel = ms;
+    st->ms_total = now_ms() - t0;
+    st->edges_relaxed = c.edges;
+    st->vertices_expanded = c.verts;
+    st->rounds = rounds;
+    st->kernel_launches = 3;
+  }
+  return rc;
+}
+
+int smdp_batch_read(smdp_batch* b, int32_t i, int32_t* dist) {
+  if (!b || !dist || i < 0 || i >= b->ng) return SMDP_ERR_ARG;
+  SMDP_HIP(hipSetDevice(b->ctx->device));
+  SMDP_HIP(hipMemcpy(dist, b->d_dist + b->vbase[i], sizeof(int32_t) * b->nv[i], hipMemcpyDeviceToHost));
+  return SMDP_OK;
+}
+
+int smdp_sssp_multi_graph(smdp_ctx* ctx, const smdp_csr* graphs, int32_t ng, const int32_t* srcs,
+                          int32_t* const* dist, smdp_stats* st) {
+  if (!ctx || !graphs || !srcs || !dist || ng <= 0) return SMDP_ERR_ARG;
+  double t0 = now_ms();
+  smdp_batch* b = nullptr;
+  SMDP_CHECK(smdp_batch_create(ctx, graphs, ng, srcs, &b));
+  int rc = smdp_batch_run(b, st);
+  for (int32_t i = 0; i < ng && rc == SMDP_OK; ++i) rc = smdp_batch_read(b, i, dist[i]);
+  smdp_batch_destroy(b);
+  if (st) st->ms_total = now_ms() - t0;
+  return rc;
+}
+
+}  // extern "C"

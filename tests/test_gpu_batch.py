@@ -1,0 +1,74 @@
+"""Batched SSSP on the device: many sources on one graph, and many small
+graphs in one captured hipGraph launch.  Distances must equal Dijkstra's."""
+import numpy as np
+import pytest
+
+import orc
+import smdp
+from conftest import random_graph
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("k", [1, 5, 64, 70])
+def test_multi_source_parity(ctx, k):
+    g = smdp.gen_rmat(12, 16, wlo=1, whi=255, seed=4)
+    srcs = smdp.gen_sources(g, k, 11)
+    d, st = ctx.sssp_multi_source(g, srcs)
+    assert d.shape == (k, g.n)
+    for i, s in enumerate(srcs):
+        assert np.array_equal(d[i], orc.dijkstra(g.row, g.col, g.w, int(s))), i
+    assert st.rounds >= 1
+
+
+def test_multi_source_duplicates_and_chunks(ctx):
+    row, col, w = random_graph(3000, 40000, 12, 1, 30)
+    g = smdp.HostGraph(row, col, w)
+    srcs = np.array([0, 0, 5, 17, 5, 2999], dtype=np.int32)
+    old = ctx.get_params(smdp.APP_MSSSP)
+    try:
+        for split, chunk in [(512, 256), (4, 64)]:
+            ctx.set_params(smdp.APP_MSSSP, split_threshold=split, chunk=chunk)
+            d, _ = ctx.sssp_multi_source(g, srcs)
+            for i, s in enumerate(srcs):
+                assert np.array_equal(d[i], orc.dijkstra(row, col, w, int(s)))
+    finally:
+        ctx.set_params(smdp.APP_MSSSP, **old.as_dict())
+
+
+def _small_graphs(count, seed=4):
+    gs = [smdp.gen_small_graph(i, vlo=64, vhi=4096, seed=seed) for i in range(count)]
+    gs.append(smdp.HostGraph([0, 0], [], [], 0))           # one vertex
+    gs.append(smdp.HostGraph([0, 1, 1, 1], [2], [9], 0))   # unreachable vertex
+    return gs
+
+
+def test_multi_graph_parity(ctx):
+    graphs = _small_graphs(40)
+    srcs = np.zeros(len(graphs), dtype=np.int32)
+    srcs[3] = graphs[3].n - 1
+    outs, st = ctx.sssp_multi_graph(graphs, srcs)
+    for i, (g, d) in enumerate(zip(graphs, outs)):
+        assert np.array_equal(d, orc.dijkstra(g.row, g.col, g.w, int(srcs[i]))), i
+
+
+def test_batch_graph_replays(ctx):
+    """The captured hipGraph is replayed with eager work in between; every
+    replay must give the same exact distances."""
+    graphs = _small_graphs(24, seed=9)
+    srcs = np.zeros(len(graphs), dtype=np.int32)
+    refs = [orc.dijkstra(g.row, g.col, g.w, 0) for g in graphs]
+    with smdp.Batch(ctx, graphs, srcs) as b:
+        for rep in range(4):
+            st = b.run()
+            for i in range(len(graphs)):
+                assert np.array_equal(b.read(i), refs[i]), (rep, i)
+            ctx.sssp(graphs[0], 0, smdp.SOL_TUNED)  # eager work between replays
+        assert st.edges_relaxed > 0
+
+
+def test_multi_graph_bad_args(ctx):
+    g = smdp.HostGraph([0, 1, 1], [1], [3])
+    with pytest.raises(smdp.SmdpError) as e:
+        ctx.sssp_multi_graph([g], np.array([5], dtype=np.int32))
+    assert e.value.code == smdp.ERR_ARG
