@@ -1,0 +1,221 @@
+// Autotuner for the tuned solution (host code).
+//
+// The reference hard-codes its launch shape and heavy-degree threshold
+// (THRESHOLD 64, child grids of 13 or 13*16 blocks: sssp_kernel.cu:7, :241,
+// :380); its own logs show the best threshold moves with the input
+// (apps/sssp/log/sssp_performance.log).  Here the tuned solution's split
+// degree, chunk size, heavy degree and workgroups per CU are chosen by timing
+// candidate sets on the resident graph itself; the winner is installed in the
+// context and cached in a JSON file keyed by GPU architecture, app and the
+// graph's log2 degree histogram, so a later run on the same kind of graph
+// skips the search.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "smdp_internal.h"
+
+namespace smdp {
+
+int run_frontier(smdp_graph* g, int32_t src, int sol, bool bfs, smdp_stats* st);
+int run_spmv(smdp_graph* g, int sol, smdp_stats* st);
+int run_pagerank(smdp_graph* g, const smdp_pr_opts* o, int sol, smdp_stats* st);
+int run_color(smdp_graph* g, int use_prio, int sol, smdp_stats* st);
+int run_multi_source(smdp_graph* g, const int32_t* srcs, int32_t k, int32_t* host_out, smdp_stats* st);
+
+static const char* app_name(int app) {
+  static const char* names[] = {"sssp", "bfs", "spmv", "pagerank", "color", "msssp"};
+  return names[app];
+}
+
+// "gfx950:sssp:n=..:m=..:h=b0/b1/.." with b_i the number of vertices whose
+// degree is in [2^i, 2^(i+1)), rounded to two significant digits.
+static int cache_key(smdp_graph* g, int app, std::string* key) {
+  std::vector<int32_t> row((size_t)g->n + 1);
+  SMDP_HIP(hipMemcpy(row.data(), g->row, sizeof(int32_t) * row.size(), hipMemcpyDeviceToHost));
+  int64_t hist[33] = {0};
+  for (int32_t v = 0; v < g->n; ++v) {
+    int32_t d = row[v + 1] - row[v];
+    int b = d <= 0 ? 0 : 1 + (31 - __builtin_clz((unsigned)d));
+    hist[b]++;
+  }
+  char arch[64];
+  strncpy(arch, g->ctx->arch, sizeof arch - 1);
+  arch[sizeof arch - 1] = 0;
+  if (char* c = strchr(arch, ':')) *c = 0;
+  char buf[128];
+  snprintf(buf, sizeof buf, "%s:%s:n=%d:m=%d:h=", arch, app_name(app), g->n, g->m);
+  std::string k = buf;
+  for (int b = 0; b < 33; ++b) {
+    if (!hist[b]) continue;
+    double x = (double)hist[b];
+    double p = std::pow(10.0, std::floor(std::log10(x)) - 1);
+    snprintf(buf, sizeof buf, "%d:%.0f/", b, std::round(x / p) * p);
+    k += buf;
+  }
+  *key = k;
+  return SMDP_OK;
+}
+
+// The cache file is a JSON object mapping keys to parameter objects, one
+// entry per line, written and read only by this file.
+static std::map<std::string, smdp_params> load_cache(const char* path) {
+  std::map<std::string, smdp_params> out;
+  if (!path) return out;
+  FILE* f = fopen(path, "r");
+  if (!f) return out;
+  char line[4096];
+  while (fgets(line, sizeof line, f)) {
+    char key[2048];
+    smdp_params p;
+    memset(&p, 0, sizeof p);
+    if (sscanf(line,
+               " \"%2047[^\"]\": {\"heavy_threshold\": %d, \"split_threshold\": %d, \"chunk\": %d, "
+               "\"wg_per_cu\": %d, \"batch_rounds\": %d, \"max_rounds\": %d}",
+               key, &p.heavy_threshold, &p.split_threshold, &p.chunk, &p.wg_per_cu, &p.batch_rounds,
+               &p.max_rounds) == 7)
+      out[key] = p;
+  }
+  fclose(f);
+  return out;
+}
+
+static void save_cache(const char* path, const std::map<std::string, smdp_params>& m) {
+  if (!path) return;
+  std::string tmp = std::string(path) + ".tmp";
+  FILE* f = fopen(tmp.c_str(), "w");
+  if (!f) return;
+  fprintf(f, "{\n");
+  size_t i = 0;
+  for (auto& kv : m) {
+    const smdp_params& p = kv.second;
+    fprintf(f,
+            "  \"%s\": {\"heavy_threshold\": %d, \"split_threshold\": %d, \"chunk\": %d, \"wg_per_cu\": %d, "
+            "\"batch_rounds\": %d, \"max_rounds\": %d}%s\n",
+            kv.first.c_str(), p.heavy_threshold, p.split_threshold, p.chunk, p.wg_per_cu, p.batch_rounds,
+            p.max_rounds, ++i < m.size() ? "," : "");
+  }
+  fprintf(f, "}\n");
+  fclose(f);
+  rename(tmp.c_str(), path);
+}
+
+// Device time of one run of `app` with the context's current parameters.
+static int time_once(smdp_graph* g, int app, int32_t src, double* ms) {
+  smdp_stats st;
+  memset(&st, 0, sizeof st);
+  int rc;
+  switch (app) {
+    case SMDP_APP_SSSP: rc = run_frontier(g, src, SMDP_SOL_TUNED, false, &st); break;
+    case SMDP_APP_BFS: rc = run_frontier(g, src, SMDP_SOL_TUNED, true, &st); break;
+    case SMDP_APP_SPMV: rc = run_spmv(g, SMDP_SOL_TUNED, &st); break;
+    case SMDP_APP_PAGERANK: {
+      smdp_pr_opts o;
+      o.damping = 0.85f;
+      o.total_rank = 1.0f;
+      o.tol_l1 = 0.0;
+      o.max_iter = 4;
+      o.init_zero = 0;
+      rc = run_pagerank(g, &o, SMDP_SOL_TUNED, &st);
+      break;
+    }
+    case SMDP_APP_COLOR: rc = run_color(g, g->prio != nullptr, SMDP_SOL_TUNED, &st); break;
+    default: {
+      std::vector<int32_t> srcs(16);
+      for (size_t i = 0; i < srcs.size(); ++i) srcs[i] = (int32_t)(((int64_t)src + 7919 * (int64_t)i) % g->n);
+      rc = run_multi_source(g, srcs.data(), (int32_t)srcs.size(), nullptr, &st);
+      break;
+    }
+  }
+  *ms = st.ms_kernel;
+  return rc;
+}
+
+static std::vector<smdp_params> candidates(int app, const smdp_params& base) {
+  std::vector<smdp_params> out;
+  auto add = [&](int heavy, int split, int chunk, int wg) {
+    smdp_params p = base;
+    p.heavy_threshold = heavy;
+    p.split_threshold = split;
+    p.chunk = chunk;
+    p.wg_per_cu = wg;
+    out.push_back(p);
+  };
+  switch (app) {
+    case SMDP_APP_SSSP:
+    case SMDP_APP_BFS:
+      for (int split : {512, 2048, 8192})
+        for (int chunk : {1024, 4096})
+          for (int wg : {2, 4, 8}) add(base.heavy_threshold, split, chunk, wg);
+      break;
+    case SMDP_APP_SPMV:
+    case SMDP_APP_PAGERANK:
+      for (int heavy : {32, 64, 128})
+        for (int split : {2048, 8192})
+          for (int wg : {2, 4, 8}) add(heavy, split, split, wg);
+      break;
+    case SMDP_APP_COLOR:
+      for (int heavy : {16, 64, 256})
+        for (int split : {1024, 8192})
+          for (int wg : {4, 8}) add(heavy, split, base.chunk, wg);
+      break;
+    default:
+      for (int split : {256, 1024, 4096})
+        for (int chunk : {128, 512})
+          for (int wg : {4, 8}) add(base.heavy_threshold, split, chunk, wg);
+      break;
+  }
+  return out;
+}
+
+}  // namespace smdp
+
+using namespace smdp;
+
+extern "C" int smdp_autotune(smdp_graph* g, int app, int32_t src, const char* cache_path, smdp_params* chosen) {
+  if (!g || app < 0 || app >= SMDP_APP_COUNT || g->n <= 0) return SMDP_ERR_ARG;
+  if ((app == SMDP_APP_SSSP || app == SMDP_APP_BFS || app == SMDP_APP_MSSSP) && (src < 0 || src >= g->n))
+    return SMDP_ERR_ARG;
+  if ((app == SMDP_APP_SSSP || app == SMDP_APP_MSSSP) && g->m > 0 && !g->w) return SMDP_ERR_ARG;
+  if (app == SMDP_APP_SPMV && (!g->val || !g->x)) return SMDP_ERR_ARG;
+  if (app == SMDP_APP_PAGERANK && !g->outdeg) return SMDP_ERR_ARG;
+  smdp_ctx* ctx = g->ctx;
+  SMDP_HIP(hipSetDevice(ctx->device));
+  std::string key;
+  SMDP_CHECK(cache_key(g, app, &key));
+  auto cache = load_cache(cache_path);
+  auto hit = cache.find(key);
+  if (hit != cache.end()) {
+    SMDP_CHECK(smdp_set_params(ctx, app, &hit->second));
+    if (chosen) *chosen = ctx->params[app];
+    return SMDP_OK;
+  }
+  const smdp_params base = ctx->params[app];
+  smdp_params best = base;
+  double best_ms = 1e30;
+  for (const smdp_params& p : candidates(app, base)) {
+    SMDP_CHECK(smdp_set_params(ctx, app, &p));
+    double ms0 = 0, ms1 = 0;
+    int rc = time_once(g, app, src, &ms0);  // first run also allocates and plans
+    if (rc == SMDP_OK) rc = time_once(g, app, src, &ms1);
+    if (rc != SMDP_OK) {
+      smdp_set_params(ctx, app, &base);
+      return rc;
+    }
+    if (ms1 < best_ms) {
+      best_ms = ms1;
+      best = ctx->params[app];
+    }
+  }
+  SMDP_CHECK(smdp_set_params(ctx, app, &best));
+  if (chosen) *chosen = best;
+  cache[key] = best;
+  save_cache(cache_path, cache);
+  return SMDP_OK;
+}
