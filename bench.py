@@ -199,6 +199,49 @@ def batched_section(ctx, args, smdp, orc, np):
     return out
 
 
+def copy_calibration(nbytes=1 << 28, reps=4):
+    """A plain device-to-device copy of known size (hipMemcpy), so that
+    rocprofv3 FETCH_SIZE / WRITE_SIZE readings can be scaled to bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: {reps} copies of {nbytes} bytes")
+
+
+def only_section(args, smdp, orc, np):
+    ctx = smdp.Context(0)
+    ctx.set_params(smdp.APP_PAGERANK, col_blocks=args.col_blocks)
+    if args.params:
+        kw = json.loads(args.params)
+        for app in (smdp.APP_SSSP, smdp.APP_PAGERANK):
+            ctx.set_params(app, **kw)
+    if args.only in ("sssp", "pagerank"):
+        g = smdp.gen_rmat(args.scale, 16, wlo=1, whi=255, seed=3)
+        if args.only == "sssp":
+            with smdp.Graph(ctx, g) as dg:
+                for _ in range(args.warmup + args.steps):
+                    st = dg.sssp(g.source, smdp.SOL_TUNED)
+                log(f"sssp: {st.ms_kernel:.3f} ms, {st.rounds} rounds, {st.edges_relaxed} arcs")
+        else:
+            ig = smdp.invert(g)
+            with smdp.Graph(ctx, smdp.HostGraph(ig.row, ig.col)) as dg:
+                dg.set_outdeg(g.outdegree())
+                for _ in range(args.warmup + args.steps):
+                    st = dg.pagerank(smdp.pr_opts(0.85, 1.0, 1e-6, 100), smdp.SOL_TUNED)
+                log(f"pagerank: {st.ms_kernel:.3f} ms, {st.rounds} iterations")
+    else:
+        log(json.dumps(batched_section(ctx, args, smdp, orc, np)))
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,6 +254,10 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-pagerank", action="store_true")
     ap.add_argument("--skip-batched", action="store_true")
+    ap.add_argument("--col-blocks", type=int, default=0, help="SpMV/PageRank layout: 0 auto, 1 plain, 8 blocked")
+    ap.add_argument("--params", default="", help="profiling runs: JSON object of smdp_params fields to set")
+    ap.add_argument("--only", choices=["sssp", "pagerank", "batched", "copy"],
+                    help="profiling runs: one section only, no JSON contract line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; this build measures one GPU "
@@ -221,6 +268,13 @@ def main():
 
     import orc
     import smdp
+
+    if args.only == "copy":
+        copy_calibration()
+        return
+    if args.only:
+        only_section(args, smdp, orc, np)
+        return
 
     t0 = time.time()
     g = smdp.gen_rmat(args.scale, 16, wlo=1, whi=255, seed=3)

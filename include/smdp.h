@@ -91,6 +91,8 @@ typedef struct {
   int32_t wg_per_cu;        /* persistent workgroups per compute unit */
   int32_t batch_rounds;     /* rounds enqueued between two host checks of the queue length */
   int32_t max_rounds;       /* safety cap on rounds (SMDP_ERR_OVERFLOW when hit) */
+  int32_t col_blocks;       /* SpMV/PageRank tuned: 1 = plain CSR; 8 = columns split into 8 blocks, one per
+                               XCD, so each XCD gathers from a slice that fits its L2; 0 = automatic */
 } smdp_params;
 
 /* Work counters and timings of the last call. */

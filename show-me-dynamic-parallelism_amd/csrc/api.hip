@@ -34,7 +34,7 @@ static smdp_params default_params(int app) {
   p.heavy_threshold = 64;  // THRESHOLD, apps/sssp/src/sssp_kernel.cu:7
   p.split_threshold = 2048;
   p.chunk = 2048;
-  p.wg_per_cu = 4;
+  p.wg_per_cu = 8;
   p.batch_rounds = 4;
   p.max_rounds = 1 << 22;
   switch (app) {
@@ -191,6 +191,7 @@ int smdp_set_params(smdp_ctx* ctx, int app, const smdp_params* p) {
   if (q.batch_rounds <= 0) q.batch_rounds = d.batch_rounds;
   if (q.max_rounds <= 0) q.max_rounds = d.max_rounds;
   if (q.wg_per_cu > 64 || q.chunk < 64) return SMDP_ERR_ARG;
+  if (q.col_blocks != 0 && q.col_blocks != 1 && q.col_blocks != 8) return SMDP_ERR_ARG;
   ctx->params[app] = q;
   return SMDP_OK;
 }
@@ -276,6 +277,7 @@ static int upload_f32(float** d, const float* s, int64_t cnt) {
 int smdp_graph_set_values(smdp_graph* g, const float* val) {
   if (!g) return SMDP_ERR_ARG;
   SMDP_HIP(hipSetDevice(g->ctx->device));
+  ++g->val_version;  // layouts derived from the values must be rebuilt
   return upload_f32(&g->val, val, g->m);
 }
 

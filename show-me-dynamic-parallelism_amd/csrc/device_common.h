@@ -83,6 +83,22 @@ struct WaveOut {
   }
 };
 
+// Owners of K edge slots in a tile: for each j, the last index i of the
+// SMDP_WG-entry non-decreasing array off[] with off[i] <= j (off[0] == 0).
+// A fixed 8-step search, interleaved over the K slots so their LDS reads
+// overlap instead of forming one dependent chain each.
+template <int K>
+__device__ __forceinline__ void tile_owners(const int* off, const int (&j)[K], int (&own)[K]) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) own[k] = 0;
+#pragma unroll
+  for (int step = SMDP_WG / 2; step > 0; step >>= 1) {
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (off[own[k] + step] <= j[k]) own[k] += step;
+  }
+}
+
 // Inclusive prefix sum across the active wave (all 64 lanes must be active).
 __device__ __forceinline__ int wave_inclusive_scan(int x) {
   const int l = lane_id();

@@ -34,6 +34,8 @@ namespace smdp {
 constexpr int kWaveQ = 256;   // per-wave LDS queue capacity (SMDP_SOL_WAVE)
 constexpr int kBlockQ = 1024; // per-workgroup LDS queue capacity (SMDP_SOL_BLOCK)
 constexpr int kOutCap = 512;  // per-wave LDS staging of next-queue entries
+constexpr int kIlp = 4;       // arcs per thread per step in the tuned kernels
+constexpr int kTileGrab = 4;  // frontier tiles claimed per atomic
 
 // Next-queue output of one wave: LDS staging + the global queue it flushes to.
 struct Sink {
@@ -75,8 +77,20 @@ __device__ __forceinline__ int arc_weight(const FArgs& a, int e) {
 
 // Relax one arc with candidate `alt`.  Returns true when this lane lowered
 // dist[v] and is the first in this round to claim v for the next queue.
+// relax_seen() takes a dist[v] read earlier (so several reads can be in
+// flight together); relax() reads it itself.
 // A plain read of dist[v] may be stale (larger) when another XCD lowered it;
 // the atomic decides, so a stale read only costs an extra atomic.
+__device__ __forceinline__ bool relax_seen(const FArgs& a, int v, int alt, int seen, int label, Sink& out) {
+  if (alt >= seen) return false;
+  ++out.tries;
+  int old = atomicMin(&a.dist[v], alt);
+  if (alt >= old) return false;
+  ++out.wins;
+  if (a.stamp[v] == label) return false;
+  return atomicExch(&a.stamp[v], label) != label;
+}
+
 __device__ __forceinline__ bool relax(const FArgs& a, int v, int alt, int label, Sink& out) {
   if (alt >= a.dist[v]) return false;
   ++out.tries;
@@ -139,16 +153,24 @@ __device__ __forceinline__ void wave_expand(const FArgs& a, int s, int e, int du
 template <bool BFS>
 __device__ __forceinline__ void wg_expand(const FArgs& a, int s, int e, int du, int label,
                                           unsigned long long& edges, Sink& out) {
-  for (int b = s; b < e; b += SMDP_WG) {
-    int k = b + (int)threadIdx.x;
-    int v = 0;
-    bool p = false;
-    if (k < e) {
-      v = a.col[k];
-      p = relax(a, v, du + arc_weight<BFS>(a, k), label, out);
-      ++edges;
+  for (int b = s; b < e; b += kIlp * SMDP_WG) {
+    int k[kIlp], v[kIlp], alt[kIlp];
+#pragma unroll
+    for (int t = 0; t < kIlp; ++t) k[t] = min(b + (int)threadIdx.x + t * SMDP_WG, e - 1);
+#pragma unroll
+    for (int t = 0; t < kIlp; ++t) v[t] = a.col[k[t]];
+#pragma unroll
+    for (int t = 0; t < kIlp; ++t) alt[t] = du + arc_weight<BFS>(a, k[t]);
+    int seen[kIlp];
+#pragma unroll
+    for (int t = 0; t < kIlp; ++t) seen[t] = a.dist[v[t]];
+#pragma unroll
+    for (int t = 0; t < kIlp; ++t) {
+      const bool ok = b + (int)threadIdx.x + t * SMDP_WG < e;
+      const bool p = ok && relax_seen(a, v[t], alt[t], seen[t], label, out);
+      edges += ok;
+      out.push(p, v[t]);
     }
-    out.push(p, v);
   }
 }
 
@@ -331,10 +353,13 @@ __global__ __launch_bounds__(SMDP_WG) void k_expand_tile(FArgs a, int r) {
 
   unsigned long long edges = 0, verts = 0, pg = 0;
   for (;;) {
-    if (threadIdx.x == 0) s_tile = atomicAdd(&cin->tile_next, 1);
+    if (threadIdx.x == 0) s_tile = atomicAdd(&cin->tile_next, kTileGrab);
     __syncthreads();
-    const int base = s_tile * SMDP_WG;
-    if (base >= L) break;
+    const int first = s_tile;
+    __syncthreads();
+    if (first * SMDP_WG >= L) break;
+    for (int tile = first; tile < first + kTileGrab && tile * SMDP_WG < L; ++tile) {
+    const int base = tile * SMDP_WG;
     const int i = base + (int)threadIdx.x;
     const bool valid = i < L;
     int u = 0, s = 0, deg = 0, du = 0;
@@ -363,26 +388,33 @@ __global__ __launch_bounds__(SMDP_WG) void k_expand_tile(FArgs a, int r) {
     const int ex = wg_exclusive_scan(split ? 0 : deg, s_scan, &tot);
     s_off[threadIdx.x] = ex;
     __syncthreads();
-    for (int jb = 0; jb < tot; jb += SMDP_WG) {
-      const int j = jb + (int)threadIdx.x;
-      int v = 0;
-      bool p = false;
-      if (j < tot) {
-        // owner: the last tile slot whose arc range starts at or before j
-        int lo = 0, hi = SMDP_WG - 1;
-        while (lo < hi) {
-          int mid = (lo + hi + 1) >> 1;
-          if (s_off[mid] <= j) lo = mid;
-          else hi = mid - 1;
-        }
-        const int k = s_start[lo] + (j - s_off[lo]);
-        v = a.col[k];
-        p = relax(a, v, s_du[lo] + arc_weight<BFS>(a, k), label, out);
-        ++edges;
+    // Each thread takes kIlp arcs per step (slots j, j+WG, ...): owners are
+    // found together and all loads are issued before any atomic, so every
+    // thread keeps kIlp gathers in flight.
+    for (int jb = 0; jb < tot; jb += kIlp * SMDP_WG) {
+      int j[kIlp], own[kIlp], k[kIlp], v[kIlp], alt[kIlp];
+#pragma unroll
+      for (int t = 0; t < kIlp; ++t) j[t] = min(jb + (int)threadIdx.x + t * SMDP_WG, tot - 1);
+      tile_owners<kIlp>(s_off, j, own);
+#pragma unroll
+      for (int t = 0; t < kIlp; ++t) k[t] = s_start[own[t]] + (j[t] - s_off[own[t]]);
+#pragma unroll
+      for (int t = 0; t < kIlp; ++t) v[t] = a.col[k[t]];
+#pragma unroll
+      for (int t = 0; t < kIlp; ++t) alt[t] = s_du[own[t]] + arc_weight<BFS>(a, k[t]);
+      int seen[kIlp];
+#pragma unroll
+      for (int t = 0; t < kIlp; ++t) seen[t] = a.dist[v[t]];
+#pragma unroll
+      for (int t = 0; t < kIlp; ++t) {
+        const bool ok = jb + (int)threadIdx.x + t * SMDP_WG < tot;
+        const bool p = ok && relax_seen(a, v[t], alt[t], seen[t], label, out);
+        edges += ok;
+        out.push(p, v[t]);
       }
-      out.push(p, v);
     }
     __syncthreads();
+    }
   }
   flush_counters(a.cnt, edges, verts, out.pushes, 0, 0, pg, out);
 }

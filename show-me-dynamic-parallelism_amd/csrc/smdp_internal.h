@@ -95,6 +95,7 @@ struct smdp_graph {
   int32_t msrc_k = 0;
   int32_t msrc_groups = 0;
   int last_res = -1;
+  int val_version = 0;      // bumped by smdp_graph_set_values
   void* rowbufs = nullptr;  // row-reduce family buffers (rowreduce.hip)
   void* colorbufs = nullptr; // coloring buffers (color.hip)
 };

@@ -39,7 +39,8 @@ class Csr(C.Structure):
 
 class Params(C.Structure):
     _fields_ = [("heavy_threshold", C.c_int32), ("split_threshold", C.c_int32), ("chunk", C.c_int32),
-                ("wg_per_cu", C.c_int32), ("batch_rounds", C.c_int32), ("max_rounds", C.c_int32)]
+                ("wg_per_cu", C.c_int32), ("batch_rounds", C.c_int32), ("max_rounds", C.c_int32),
+                ("col_blocks", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -157,3 +157,52 @@ def test_pagerank_max_iter_and_resident(ctx):
         r = dg.read(smdp.RES_RANK, np.float32)
         ref, _, _ = orc.pagerank(ig.row, ig.col, od, fixed_iters=7)
         assert np.abs(r - ref).sum() <= 1e-6
+
+
+@pytest.mark.parametrize("name", sorted(MATS))
+def test_column_blocked_layout(ctx, name):
+    """Tuned solution with the sources split into 8 XCD blocks (forced on
+    small inputs; chosen automatically once the gathered vector outgrows an
+    L2)."""
+    g = MATS[name]
+    old_s, old_p = ctx.get_params(smdp.APP_SPMV), ctx.get_params(smdp.APP_PAGERANK)
+    try:
+        for app in (smdp.APP_SPMV, smdp.APP_PAGERANK):
+            ctx.set_params(app, col_blocks=8, split_threshold=256, heavy_threshold=16)
+        val = smdp.gen_uniform_f32(g.m, -1.0, 1.0, 4, 6)
+        x = smdp.gen_uniform_f32(g.n, 0.0, 1.0, 4, 9)
+        y, _ = ctx.spmv(g, val, x, smdp.SOL_TUNED)
+        y64 = orc.spmv(g.row, g.col, val, x)
+        assert np.all(np.abs(y - y64) <= _spmv_bound(g, val, x, y64))
+        data, xr = smdp.gen_spmv_reference_inputs(g.n, g.m)
+        yr, _ = ctx.spmv(g, data, xr, smdp.SOL_TUNED)
+        assert np.array_equal(yr.astype(np.float64), orc.spmv(g.row, g.col, data, xr))
+        ig = smdp.invert(g)
+        od = g.outdegree()
+        r1, st = ctx.pagerank(ig, od, smdp.pr_opts(), smdp.SOL_TUNED)
+        ref, _, _ = orc.pagerank(ig.row, ig.col, od, fixed_iters=int(st.rounds))
+        assert np.abs(r1 - ref).sum() <= 1e-6
+        r2, _ = ctx.pagerank(ig, od, smdp.pr_opts(), smdp.SOL_TUNED)
+        assert np.array_equal(r1, r2)
+    finally:
+        ctx.set_params(smdp.APP_SPMV, **old_s.as_dict())
+        ctx.set_params(smdp.APP_PAGERANK, **old_p.as_dict())
+
+
+def test_values_change_rebuilds_blocked_layout(ctx):
+    g = MATS["rmat12"]
+    old = ctx.get_params(smdp.APP_SPMV)
+    try:
+        ctx.set_params(smdp.APP_SPMV, col_blocks=8)
+        x = smdp.gen_uniform_f32(g.n, 0.0, 1.0, 5, 9)
+        with smdp.Graph(ctx, g) as dg:
+            dg.set_x(x)
+            for seed in (1, 2):
+                val = smdp.gen_uniform_f32(g.m, -1.0, 1.0, seed, 6)
+                dg.set_values(val)
+                dg.spmv(smdp.SOL_TUNED)
+                y = dg.read(smdp.RES_Y, np.float32)
+                y64 = orc.spmv(g.row, g.col, val, x)
+                assert np.all(np.abs(y - y64) <= _spmv_bound(g, val, x, y64))
+    finally:
+        ctx.set_params(smdp.APP_SPMV, **old.as_dict())
