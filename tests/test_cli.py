@@ -1,0 +1,76 @@
+"""The command-line drivers: reference flags, messages and output formats."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import PKG
+
+BIN = os.path.join(PKG, "bin")
+
+
+def run(app, *args):
+    return subprocess.run([os.path.join(BIN, app), *args], capture_output=True, text=True, timeout=120)
+
+
+@pytest.mark.parametrize("app", ["gpu-sssp", "gpu-SpMV", "gpu-pg", "gpu-graph-color", "gpu-bfs-rec"])
+def test_usage_and_input_errors(app, tmp_path):
+    r = run(app)
+    assert r.returncode == 0 and "Usage:" in r.stderr          # no arguments: usage, exit 0
+    assert "Usage:" in run(app, "-h").stderr
+    r = run(app, "-i", str(tmp_path / "missing.gr"))
+    assert r.returncode == 1 and "Error: NULL file pointer." in r.stderr
+    bad = tmp_path / "g.gr"
+    bad.write_text("p sp 1 0\n")
+    r = run(app, "-i", str(bad), "-f", "9")
+    assert r.returncode == 1 and "Wrong code for dataset" in r.stderr
+
+
+def test_smdp_gen_writes_dimacs(tmp_path):
+    out = tmp_path / "u.gr"
+    r = run("smdp-gen", "uniform", "1000", "8", "1", "100", "1", str(out))
+    assert r.returncode == 0
+    text = out.read_text()
+    assert "p sp 1000 8000" in text and "s 1" in text
+
+
+def _gpu_run(app, path, *extra):
+    r = run(app, "-i", path, "-f", "0", *extra)
+    assert r.returncode == 0, r.stderr
+    return r
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sol", ["0", "1", "2", "3", "4", "6"])
+def test_gpu_sssp_cli(golden, data_test_path, sol):
+    r = _gpu_run("gpu-sssp", data_test_path, "-s", sol, "-d")
+    got = re.findall(r"From Node (\d+) -> (\d+) is (\d+)", r.stdout)
+    assert len(got) == 20 and all(int(a) == golden["source"] + 1 for a, _, _ in got)
+    assert [int(c) for _, _, c in got] == golden["dist"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sol", ["0", "1", "2", "3", "4", "5", "6"])
+def test_gpu_bfs_cli(golden, data_test_path, sol):
+    r = _gpu_run("gpu-bfs-rec", data_test_path, "-s", sol, "-d")
+    assert "PASS: GPU bfs rec !" in r.stdout
+    lv = [int(x) for x in re.findall(r"Node \d+ is in level (\d+)", r.stdout)]
+    assert lv == golden["level"]
+
+
+@pytest.mark.gpu
+def test_gpu_color_cli(golden, data_test_path):
+    r = _gpu_run("gpu-graph-color", data_test_path, "-s", "3", "-d")
+    assert [int(x) for x in re.findall(r"Node \d+: (\d+)", r.stdout)] == golden["color"]
+
+
+@pytest.mark.gpu
+def test_gpu_spmv_and_pagerank_cli(data_test_path):
+    r = _gpu_run("gpu-SpMV", data_test_path, "-s", "4", "-v")
+    err = float(re.search(r"L-inf error vs FP64: (\S+)", r.stdout).group(1))
+    assert err == 0.0  # the reference's inputs make every sum exact
+    r = _gpu_run("gpu-pg", data_test_path, "-s", "6", "-d")
+    ranks = np.array([float(x) for x in re.findall(r"Node \d+: (\S+)", r.stdout)])
+    assert len(ranks) == 20 and abs(ranks.sum() - 1000.0) < 1e-2
