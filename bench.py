@@ -23,6 +23,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [os.path.join(ROOT, "show-me-dynamic-parallelism_amd"), os.path.join(ROOT, "oracle")]
 
 HBM_PEAK_GBS = 8000.0
+# Autotuner results for this library on MI355X, keyed by arch and degree
+# histogram; a miss runs the search and adds the entry.
+TUNE_CACHE = os.path.join(ROOT, "show-me-dynamic-parallelism_amd", "tune_cache.json")
 METRIC = "GTEPS (SSSP, PageRank) on RMAT-22, 1 MI355X; % HBM roofline; batched vs plain"
 
 
@@ -46,8 +49,7 @@ def sssp_section(ctx, g, dg, args, smdp, orc, np):
     src = g.source
     sol = smdp.SOL_TUNED
     t0 = time.time()
-    params = dg.autotune(smdp.APP_SSSP, src, os.path.join(ROOT, "gpurun_out", "tune_cache.json")) \
-        if args.tune else ctx.get_params(smdp.APP_SSSP)
+    params = dg.autotune(smdp.APP_SSSP, src, TUNE_CACHE) if args.tune else ctx.get_params(smdp.APP_SSSP)
     log(f"sssp: params {params.as_dict()} (tuning {time.time() - t0:.1f}s)")
     for _ in range(args.warmup):
         dg.sssp(src, sol)
@@ -122,6 +124,8 @@ def pagerank_section(ctx, g, args, smdp, orc, np):
     with smdp.Graph(ctx, smdp.HostGraph(ig.row, ig.col)) as dg:
         dg.set_outdeg(od)
         sol = smdp.SOL_TUNED
+        if args.tune:
+            dg.autotune(smdp.APP_PAGERANK, 0, TUNE_CACHE)
         for _ in range(max(1, args.warmup)):
             dg.pagerank(opts, sol)
         t0 = time.perf_counter()
@@ -135,14 +139,17 @@ def pagerank_section(ctx, g, args, smdp, orc, np):
     iters = int(st.rounds)
     gteps = g.m * iters / (ms * 1e6)
     per_iter = 8 * g.m + 20 * g.n
-    ach = per_iter * pst.dominant_launches / (pst.ms_dominant * 1e-3) / 1e9 if pst.ms_dominant > 0 else 0.0
+    # every launch after the stop rule fired returns at once: bytes of the
+    # iterations run, over the summed duration of all launches
+    ach = per_iter * iters / (pst.ms_dominant * 1e-3) / 1e9 if pst.ms_dominant > 0 else 0.0
     out = {"gteps": round(gteps, 3), "ms_per_solve": round(ms, 4), "iterations": iters,
            "residual": st.residual, "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
                                                  "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                                                 "kernel": "k_pr_iter", "traffic": traffic_for("k_pr_iter"),
+                                                 "kernel": "k_rows_tuned<true>",
+                                                 "traffic": traffic_for("k_rows_tuned<true>"),
                                                  "launches": pst.dominant_launches,
                                                  "avg_launch_us": round(pst.ms_dominant * 1e3 / max(1, pst.dominant_launches), 3),
-                                                 "algorithmic_bytes_per_launch": per_iter}}
+                                                 "algorithmic_bytes_per_launch": int(per_iter * iters / max(1, pst.dominant_launches))}}
     if not args.skip_check:
         ref, _, _ = orc.pagerank(ig.row, ig.col, od, d=0.85, T=1.0, fixed_iters=iters)
         out["l1_vs_fp64_iteration_matched"] = float(np.abs(r.astype(np.float64) - ref).sum())
@@ -227,13 +234,27 @@ def only_section(args, smdp, orc, np):
         g = smdp.gen_rmat(args.scale, 16, wlo=1, whi=255, seed=3)
         if args.only == "sssp":
             with smdp.Graph(ctx, g) as dg:
+                if args.tune and not args.params:
+                    dg.autotune(smdp.APP_SSSP, g.source, TUNE_CACHE)
                 for _ in range(args.warmup + args.steps):
                     st = dg.sssp(g.source, smdp.SOL_TUNED)
-                log(f"sssp: {st.ms_kernel:.3f} ms, {st.rounds} rounds, {st.edges_relaxed} arcs")
+                log(f"sssp: {st.ms_kernel:.3f} ms, {st.rounds} rounds, {st.edges_relaxed} arcs, "
+                    f"params {ctx.get_params(smdp.APP_SSSP).as_dict()}")
+                if args.round_log:
+                    ctx.set_profiling(True)
+                    dg.sssp(g.source, smdp.SOL_TUNED)
+                    q, arcs, ms = ctx.round_log()
+                    ctx.set_profiling(False)
+                    for i in range(len(q)):
+                        if q[i] or arcs[i]:
+                            log(f"  round {i}: queue {q[i]} arcs {arcs[i]} {ms[i]:.3f} ms "
+                                f"{arcs[i] / max(ms[i], 1e-9) / 1e6:.1f} Garcs/s")
         else:
             ig = smdp.invert(g)
             with smdp.Graph(ctx, smdp.HostGraph(ig.row, ig.col)) as dg:
                 dg.set_outdeg(g.outdegree())
+                if args.tune and not args.params and args.col_blocks == 0:
+                    dg.autotune(smdp.APP_PAGERANK, 0, TUNE_CACHE)
                 for _ in range(args.warmup + args.steps):
                     st = dg.pagerank(smdp.pr_opts(0.85, 1.0, 1e-6, 100), smdp.SOL_TUNED)
                 log(f"pagerank: {st.ms_kernel:.3f} ms, {st.rounds} iterations")
@@ -256,6 +277,7 @@ def main():
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--col-blocks", type=int, default=0, help="SpMV/PageRank layout: 0 auto, 1 plain, 8 blocked")
     ap.add_argument("--params", default="", help="profiling runs: JSON object of smdp_params fields to set")
+    ap.add_argument("--round-log", action="store_true", help="profiling runs: print SSSP per-round log")
     ap.add_argument("--only", choices=["sssp", "pagerank", "batched", "copy"],
                     help="profiling runs: one section only, no JSON contract line")
     args = ap.parse_args()

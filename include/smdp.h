@@ -141,8 +141,13 @@ int         smdp_set_params(smdp_ctx* ctx, int app, const smdp_params* p);
 /* name: device name; cus: compute units; arch: e.g. "gfx950". */
 int         smdp_device_info(smdp_ctx* ctx, char* name, int name_len, char* arch, int arch_len,
                              int32_t* cus);
-/* 1: record per-launch HIP events for the dominant kernel (fills ms_dominant). */
+/* 1: record per-launch HIP events for the dominant kernel (fills ms_dominant)
+ * and, for SSSP/BFS, a per-round log of queue lengths and arcs scanned. */
 int         smdp_set_profiling(smdp_ctx* ctx, int on);
+/* Per-round log of the last SSSP/BFS call made with profiling on: queue
+ * length, arcs scanned and device time (phase 1 + phase 2) of round i.
+ * Returns the number of rounds logged (at most cap), or a negative error. */
+int         smdp_round_log(smdp_ctx* ctx, int64_t* qlen, int64_t* arcs, double* ms, int32_t cap);
 
 /* apps, for smdp_get/set_params and the autotuner */
 #define SMDP_APP_SSSP     0

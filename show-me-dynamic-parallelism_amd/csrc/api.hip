@@ -8,6 +8,7 @@
 
 #include <chrono>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 
 #include "smdp_internal.h"
@@ -168,6 +169,7 @@ void smdp_destroy(smdp_ctx* c) {
   (void)hipSetDevice(c->device);
   prof_reset(c);
   if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->rlog) (void)hipFree(c->rlog);
   if (c->ev_begin) (void)hipEventDestroy(c->ev_begin);
   if (c->ev_end) (void)hipEventDestroy(c->ev_end);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -212,9 +214,22 @@ int smdp_device_info(smdp_ctx* ctx, char* name, int name_len, char* arch, int ar
 
 int smdp_set_profiling(smdp_ctx* ctx, int on) {
   if (!ctx) return SMDP_ERR_ARG;
+  SMDP_HIP(hipSetDevice(ctx->device));
+  if (on && !ctx->rlog) SMDP_CHECK(alloc_once(&ctx->rlog, 2 * (int64_t)kRoundLogCap));
   ctx->profiling = on ? 1 : 0;
   prof_reset(ctx);
   return SMDP_OK;
+}
+
+int smdp_round_log(smdp_ctx* ctx, int64_t* qlen, int64_t* arcs, double* ms, int32_t cap) {
+  if (!ctx || cap < 0) return SMDP_ERR_ARG;
+  const int32_t k = std::min<int32_t>(cap, (int32_t)ctx->log_qlen.size());
+  for (int32_t i = 0; i < k; ++i) {
+    if (qlen) qlen[i] = ctx->log_qlen[i];
+    if (arcs) arcs[i] = ctx->log_arcs[i];
+    if (ms) ms[i] = i < (int32_t)ctx->log_ms.size() ? ctx->log_ms[i] : 0.0;
+  }
+  return k;
 }
 
 int smdp_graph_create(smdp_ctx* ctx, const smdp_csr* h, smdp_graph** out) {

@@ -11,6 +11,14 @@ namespace smdp {
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
+// Streaming load: CSR arrays are read once per pass, so they are loaded with
+// the non-temporal hint and do not push the randomly gathered vectors (dist,
+// x, PageRank contributions) out of L2.
+template <typename T>
+__device__ __forceinline__ T stream_load(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
+
 // Number of set bits of `mask` below this lane (v_mbcnt_lo/hi).
 __device__ __forceinline__ int mbcnt(uint64_t mask) {
   return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),

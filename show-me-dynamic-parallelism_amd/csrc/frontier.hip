@@ -68,11 +68,19 @@ struct FArgs {
   Counters* cnt;
   int32_t heavy;  // lane solutions: heavy degree; tuned: split degree
   int32_t chunk;  // grid-queue chunk in edges; 0 = whole vertex
+  unsigned long long* rlog;  // profiling: per round (queue length, arcs), or nullptr
 };
+
+// Profiling log: round r's queue length and arcs (one atomic per wave).
+__device__ __forceinline__ void log_round(const FArgs& a, int r, int L, unsigned long long wave_arcs) {
+  if (!a.rlog || r >= kRoundLogCap) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && L >= 0) a.rlog[2 * r] = (unsigned long long)L;
+  if (lane_id() == 0 && wave_arcs) atomicAdd(&a.rlog[2 * r + 1], wave_arcs);
+}
 
 template <bool BFS>
 __device__ __forceinline__ int arc_weight(const FArgs& a, int e) {
-  return BFS ? 1 : a.w[e];
+  return BFS ? 1 : stream_load(a.w + e);
 }
 
 // Relax one arc with candidate `alt`.  Returns true when this lane lowered
@@ -158,7 +166,7 @@ __device__ __forceinline__ void wg_expand(const FArgs& a, int s, int e, int du, 
 #pragma unroll
     for (int t = 0; t < kIlp; ++t) k[t] = min(b + (int)threadIdx.x + t * SMDP_WG, e - 1);
 #pragma unroll
-    for (int t = 0; t < kIlp; ++t) v[t] = a.col[k[t]];
+    for (int t = 0; t < kIlp; ++t) v[t] = stream_load(a.col + k[t]);
 #pragma unroll
     for (int t = 0; t < kIlp; ++t) alt[t] = du + arc_weight<BFS>(a, k[t]);
     int seen[kIlp];
@@ -321,6 +329,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_expand_lane(FArgs a, int r) {
       }
     }
   }
+  log_round(a, r, L, wave_sum(edges));
   flush_counters(a.cnt, edges, verts, out.pushes, pw, pb, pg, out);
 }
 
@@ -399,7 +408,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_expand_tile(FArgs a, int r) {
 #pragma unroll
       for (int t = 0; t < kIlp; ++t) k[t] = s_start[own[t]] + (j[t] - s_off[own[t]]);
 #pragma unroll
-      for (int t = 0; t < kIlp; ++t) v[t] = a.col[k[t]];
+      for (int t = 0; t < kIlp; ++t) v[t] = stream_load(a.col + k[t]);
 #pragma unroll
       for (int t = 0; t < kIlp; ++t) alt[t] = s_du[own[t]] + arc_weight<BFS>(a, k[t]);
       int seen[kIlp];
@@ -416,6 +425,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_expand_tile(FArgs a, int r) {
     __syncthreads();
     }
   }
+  log_round(a, r, L, wave_sum(edges));
   flush_counters(a.cnt, edges, verts, out.pushes, 0, 0, pg, out);
 }
 
@@ -445,6 +455,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_drain(FArgs a, int r) {
     const int end = (a.chunk > 0 && vend - c.first > a.chunk) ? c.first + a.chunk : vend;
     wg_expand<BFS>(a, c.first, end, a.dist[c.v], label, edges, out);
   }
+  log_round(a, r, -1, wave_sum(edges));
   flush_counters(a.cnt, edges, 0, out.pushes, 0, 0, 0, out);
   unsigned long long e2 = wave_sum(edges);
   if (lane_id() == 0 && e2) atomicAdd(&a.cnt->edges2, e2);
@@ -492,6 +503,30 @@ static int ensure_frontier(smdp_graph* g, int32_t chunk) {
   return ensure(&g->hq, &g->hq_cap, need);
 }
 
+// Copies the per-round log and pairs it with the recorded launch events
+// (phase 1 and, for two-phase solutions, phase 2 of each round).
+static int collect_round_log(smdp_ctx* ctx, int rounds, bool two_phase) {
+  const int k = std::min(rounds, kRoundLogCap);
+  std::vector<unsigned long long> h(2 * (size_t)k);
+  if (k) SMDP_HIP(hipMemcpy(h.data(), ctx->rlog, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost));
+  ctx->log_qlen.assign(k, 0);
+  ctx->log_arcs.assign(k, 0);
+  ctx->log_ms.assign(k, 0.0);
+  const size_t per_round = two_phase ? 4 : 2;  // events per round
+  for (int i = 0; i < k; ++i) {
+    ctx->log_qlen[i] = (int64_t)h[2 * i];
+    ctx->log_arcs[i] = (int64_t)h[2 * i + 1];
+    for (size_t e = 0; e + 1 < per_round; e += 2) {
+      const size_t at = i * per_round + e;
+      if (at + 1 >= ctx->prof_ev.size()) break;
+      float t = 0;
+      SMDP_HIP(hipEventElapsedTime(&t, ctx->prof_ev[at], ctx->prof_ev[at + 1]));
+      ctx->log_ms[i] += t;
+    }
+  }
+  return SMDP_OK;
+}
+
 // Runs SSSP (bfs = false) or BFS from src with the given solution; results stay
 // in g->dist.
 int run_frontier(smdp_graph* g, int32_t src, int sol, bool bfs, smdp_stats* st) {
@@ -519,6 +554,8 @@ int run_frontier(smdp_graph* g, int32_t src, int sol, bool bfs, smdp_stats* st) 
   a.cnt = g->cnt;
   a.heavy = tuned ? p.split_threshold : (sol == SMDP_SOL_HIER ? 1 : p.heavy_threshold);
   a.chunk = tuned ? p.chunk : 0;
+  a.rlog = ctx->profiling ? ctx->rlog : nullptr;
+  if (a.rlog) SMDP_HIP(hipMemsetAsync(a.rlog, 0, sizeof(unsigned long long) * 2 * kRoundLogCap, ctx->stream));
   const int grid = grid_size(ctx, p);
   const int32_t inf = bfs ? SMDP_BFS_UNDEF : SMDP_SSSP_INF;
   const bool two_phase = sol == SMDP_SOL_GRID || tuned;
@@ -560,6 +597,7 @@ int run_frontier(smdp_graph* g, int32_t src, int sol, bool bfs, smdp_stats* st) 
                      g->n, inf, g->cnt);
   SMDP_HIP(hipGetLastError());
   g->last_res = bfs ? SMDP_RES_LEVEL : SMDP_RES_DIST;
+  if (a.rlog) SMDP_CHECK(collect_round_log(ctx, r, two_phase));
   if (st) {
     SMDP_CHECK(fill_stats(g, st, ms, launches));
     SMDP_CHECK(prof_collect(ctx, st));

@@ -176,11 +176,11 @@ __device__ __forceinline__ CSum wg_row(const RArgs& a, int s, int e, float2* lds
     int c[4];
     float g[4], va[4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) c[t] = a.col[k + t * SMDP_WG];
+    for (int t = 0; t < 4; ++t) c[t] = stream_load(a.col + k + t * SMDP_WG);
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       g[t] = a.x[c[t]];
-      va[t] = PR ? 0.f : a.val[k + t * SMDP_WG];
+      va[t] = PR ? 0.f : stream_load(a.val + k + t * SMDP_WG);
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
@@ -464,7 +464,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_rows_tuned(RArgs a, int k) {
 #pragma unroll
         for (int i = 0; i < K; ++i) q[i] = s_start[own[i]] + (j[i] - s_off[own[i]]);
 #pragma unroll
-        for (int i = 0; i < K; ++i) c[i] = a.col[q[i]];
+        for (int i = 0; i < K; ++i) c[i] = stream_load(a.col + q[i]);
         if (PR) {
           float g[K];
 #pragma unroll
@@ -477,7 +477,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_rows_tuned(RArgs a, int k) {
           float va[K], xb[K];
 #pragma unroll
           for (int i = 0; i < K; ++i) {
-            va[i] = a.val[q[i]];
+            va[i] = stream_load(a.val + q[i]);
             xb[i] = a.x[c[i]];
           }
 #pragma unroll

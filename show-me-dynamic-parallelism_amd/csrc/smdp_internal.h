@@ -57,7 +57,14 @@ struct smdp_ctx {
   std::vector<hipEvent_t> prof_ev;  // pairs of events around profiled launches
   std::vector<int> prof_kind;       // kernel kind of each pair
   int32_t* pinned = nullptr;        // small host readback buffer (hipHostMalloc)
+  unsigned long long* rlog = nullptr;  // per-round (queue length, arcs) when profiling
+  std::vector<int64_t> log_qlen, log_arcs;
+  std::vector<double> log_ms;
 };
+
+namespace smdp {
+constexpr int kRoundLogCap = 4096;
+}
 
 struct smdp_graph {
   smdp_ctx* ctx = nullptr;

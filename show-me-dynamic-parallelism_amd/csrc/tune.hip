@@ -205,15 +205,17 @@ extern "C" int smdp_autotune(smdp_graph* g, int app, int32_t src, const char* ca
   double best_ms = 1e30;
   for (const smdp_params& p : candidates(app, base)) {
     SMDP_CHECK(smdp_set_params(ctx, app, &p));
-    double ms0 = 0, ms1 = 0;
-    int rc = time_once(g, app, src, &ms0);  // first run also allocates and plans
-    if (rc == SMDP_OK) rc = time_once(g, app, src, &ms1);
+    // first run also allocates and plans; then the median of three
+    double ms[4] = {0, 0, 0, 0};
+    int rc = SMDP_OK;
+    for (int i = 0; i < 4 && rc == SMDP_OK; ++i) rc = time_once(g, app, src, &ms[i]);
     if (rc != SMDP_OK) {
       smdp_set_params(ctx, app, &base);
       return rc;
     }
-    if (ms1 < best_ms) {
-      best_ms = ms1;
+    std::sort(ms + 1, ms + 4);
+    if (ms[2] < best_ms) {
+      best_ms = ms[2];
       best = ctx->params[app];
     }
   }

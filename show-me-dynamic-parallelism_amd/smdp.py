@@ -97,6 +97,7 @@ def _declare(L):
         "smdp_set_params": (C.c_int, [vp, C.c_int, P(Params)]),
         "smdp_device_info": (C.c_int, [vp, cp, C.c_int, cp, C.c_int, P(i32)]),
         "smdp_set_profiling": (C.c_int, [vp, C.c_int]),
+        "smdp_round_log": (C.c_int, [vp, vp, vp, vp, i32]),
         "smdp_sssp": (C.c_int, [vp, P(Csr), i32, C.c_int, vp, P(Stats)]),
         "smdp_bfs": (C.c_int, [vp, P(Csr), i32, C.c_int, vp, P(Stats)]),
         "smdp_spmv_f32": (C.c_int, [vp, P(Csr), vp, vp, C.c_int, vp, P(Stats)]),
@@ -307,6 +308,15 @@ class Context:
 
     def set_profiling(self, on):
         _check(lib().smdp_set_profiling(self.h, int(on)), "smdp_set_profiling")
+
+    def round_log(self, cap=4096):
+        q = np.zeros(cap, np.int64)
+        a = np.zeros(cap, np.int64)
+        t = np.zeros(cap, np.float64)
+        k = lib().smdp_round_log(self.h, _ptr(q), _ptr(a), _ptr(t), cap)
+        if k < 0:
+            raise SmdpError(k, "smdp_round_log")
+        return q[:k], a[:k], t[:k]
 
     # drop-in calls ---------------------------------------------------------
     def sssp(self, g, src, sol):
