@@ -239,15 +239,22 @@ __global__ __launch_bounds__(SMDP_WG) void k_mg_reset(int32_t* next, Counters* c
   for (int i = threadIdx.x; i < (int)(sizeof(Counters) / 8); i += blockDim.x) p[i] = 0;
 }
 
-// One workgroup solves whole graphs, one after another.
-__global__ __launch_bounds__(SMDP_WG) void k_mg(GArgs a) {
+// One workgroup of NT threads solves whole graphs, one after another.  Each
+// round is an edge-balanced pass over the queue in tiles of NT entries; every
+// thread relaxes kMgIlp arcs per step with all loads issued before the atomics.
+constexpr int kMgIlp = 4;
+constexpr int kMgThreads = 1024;
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_mg(GArgs a) {
+  constexpr int W = NT / SMDP_WAVE;
   __shared__ int s_g;
   __shared__ int32_t s_qlen[2];
-  __shared__ int s_start[SMDP_WG];
-  __shared__ int s_du[SMDP_WG];
-  __shared__ int s_off[SMDP_WG];
-  __shared__ int s_scan[SMDP_WAVES_PER_WG];
-  __shared__ int32_t s_out[SMDP_WAVES_PER_WG * kMgOut];
+  __shared__ int s_start[NT];
+  __shared__ int s_du[NT];
+  __shared__ int s_off[NT];
+  __shared__ int s_scan[W];
+  __shared__ int32_t s_out[W * kMgOut];
   const int wave = threadIdx.x / SMDP_WAVE;
   unsigned long long edges = 0, verts = 0;
   for (;;) {
@@ -265,7 +272,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_mg(GArgs a) {
     int32_t* stamp = a.stamp + a.vbase[g];
     int32_t* q[2] = {a.qa + a.vbase[g], a.qb + a.vbase[g]};
     const int src = a.src[g];
-    for (int i = threadIdx.x; i < n; i += SMDP_WG) {
+    for (int i = threadIdx.x; i < n; i += NT) {
       dist[i] = i == src ? 0 : SMDP_SSSP_INF;
       stamp[i] = i == src ? 0 : -1;
     }
@@ -288,7 +295,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_mg(GArgs a) {
       WaveOut<kMgOut> out;
       out.init(s_out + wave * kMgOut);
       int32_t* qout = q[cur ^ 1];
-      for (int base = 0; base < L; base += SMDP_WG) {
+      for (int base = 0; base < L; base += NT) {
         const int i = base + (int)threadIdx.x;
         int s = 0, deg = 0, du = 0;
         if (i < L) {
@@ -304,30 +311,33 @@ __global__ __launch_bounds__(SMDP_WG) void k_mg(GArgs a) {
         s_start[threadIdx.x] = s;
         s_du[threadIdx.x] = du;
         int tot;
-        const int ex = wg_exclusive_scan(deg, s_scan, &tot);
+        const int ex = wg_exclusive_scan_n<NT>(deg, s_scan, &tot);
         s_off[threadIdx.x] = ex;
         __syncthreads();
-        for (int jb = 0; jb < tot; jb += SMDP_WG) {
-          const int j = jb + (int)threadIdx.x;
-          int v = 0;
-          bool p = false;
-          if (j < tot) {
-            int lo = 0, hi = SMDP_WG - 1;
-            while (lo < hi) {
-              int mid = (lo + hi + 1) >> 1;
-              if (s_off[mid] <= j) lo = mid;
-              else hi = mid - 1;
+        for (int jb = 0; jb < tot; jb += kMgIlp * NT) {
+          int j[kMgIlp], own[kMgIlp], k[kMgIlp], v[kMgIlp], alt[kMgIlp], seen[kMgIlp];
+#pragma unroll
+          for (int t = 0; t < kMgIlp; ++t) j[t] = min(jb + (int)threadIdx.x + t * NT, tot - 1);
+          tile_owners<kMgIlp, NT>(s_off, j, own);
+#pragma unroll
+          for (int t = 0; t < kMgIlp; ++t) k[t] = s_start[own[t]] + (j[t] - s_off[own[t]]);
+#pragma unroll
+          for (int t = 0; t < kMgIlp; ++t) v[t] = stream_load(a.col + k[t]);
+#pragma unroll
+          for (int t = 0; t < kMgIlp; ++t) alt[t] = s_du[own[t]] + stream_load(a.w + k[t]);
+#pragma unroll
+          for (int t = 0; t < kMgIlp; ++t) seen[t] = dist[v[t]];
+#pragma unroll
+          for (int t = 0; t < kMgIlp; ++t) {
+            const bool ok = jb + (int)threadIdx.x + t * NT < tot;
+            bool p = false;
+            if (ok && alt[t] < seen[t]) {
+              const int old = atomicMin(&dist[v[t]], alt[t]);
+              if (alt[t] < old && stamp[v[t]] != label) p = atomicExch(&stamp[v[t]], label) != label;
             }
-            const int k = s_start[lo] + (j - s_off[lo]);
-            v = a.col[k];
-            const int alt = s_du[lo] + a.w[k];
-            if (alt < dist[v]) {
-              const int old = atomicMin(&dist[v], alt);
-              if (alt < old && stamp[v] != label) p = atomicExch(&stamp[v], label) != label;
-            }
-            ++edges;
+            edges += ok;
+            out.push(p, v[t], qout, &s_qlen[cur ^ 1]);
           }
-          out.push(p, v, qout, &s_qlen[cur ^ 1]);
         }
         __syncthreads();
       }
@@ -622,7 +632,8 @@ int smdp_batch_create(smdp_ctx* ctx, const smdp_csr* graphs, int32_t ng, const i
   a.cnt = b->d_cnt;
   a.ng = ng;
   const smdp_params& p = params_of(ctx, SMDP_APP_MSSSP);
-  b->grid = std::max(1, std::min<int>(ng, ctx->cus * std::max(1, p.wg_per_cu)));
+  // one 1024-thread workgroup per CU (two when a CU can hold them)
+  b->grid = std::max(1, std::min<int>(ng, ctx->cus * std::max(1, std::min(2, p.wg_per_cu))));
   // Capture reset + solve once; every run replays the graph.  No memset nodes:
   // counters are cleared by a kernel.
   hipStream_t s = ctx->stream;
@@ -630,7 +641,7 @@ int smdp_batch_create(smdp_ctx* ctx, const smdp_csr* graphs, int32_t ng, const i
   if (rc == SMDP_OK) {
     hipLaunchKernelGGL(k_mg_reset, dim3(1), dim3(64), 0, s, b->d_next, b->d_cnt);
     hipLaunchKernelGGL(k_mg_status_clear, dim3(std::max(1, (ng + 255) / 256)), dim3(256), 0, s, b->d_status, ng);
-    hipLaunchKernelGGL(k_mg, dim3(b->grid), dim3(SMDP_WG), 0, s, b->args);
+    hipLaunchKernelGGL(k_mg<kMgThreads>, dim3(b->grid), dim3(kMgThreads), 0, s, b->args);
     if (hipStreamEndCapture(s, &b->graph) != hipSuccess) rc = SMDP_ERR_DEVICE;
   }
   if (rc == SMDP_OK && hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess)

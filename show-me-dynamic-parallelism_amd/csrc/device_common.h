@@ -95,12 +95,12 @@ struct WaveOut {
 // SMDP_WG-entry non-decreasing array off[] with off[i] <= j (off[0] == 0).
 // A fixed 8-step search, interleaved over the K slots so their LDS reads
 // overlap instead of forming one dependent chain each.
-template <int K>
+template <int K, int N = SMDP_WG>
 __device__ __forceinline__ void tile_owners(const int* off, const int (&j)[K], int (&own)[K]) {
 #pragma unroll
   for (int k = 0; k < K; ++k) own[k] = 0;
 #pragma unroll
-  for (int step = SMDP_WG / 2; step > 0; step >>= 1) {
+  for (int step = N / 2; step > 0; step >>= 1) {
 #pragma unroll
     for (int k = 0; k < K; ++k)
       if (off[own[k] + step] <= j[k]) own[k] += step;
@@ -142,6 +142,27 @@ __device__ __forceinline__ int wg_exclusive_scan(int x, int* lds, int* total) {
   int before = 0, sum = 0;
 #pragma unroll
   for (int k = 0; k < SMDP_WAVES_PER_WG; ++k) {
+    int v = lds[k];
+    before += (k < wave) ? v : 0;
+    sum += v;
+  }
+  __syncthreads();
+  *total = sum;
+  return before + inc - x;
+}
+
+// Exclusive scan over a workgroup of NT threads (NT a multiple of 64); `lds`
+// needs NT/64 ints.  Contains barriers: every thread must call it.
+template <int NT>
+__device__ __forceinline__ int wg_exclusive_scan_n(int x, int* lds, int* total) {
+  constexpr int W = NT / SMDP_WAVE;
+  const int wave = threadIdx.x / SMDP_WAVE;
+  int inc = wave_inclusive_scan(x);
+  if (lane_id() == SMDP_WAVE - 1) lds[wave] = inc;
+  __syncthreads();
+  int before = 0, sum = 0;
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
     int v = lds[k];
     before += (k < wave) ? v : 0;
     sum += v;
