@@ -139,17 +139,34 @@ def pagerank_section(ctx, g, args, smdp, orc, np):
     iters = int(st.rounds)
     gteps = g.m * iters / (ms * 1e6)
     per_iter = 8 * g.m + 20 * g.n
+    params = ctx.get_params(smdp.APP_PAGERANK)
+    binned = params.layout == smdp.LAYOUT_BINNED or (params.layout == smdp.LAYOUT_AUTO and g.m >= 1 << 20)
     # every launch after the stop rule fired returns at once: bytes of the
-    # iterations run, over the summed duration of all launches
-    ach = per_iter * iters / (pst.ms_dominant * 1e-3) / 1e9 if pst.ms_dominant > 0 else 0.0
+    # iterations run, over the summed duration of all launches.  The binned
+    # iteration is two kernels (scatter, reduce); the roofline covers both.
+    if binned:
+        kernel = "k_bin_scatter + k_bin_reduce"
+        ms_k = pst.ms_dominant + pst.ms_phase2
+        launches = pst.dominant_launches
+        tr = [traffic_for("k_bin_scatter"), traffic_for("k_bin_reduce")]
+        traffic = None if None in tr else tr[0] + tr[1]
+    else:
+        kernel = "k_rows_tuned<true>"
+        ms_k = pst.ms_dominant
+        launches = pst.dominant_launches
+        traffic = traffic_for(kernel)
+    ach = per_iter * iters / (ms_k * 1e-3) / 1e9 if ms_k > 0 else 0.0
     out = {"gteps": round(gteps, 3), "ms_per_solve": round(ms, 4), "iterations": iters,
-           "residual": st.residual, "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
-                                                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                                                 "kernel": "k_rows_tuned<true>",
-                                                 "traffic": traffic_for("k_rows_tuned<true>"),
-                                                 "launches": pst.dominant_launches,
-                                                 "avg_launch_us": round(pst.ms_dominant * 1e3 / max(1, pst.dominant_launches), 3),
-                                                 "algorithmic_bytes_per_launch": int(per_iter * iters / max(1, pst.dominant_launches))}}
+           "residual": st.residual, "layout": "binned" if binned else params.layout,
+           "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "kernel": kernel,
+                        "traffic": traffic, "launches": launches,
+                        "avg_launch_us": round(ms_k * 1e3 / max(1, launches), 3),
+                        "algorithmic_bytes_per_launch": int(per_iter * iters / max(1, launches))},
+           "kernels": [{"kernel": "k_bin_reduce" if binned else kernel, "ms_total": round(pst.ms_dominant, 4),
+                        "launches": pst.dominant_launches},
+                       {"kernel": "k_bin_scatter" if binned else "phase 2", "ms_total": round(pst.ms_phase2, 4),
+                        "launches": pst.phase2_launches}]}
     if not args.skip_check:
         ref, _, _ = orc.pagerank(ig.row, ig.col, od, d=0.85, T=1.0, fixed_iters=iters)
         out["l1_vs_fp64_iteration_matched"] = float(np.abs(r.astype(np.float64) - ref).sum())
@@ -225,7 +242,7 @@ def copy_calibration(nbytes=1 << 28, reps=4):
 
 def only_section(args, smdp, orc, np):
     ctx = smdp.Context(0)
-    ctx.set_params(smdp.APP_PAGERANK, col_blocks=args.col_blocks)
+    ctx.set_params(smdp.APP_PAGERANK, layout=args.layout)
     if args.params:
         kw = json.loads(args.params)
         for app in (smdp.APP_SSSP, smdp.APP_PAGERANK):
@@ -253,11 +270,17 @@ def only_section(args, smdp, orc, np):
             ig = smdp.invert(g)
             with smdp.Graph(ctx, smdp.HostGraph(ig.row, ig.col)) as dg:
                 dg.set_outdeg(g.outdegree())
-                if args.tune and not args.params and args.col_blocks == 0:
+                if args.tune and not args.params and args.layout == 0:
                     dg.autotune(smdp.APP_PAGERANK, 0, TUNE_CACHE)
+                t0 = time.time()
                 for _ in range(args.warmup + args.steps):
                     st = dg.pagerank(smdp.pr_opts(0.85, 1.0, 1e-6, 100), smdp.SOL_TUNED)
-                log(f"pagerank: {st.ms_kernel:.3f} ms, {st.rounds} iterations")
+                ctx.set_profiling(True)
+                pst = dg.pagerank(smdp.pr_opts(0.85, 1.0, 1e-6, 100), smdp.SOL_TUNED)
+                ctx.set_profiling(False)
+                log(f"pagerank: {st.ms_kernel:.3f} ms, {st.rounds} iterations, kernels {pst.ms_dominant:.3f} ms "
+                    f"({pst.dominant_launches} launches) + {pst.ms_phase2:.3f} ms ({pst.phase2_launches}), "
+                    f"params {ctx.get_params(smdp.APP_PAGERANK).as_dict()}, {time.time() - t0:.1f} s with set-up")
     else:
         log(json.dumps(batched_section(ctx, args, smdp, orc, np)))
     ctx.close()
@@ -275,7 +298,8 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-pagerank", action="store_true")
     ap.add_argument("--skip-batched", action="store_true")
-    ap.add_argument("--col-blocks", type=int, default=0, help="SpMV/PageRank layout: 0 auto, 1 plain, 8 blocked")
+    ap.add_argument("--layout", type=int, default=0,
+                    help="PageRank arc layout: 0 auto, 1 plain CSR, 2 binned, 8 XCD blocks")
     ap.add_argument("--params", default="", help="profiling runs: JSON object of smdp_params fields to set")
     ap.add_argument("--round-log", action="store_true", help="profiling runs: print SSSP per-round log")
     ap.add_argument("--only", choices=["sssp", "pagerank", "batched", "copy"],

@@ -91,9 +91,16 @@ typedef struct {
   int32_t wg_per_cu;        /* persistent workgroups per compute unit */
   int32_t batch_rounds;     /* rounds enqueued between two host checks of the queue length */
   int32_t max_rounds;       /* safety cap on rounds (SMDP_ERR_OVERFLOW when hit) */
-  int32_t col_blocks;       /* SpMV/PageRank tuned: 1 = plain CSR; 8 = columns split into 8 blocks, one per
-                               XCD, so each XCD gathers from a slice that fits its L2; 0 = automatic */
+  int32_t layout;           /* SpMV/PageRank tuned: arc layout, one of SMDP_LAYOUT_* (0 = automatic) */
 } smdp_params;
+
+/* Arc layouts of the tuned SpMV/PageRank solution (smdp_params.layout). */
+#define SMDP_LAYOUT_AUTO    0  /* PageRank: BINNED when the graph has at least 2^20 arcs, else CSR; SpMV: CSR */
+#define SMDP_LAYOUT_CSR     1  /* gather from the in-CSR as given */
+#define SMDP_LAYOUT_BINNED  2  /* PageRank only: arcs regrouped once per graph by (source chunk, row bin); each
+                                  iteration is two streaming passes through LDS instead of random gathers */
+#define SMDP_LAYOUT_XCD8    8  /* sources split into 8 blocks, one per XCD, so each XCD gathers from a slice
+                                  that fits its L2 */
 
 /* Work counters and timings of the last call. */
 typedef struct {

@@ -193,7 +193,10 @@ int smdp_set_params(smdp_ctx* ctx, int app, const smdp_params* p) {
   if (q.batch_rounds <= 0) q.batch_rounds = d.batch_rounds;
   if (q.max_rounds <= 0) q.max_rounds = d.max_rounds;
   if (q.wg_per_cu > 64 || q.chunk < 64) return SMDP_ERR_ARG;
-  if (q.col_blocks != 0 && q.col_blocks != 1 && q.col_blocks != 8) return SMDP_ERR_ARG;
+  if (q.layout != SMDP_LAYOUT_AUTO && q.layout != SMDP_LAYOUT_CSR && q.layout != SMDP_LAYOUT_BINNED &&
+      q.layout != SMDP_LAYOUT_XCD8)
+    return SMDP_ERR_ARG;
+  if (q.layout == SMDP_LAYOUT_BINNED && app != SMDP_APP_PAGERANK) return SMDP_ERR_ARG;
   ctx->params[app] = q;
   return SMDP_OK;
 }

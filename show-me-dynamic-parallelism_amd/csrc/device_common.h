@@ -226,4 +226,74 @@ __device__ __forceinline__ CSum wave_csum(CSum x) {
   return x;
 }
 
+// ---- DPP reductions of compensated sums -----------------------------------
+// The shuffles move (sum, error) pairs with DPP or readlane, so no LDS is
+// touched.  CSum::merge is not commutative in the last bit of the error
+// term, so lanes of a group may end with slightly different values; every
+// result is read from one fixed lane, which makes it a fixed function of the
+// inputs.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ CSum dpp_merge(CSum x) {
+  CSum o;
+  o.s = dpp_f<CTRL>(x.s);
+  o.c = dpp_f<CTRL>(x.c);
+  x.merge(o);
+  return x;
+}
+constexpr int kDppXor1 = 0xB1;   // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4E;   // quad_perm [2,3,0,1]
+constexpr int kDppShr4 = 0x114;  // row_shr:4
+constexpr int kDppShr8 = 0x118;  // row_shr:8
+
+// Sum of each aligned group of 8 lanes, read from lane 4 of the group.
+__device__ __forceinline__ CSum group8_csum(CSum x) {
+  x = dpp_merge<kDppXor1>(x);
+  x = dpp_merge<kDppXor2>(x);
+  return dpp_merge<kDppShr4>(x);
+}
+
+// Sum over the wave; valid in every lane.
+__device__ __forceinline__ CSum wave_csum_dpp(CSum x) {
+  x = group8_csum(x);
+  x = dpp_merge<kDppShr8>(x);  // lane 15 of each row of 16 holds the row's sum
+  CSum t;
+#pragma unroll
+  for (int r = 0; r < SMDP_WAVE / 16; ++r) {
+    CSum o;
+    o.s = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x.s), 16 * r + 15));
+    o.c = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x.c), 16 * r + 15));
+    t.merge(o);
+  }
+  return t;
+}
+
+// FP64 sum over the wave with DPP steps; every lane returns the same value,
+// formed from the row sums in lanes 15, 31, 47, 63 in that order.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double x) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+__device__ __forceinline__ double wave_sum_dpp(double x) {
+  x += dpp_d<kDppXor1>(x);
+  x += dpp_d<kDppXor2>(x);
+  x += dpp_d<kDppShr4>(x);
+  x += dpp_d<kDppShr8>(x);
+  const uint64_t u = __builtin_bit_cast(uint64_t, x);
+  double t = 0.0;
+#pragma unroll
+  for (int r = 0; r < SMDP_WAVE / 16; ++r) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, 16 * r + 15);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), 16 * r + 15);
+    t += __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+  }
+  return t;
+}
+
 }  // namespace smdp

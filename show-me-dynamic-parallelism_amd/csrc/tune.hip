@@ -77,9 +77,9 @@ static std::map<std::string, smdp_params> load_cache(const char* path) {
     memset(&p, 0, sizeof p);
     if (sscanf(line,
                " \"%2047[^\"]\": {\"heavy_threshold\": %d, \"split_threshold\": %d, \"chunk\": %d, "
-               "\"wg_per_cu\": %d, \"batch_rounds\": %d, \"max_rounds\": %d, \"col_blocks\": %d}",
+               "\"wg_per_cu\": %d, \"batch_rounds\": %d, \"max_rounds\": %d, \"layout\": %d}",
                key, &p.heavy_threshold, &p.split_threshold, &p.chunk, &p.wg_per_cu, &p.batch_rounds,
-               &p.max_rounds, &p.col_blocks) == 8)
+               &p.max_rounds, &p.layout) == 8)
       out[key] = p;
   }
   fclose(f);
@@ -97,9 +97,9 @@ static void save_cache(const char* path, const std::map<std::string, smdp_params
     const smdp_params& p = kv.second;
     fprintf(f,
             "  \"%s\": {\"heavy_threshold\": %d, \"split_threshold\": %d, \"chunk\": %d, \"wg_per_cu\": %d, "
-            "\"batch_rounds\": %d, \"max_rounds\": %d, \"col_blocks\": %d}%s\n",
+            "\"batch_rounds\": %d, \"max_rounds\": %d, \"layout\": %d}%s\n",
             kv.first.c_str(), p.heavy_threshold, p.split_threshold, p.chunk, p.wg_per_cu, p.batch_rounds,
-            p.max_rounds, p.col_blocks, ++i < m.size() ? "," : "");
+            p.max_rounds, p.layout, ++i < m.size() ? "," : "");
   }
   fprintf(f, "}\n");
   fclose(f);
@@ -156,13 +156,18 @@ static std::vector<smdp_params> candidates(int app, const smdp_params& base) {
       break;
     case SMDP_APP_SPMV:
     case SMDP_APP_PAGERANK:
-      for (int blocks : {1, 8})
+      for (int layout : {SMDP_LAYOUT_CSR, SMDP_LAYOUT_XCD8})
         for (int heavy : {32, 128})
           for (int split : {2048, 8192})
             for (int wg : {4, 8}) {
               add(heavy, split, split, wg);
-              out.back().col_blocks = blocks;
+              out.back().layout = layout;
             }
+      // the binned layout has its own fixed launch shape
+      if (app == SMDP_APP_PAGERANK) {
+        add(base.heavy_threshold, base.split_threshold, base.chunk, base.wg_per_cu);
+        out.back().layout = SMDP_LAYOUT_BINNED;
+      }
       break;
     case SMDP_APP_COLOR:
       for (int heavy : {16, 64, 256})

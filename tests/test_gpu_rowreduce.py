@@ -168,7 +168,7 @@ def test_column_blocked_layout(ctx, name):
     old_s, old_p = ctx.get_params(smdp.APP_SPMV), ctx.get_params(smdp.APP_PAGERANK)
     try:
         for app in (smdp.APP_SPMV, smdp.APP_PAGERANK):
-            ctx.set_params(app, col_blocks=8, split_threshold=256, heavy_threshold=16)
+            ctx.set_params(app, layout=smdp.LAYOUT_XCD8, split_threshold=256, heavy_threshold=16)
         val = smdp.gen_uniform_f32(g.m, -1.0, 1.0, 4, 6)
         x = smdp.gen_uniform_f32(g.n, 0.0, 1.0, 4, 9)
         y, _ = ctx.spmv(g, val, x, smdp.SOL_TUNED)
@@ -193,7 +193,7 @@ def test_values_change_rebuilds_blocked_layout(ctx):
     g = MATS["rmat12"]
     old = ctx.get_params(smdp.APP_SPMV)
     try:
-        ctx.set_params(smdp.APP_SPMV, col_blocks=8)
+        ctx.set_params(smdp.APP_SPMV, layout=smdp.LAYOUT_XCD8)
         x = smdp.gen_uniform_f32(g.n, 0.0, 1.0, 5, 9)
         with smdp.Graph(ctx, g) as dg:
             dg.set_x(x)
@@ -206,3 +206,94 @@ def test_values_change_rebuilds_blocked_layout(ctx):
                 assert np.all(np.abs(y - y64) <= _spmv_bound(g, val, x, y64))
     finally:
         ctx.set_params(smdp.APP_SPMV, **old.as_dict())
+
+
+def _wide_graph():
+    """Four source chunks of 2^15 ids (the last one partial), a row with 70000
+    in-arcs (cut into several pieces), a row with 300000 in-arcs all from the
+    first chunk (one (chunk, row) segment longer than a scatter work item of
+    262144 arcs), a
+    row at and one just below the wide-row degree 4096, empty rows and
+    dangling vertices."""
+    rng = np.random.default_rng(11)
+    n = 3 * 32768 + 1000
+    parts_src = [rng.integers(0, n, 300000), rng.integers(0, 32768, 300000)]
+    parts_dst = [rng.integers(0, n, 300000), np.full(300000, 6)]
+    for v, k in ((5, 70000), (n - 1, 4096), (7, 4095)):
+        parts_src.append(rng.integers(0, n, k))
+        parts_dst.append(np.full(k, v))
+    src = np.concatenate(parts_src)
+    dst = np.concatenate(parts_dst)
+    keep = (src % 97) != 3  # about 1% of the vertices keep no out-arc: dangling
+    src, dst = src[keep], dst[keep]
+    order = np.argsort(src, kind="stable")
+    row = np.zeros(n + 1, dtype=np.int32)
+    np.add.at(row, src + 1, 1)
+    return smdp.HostGraph(np.cumsum(row).astype(np.int32), dst[order].astype(np.int32))
+
+
+PR_BIN = {"wide": _wide_graph(), **{k: MATS[k] for k in ("rmat12", "random", "rmat10_noperm", "uniform")}}
+
+
+@pytest.fixture
+def binned_ctx(ctx):
+    old = ctx.get_params(smdp.APP_PAGERANK)
+    ctx.set_params(smdp.APP_PAGERANK, layout=smdp.LAYOUT_BINNED)
+    yield ctx
+    ctx.set_params(smdp.APP_PAGERANK, **old.as_dict())
+
+
+@pytest.mark.parametrize("name", sorted(PR_BIN))
+def test_pagerank_binned_layout(binned_ctx, name):
+    """Tuned PageRank on the binned layout (scatter + reduce passes) against
+    the iteration-matched FP64 oracle; two runs are bit-identical."""
+    ctx = binned_ctx
+    g = PR_BIN[name]
+    ig, od = smdp.invert(g), g.outdegree()
+    print(f"pagerank binned {name} n={g.n} m={g.m}", flush=True)
+    opts = smdp.pr_opts(0.85, 1.0, 1e-6, 100)
+    r, st = ctx.pagerank(ig, od, opts, smdp.SOL_TUNED)
+    it = int(st.rounds)
+    ref, _, _ = orc.pagerank(ig.row, ig.col, od, d=0.85, T=1.0, fixed_iters=it)
+    l1 = np.abs(r.astype(np.float64) - ref).sum()
+    assert l1 <= 1e-6, l1
+    assert st.residual < 1e-6 or it == 100
+    _, it_conv, _ = orc.pagerank(ig.row, ig.col, od, d=0.85, T=1.0, tol=1e-6, max_iter=100)
+    assert abs(it_conv - it) <= 1
+    assert st.edges_relaxed == g.m * it
+    assert st.vertices_expanded == g.n * it
+    r2, _ = ctx.pagerank(ig, od, opts, smdp.SOL_TUNED)
+    assert np.array_equal(r, r2)
+
+
+def test_pagerank_binned_reference_parameters(binned_ctx):
+    # damping 0.5, total rank 1000, ranks starting at 0 (pagerank_wrapper.cu:7,115; pagerank.h:15)
+    g = PR_BIN["wide"]
+    ig, od = smdp.invert(g), g.outdegree()
+    opts = smdp.pr_opts(0.5, 1000.0, 1e-6, 100, init_zero=True)
+    r, st = binned_ctx.pagerank(ig, od, opts, smdp.SOL_TUNED)
+    ref, _, _ = orc.pagerank(ig.row, ig.col, od, d=0.5, T=1000.0, init_zero=True, fixed_iters=int(st.rounds))
+    assert np.abs(r.astype(np.float64) - ref).sum() <= 1e-6 * 1000.0
+
+
+def test_pagerank_binned_resident_reuse(binned_ctx):
+    """The layout is built once per resident graph; later solves reuse it and
+    give the same ranks as a fresh drop-in call."""
+    ctx = binned_ctx
+    g = PR_BIN["wide"]
+    ig, od = smdp.invert(g), g.outdegree()
+    opts = smdp.pr_opts(0.85, 1.0, 0.0, 9)
+    fresh, _ = ctx.pagerank(ig, od, opts, smdp.SOL_TUNED)
+    with smdp.Graph(ctx, smdp.HostGraph(ig.row, ig.col)) as dg:
+        dg.set_outdeg(od)
+        for _ in range(3):
+            st = dg.pagerank(opts, smdp.SOL_TUNED)
+            assert st.rounds == 9
+            assert np.array_equal(dg.read(smdp.RES_RANK, np.float32), fresh)
+
+
+def test_binned_layout_is_pagerank_only(ctx):
+    with pytest.raises(smdp.SmdpError):
+        ctx.set_params(smdp.APP_SPMV, layout=smdp.LAYOUT_BINNED)
+    with pytest.raises(smdp.SmdpError):
+        ctx.set_params(smdp.APP_PAGERANK, layout=3)
