@@ -43,7 +43,8 @@ constexpr int kW = kBinNT / SMDP_WAVE;
 constexpr int kSW = kScatNT / SMDP_WAVE;
 constexpr int kSrcChunkLog = 15;
 constexpr int kSrcChunk = 1 << kSrcChunkLog;  // sources per chunk: 128 KB of contributions in LDS
-constexpr int kBinCap = 16384;                // arcs per bin: 64 KB of values in LDS
+constexpr int kBinCap = 16380;                // arcs per bin: 64 KB of values in LDS; with a start
+                                              // off a 4-arc boundary, 4096 groups of 4 still cover it
 constexpr int kRowsPerThread = 3;
 constexpr int kBinRows = kRowsPerThread * kBinNT;  // rows per bin
 constexpr int kBinWide = kBinCap;             // rows with at least this many arcs get their own bin
@@ -57,7 +58,6 @@ constexpr int kWidePiece = 16384;             // arcs per piece of a wide row
 constexpr int kQuads = 8;                     // 4-arc groups each thread loads at once
 constexpr int kPad = 8;                       // entries past m in vals and dst: 4-arc loads may touch them
 constexpr int kSrcPad = kScatterBlock + 8;    // entries past m in src: a scatter block may read that far
-constexpr int kRowCost = 16;                  // reduce work per row, in arcs, when items are split over workgroups
 constexpr int64_t kMaxSegments = int64_t(1) << 27;  // bound on nchunks x nbins (host plan memory)
 
 static_assert(kBinCap <= 65535 && kSrcChunk <= 65536, "positions and offsets are stored in 16 bits");
@@ -164,9 +164,10 @@ __device__ __forceinline__ double2 bin_sum2(double2 v, double2* lds) {
   return t;
 }
 
-// Workgroup g takes the reduce items of r_group[g] .. r_group[g+1]-1.  Each
-// item's descriptor is read while the item before is summed.
-__global__ __launch_bounds__(kBinNT) void k_bin_reduce(RArgs a, BArgs b, int k) {
+// Reduce items are claimed from a counter, one at a time.  A workgroup claims
+// its next item when it starts one and reads that item's descriptor while it
+// sums the rows of the current one, so neither latency is on its path.
+__global__ __launch_bounds__(kBinNT, 2) void k_bin_reduce(RArgs a, BArgs b, int k) {
   if (a.st->done) return;
   const float base = iteration_base(a, k);
   const int lane = lane_id();
@@ -176,17 +177,23 @@ __global__ __launch_bounds__(kBinNT) void k_bin_reduce(RArgs a, BArgs b, int k) 
   __shared__ uint16_t s_perm[kBinRows];     // the bin's rows by falling degree
   __shared__ float s_res[kBinRows];         // row sums, by row
   __shared__ int s_last;
+  __shared__ int s_next;
   __shared__ float2 s_red[kW];
   __shared__ double2 s_red2[kW];
   unsigned long long edges = 0, rows = 0, pushes = 0;
-  const int it_end = b.r_group[blockIdx.x + 1];
-  int it = b.r_group[blockIdx.x];
+  const int nitems = b.nitems;
+  if (threadIdx.x == 0) s_next = atomicAdd(&a.st->work_next, 1);
+  __syncthreads();
+  int it = s_next;
+  __syncthreads();  // s_next is written again below
   int4 d0 = make_int4(0, 0, 0, 0), d1 = make_int4(0, 0, 0, 0);
-  if (it < it_end) {
+  if (it < nitems) {
     d0 = b.r_item[2 * it];
     d1 = b.r_item[2 * it + 1];
   }
-  for (; it < it_end; ++it) {
+  while (it < nitems) {
+    int nxt = 0;
+    if (threadIdx.x == 0) nxt = atomicAdd(&a.st->work_next, 1);  // the next item, claimed now
     const int beg = d0.x, end = d0.y, r0 = d0.z, nr = d0.w;
     const int nwave_rows = d1.x, ngroup_rows = d1.y, part = d1.z;
     // the run [beg, end) is read as aligned groups of 4 arcs
@@ -194,9 +201,12 @@ __global__ __launch_bounds__(kBinNT) void k_bin_reduce(RArgs a, BArgs b, int k) 
     const int nq = (end - a0 + 3) >> 2;
     if (nr < 0) {
       // ---- a piece of a wide row: one compensated sum over a contiguous run
-      if (it + 1 < it_end) {
-        d0 = b.r_item[2 * it + 2];
-        d1 = b.r_item[2 * it + 3];
+      if (threadIdx.x == 0) s_next = nxt;
+      __syncthreads();
+      it = s_next;
+      if (it < nitems) {
+        d0 = b.r_item[2 * it];
+        d1 = b.r_item[2 * it + 1];
       }
       CSum acc;
       for (int qb = 0; qb < nq; qb += kQuads * kBinNT) {
@@ -221,43 +231,62 @@ __global__ __launch_bounds__(kBinNT) void k_bin_reduce(RArgs a, BArgs b, int k) 
       if (threadIdx.x == 0) b.wpart[part] = make_float2(h.s, h.c);
       continue;
     }
-    // ---- a bin of rows.  Everything it needs is requested at once: row
-    // offsets and the degree order, old ranks and out-degrees of this
-    // thread's rows, the values.
-    for (int i = threadIdx.x; i <= nr; i += kBinNT) s_row[i] = (uint16_t)(a.row[r0 + i] - beg);
-    for (int i = threadIdx.x; i < nr; i += kBinNT) s_perm[i] = b.rperm[r0 + i];
+    // ---- a bin of rows.  Every load the bin needs is issued before any of
+    // them is used: row offsets, the degree order, old ranks and
+    // out-degrees of this thread's rows, and the whole run of values and
+    // positions (kQuads groups of 4 arcs per thread cover a full bin).
+    constexpr int kRowLoads = (kBinRows + 1 + kBinNT - 1) / kBinNT;
+    int roff[kRowLoads];
+    uint16_t perm[kRowsPerThread];
     float rold[kRowsPerThread];
     int od[kRowsPerThread];
 #pragma unroll
+    for (int q = 0; q < kRowLoads; ++q) {
+      const int i = (int)threadIdx.x + q * kBinNT;
+      roff[q] = i <= nr ? a.row[r0 + i] : 0;
+    }
+#pragma unroll
     for (int q = 0; q < kRowsPerThread; ++q) {
       const int r = (int)threadIdx.x + q * kBinNT;
+      perm[q] = r < nr ? b.rperm[r0 + r] : (uint16_t)0;
       rold[q] = r < nr ? a.rold[r0 + r] : 0.f;
       od[q] = r < nr ? a.outdeg[r0 + r] : 0;
     }
-    for (int qb = 0; qb < nq; qb += kQuads * kBinNT) {
-      f32x4 v[kQuads];
-      u16x4 dd[kQuads];
+    static_assert(kQuads * kBinNT * 4 >= kBinCap + 3, "one round of quads covers a bin");
+    f32x4 v[kQuads];
+    u16x4 dd[kQuads];
 #pragma unroll
-      for (int t = 0; t < kQuads; ++t) {
-        const int qi = min(qb + t * kBinNT + (int)threadIdx.x, nq - 1);
-        v[t] = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(b.vals + a0) + qi);
-        dd[t] = __builtin_nontemporal_load(reinterpret_cast<const u16x4*>(b.dst + a0) + qi);
-      }
+    for (int t = 0; t < kQuads; ++t) {
+      const int qi = min(t * kBinNT + (int)threadIdx.x, nq - 1);
+      v[t] = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(b.vals + a0) + qi);
+      dd[t] = __builtin_nontemporal_load(reinterpret_cast<const u16x4*>(b.dst + a0) + qi);
+    }
 #pragma unroll
-      for (int t = 0; t < kQuads; ++t) {
-        const int qi = qb + t * kBinNT + (int)threadIdx.x;
+    for (int q = 0; q < kRowLoads; ++q) {
+      const int i = (int)threadIdx.x + q * kBinNT;
+      if (i <= nr) s_row[i] = (uint16_t)(roff[q] - beg);
+    }
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int j = a0 + 4 * qi + e;
-          if (qi < nq && j >= beg && j < end) s_v[dd[t][e]] = v[t][e];
-        }
+    for (int q = 0; q < kRowsPerThread; ++q) {
+      const int r = (int)threadIdx.x + q * kBinNT;
+      if (r < nr) s_perm[r] = perm[q];
+    }
+#pragma unroll
+    for (int t = 0; t < kQuads; ++t) {
+      const int qi = t * kBinNT + (int)threadIdx.x;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int j = a0 + 4 * qi + e;
+        if (qi < nq && j >= beg && j < end) s_v[dd[t][e]] = v[t][e];
       }
     }
-    if (it + 1 < it_end) {
-      d0 = b.r_item[2 * it + 2];
-      d1 = b.r_item[2 * it + 3];
-    }
+    if (threadIdx.x == 0) s_next = nxt;
     __syncthreads();
+    it = s_next;
+    if (it < nitems) {
+      d0 = b.r_item[2 * it];
+      d1 = b.r_item[2 * it + 1];
+    }
     // Row sums.  Rows in falling degree order: the first nwave_rows take a
     // wave each, the next ngroup_rows 8 lanes each, the rest one lane each,
     // so the lanes of one task have rows of about the same length.
@@ -346,7 +375,6 @@ struct BinnedPlan {
   int32_t* s_item = nullptr;
   int32_t* s_group = nullptr;
   int4* r_item = nullptr;
-  int32_t* r_group = nullptr;
   int32_t* wfirst = nullptr;
   int32_t* wrow = nullptr;
   float2* wpart = nullptr;
@@ -364,30 +392,13 @@ void parallel_ranges(int64_t count, F body) {
   for (auto& x : th) x.join();
 }
 
-// Splits items with the given costs into `groups` contiguous runs of about
-// equal cost; returns groups+1 offsets.
-std::vector<int32_t> split_runs(const std::vector<int64_t>& cost, int64_t groups) {
-  int64_t total = 0;
-  for (int64_t c : cost) total += c;
-  std::vector<int32_t> first(1, 0);
-  int64_t acc = 0;
-  const int64_t n = (int64_t)cost.size();
-  for (int64_t g = 1; g < groups; ++g) {
-    const int64_t target = total * g / groups;
-    int64_t i = first.back();
-    while (i < n && acc + cost[(size_t)i] / 2 < target) acc += cost[(size_t)i++];
-    first.push_back((int32_t)i);
-  }
-  first.push_back((int32_t)n);
-  return first;
-}
 }  // namespace
 
 void binned_free(void* plan) {
   BinnedPlan* p = static_cast<BinnedPlan*>(plan);
   if (!p) return;
   void* ptrs[] = {p->cm,      p->bm,     p->src,     p->dst,    p->vals,   p->rperm, p->s_item,
-                  p->s_group, p->r_item, p->r_group, p->wfirst, p->wrow,   p->wpart, p->bpart};
+                  p->s_group, p->r_item, p->wfirst,  p->wrow,   p->wpart,  p->bpart};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete p;
@@ -540,10 +551,9 @@ int binned_ensure(smdp_graph* g, void** slot, BArgs* out) {
       s_group.push_back((int32_t)(s_item.size() / 5));
     }
   }
-  // reduce: items in row order (bins, and the pieces of each wide row), then
-  // two workgroups per CU, each a run of items of about equal cost
+  // reduce: items in row order (bins, and the pieces of each wide row),
+  // claimed by the workgroups one at a time
   std::vector<int4> r_item;
-  std::vector<int64_t> rcost;
   std::vector<int32_t> wfirst(1, 0), wrow;
   int32_t nparts = 0, npieces = 0;
   for (int64_t b = 0; b < B; ++b) {
@@ -553,7 +563,6 @@ int binned_ensure(smdp_graph* g, void** slot, BArgs* out) {
         const int32_t pe = (int32_t)std::min<int64_t>(e1, p + kWidePiece);
         r_item.push_back(make_int4((int32_t)p, pe, brow[b], -1));
         r_item.push_back(make_int4(0, 0, npieces++, 0));
-        rcost.push_back(pe - p);
       }
       wfirst.push_back(npieces);
       wrow.push_back(brow[b]);
@@ -561,10 +570,8 @@ int binned_ensure(smdp_graph* g, void** slot, BArgs* out) {
       const int32_t nr = brow[b + 1] - brow[b];
       r_item.push_back(make_int4(e0, e1, brow[b], nr));
       r_item.push_back(make_int4(bcls[(size_t)(2 * b)], bcls[(size_t)(2 * b + 1)], nparts++, 0));
-      rcost.push_back((int64_t)(e1 - e0) + (int64_t)kRowCost * nr);
     }
   }
-  const std::vector<int32_t> r_group = split_runs(rcost, 2 * cus);
 
   BinnedPlan* p = new BinnedPlan();
   int rc = SMDP_OK;
@@ -579,7 +586,6 @@ int binned_ensure(smdp_graph* g, void** slot, BArgs* out) {
   step(upload(&p->s_item, s_item));
   step(upload(&p->s_group, s_group));
   step(upload(&p->r_item, r_item));
-  step(upload(&p->r_group, r_group));
   step(upload(&p->wfirst, wfirst));
   step(upload(&p->wrow, wrow));
   if (rc == SMDP_OK) step(alloc_once(&p->vals, m + kPad));
@@ -599,7 +605,6 @@ int binned_ensure(smdp_graph* g, void** slot, BArgs* out) {
   a.s_item = p->s_item;
   a.s_group = p->s_group;
   a.r_item = p->r_item;
-  a.r_group = p->r_group;
   a.wfirst = p->wfirst;
   a.wrow = p->wrow;
   a.wpart = p->wpart;
@@ -607,7 +612,7 @@ int binned_ensure(smdp_graph* g, void** slot, BArgs* out) {
   a.nchunks = (int32_t)K;
   a.nbins = (int32_t)B;
   a.nsgroups = (int32_t)(s_group.size() - 1);
-  a.nrgroups = (int32_t)(r_group.size() - 1);
+  a.nitems = (int32_t)(r_item.size() / 2);
   a.nwide = (int32_t)wrow.size();
   a.nparts = nparts;
   *slot = p;
@@ -615,14 +620,14 @@ int binned_ensure(smdp_graph* g, void** slot, BArgs* out) {
   return SMDP_OK;
 }
 
-// One iteration: the scatter holds ~146 KB of LDS per workgroup (one per CU),
-// the reduce ~76 KB (two per CU).  Both grids match the plan's work runs.
+// One iteration: the scatter holds ~146 KB of LDS per workgroup (one per CU)
+// and runs the plan's work runs; the reduce holds ~76 KB (two per CU).
 void binned_pass(smdp_ctx* ctx, hipStream_t s, const RArgs& a, const BArgs& b, int k, int* launches) {
   prof_mark(ctx, 1);
   hipLaunchKernelGGL(k_bin_scatter, dim3(b.nsgroups), dim3(kScatNT), 0, s, a, b);
   prof_mark(ctx, 1);
   prof_mark(ctx, 0);
-  hipLaunchKernelGGL(k_bin_reduce, dim3(b.nrgroups), dim3(kBinNT), 0, s, a, b, k);
+  hipLaunchKernelGGL(k_bin_reduce, dim3(2 * ctx->cus), dim3(kBinNT), 0, s, a, b, k);
   prof_mark(ctx, 0);
   *launches += 2;
 }

@@ -85,7 +85,7 @@ struct BArgs {
   const int32_t* __restrict__ s_group; // scatter workgroup g takes items s_group[g] .. s_group[g+1]-1
   const int4* __restrict__ r_item;     // reduce items, two int4 each: (beg, end, r0, nr), (nwave, ngroup, part, 0);
                                        // nr < 0 marks a piece of a wide row, part indexes wpart, else bpart
-  const int32_t* __restrict__ r_group; // reduce workgroup g takes items r_group[g] .. r_group[g+1]-1
+
   const int32_t* __restrict__ wfirst;  // nwide+1: first piece of each wide row
   const int32_t* __restrict__ wrow;    // nwide: the wide rows
   float2* wpart;                       // per wide piece: (sum, error)
@@ -93,7 +93,7 @@ struct BArgs {
   int32_t nchunks;
   int32_t nbins;
   int32_t nsgroups;                    // scatter workgroups
-  int32_t nrgroups;                    // reduce workgroups
+  int32_t nitems;                      // reduce items (claimed from RState::work_next)
   int32_t nwide;
   int32_t nparts;                      // bins of rows (entries of bpart)
 };
