@@ -35,7 +35,9 @@ constexpr int kWaveQ = 256;   // per-wave LDS queue capacity (SMDP_SOL_WAVE)
 constexpr int kBlockQ = 1024; // per-workgroup LDS queue capacity (SMDP_SOL_BLOCK)
 constexpr int kOutCap = 512;  // per-wave LDS staging of next-queue entries
 constexpr int kIlp = 4;       // arcs per thread per step in the tuned kernels
-constexpr int kTileGrab = 4;  // frontier tiles claimed per atomic
+constexpr int kTileGrab = 1;  // frontier tiles claimed per atomic
+constexpr int kSmallSplit = 32;  // split degree of the tuned solution when the queue is short
+constexpr int kChunkGrab = 4;    // grid-queue chunks claimed per atomic
 
 // Next-queue output of one wave: LDS staging + the global queue it flushes to.
 struct Sink {
@@ -68,6 +70,7 @@ struct FArgs {
   Counters* cnt;
   int32_t heavy;  // lane solutions: heavy degree; tuned: split degree
   int32_t chunk;  // grid-queue chunk in edges; 0 = whole vertex
+  int32_t small_q;  // tuned: a queue shorter than this splits vertices of >= kSmallSplit arcs
   unsigned long long* rlog;  // profiling: per round (queue length, arcs), or nullptr
 };
 
@@ -379,7 +382,10 @@ __global__ __launch_bounds__(SMDP_WG) void k_expand_tile(FArgs a, int r) {
       du = a.dist[u];
       ++verts;
     }
-    const bool split = deg >= a.heavy;
+    // A short queue has too few tiles to keep every workgroup busy, and its
+    // vertices tend to be hubs: most of their arcs then go to the grid queue,
+    // which spreads them over all workgroups.
+    const bool split = deg >= (L < a.small_q ? min(a.heavy, kSmallSplit) : a.heavy);
     const int nch = split ? (deg + a.chunk - 1) / a.chunk : 0;
     const int incl = wave_inclusive_scan(nch);
     const int wtot = __shfl(incl, SMDP_WAVE - 1);
@@ -445,15 +451,17 @@ __global__ __launch_bounds__(SMDP_WG) void k_drain(FArgs a, int r) {
   out.init(s_out + (threadIdx.x / SMDP_WAVE) * kOutCap, qout, &cout->qlen);
   unsigned long long edges = 0;
   for (;;) {
-    if (threadIdx.x == 0) s_idx = atomicAdd(&cin->chunk_next, 1);
+    if (threadIdx.x == 0) s_idx = atomicAdd(&cin->chunk_next, kChunkGrab);
     __syncthreads();
-    const int idx = s_idx;
+    const int first = s_idx;
     __syncthreads();
-    if (idx >= H) break;
-    const Chunk c = a.hq[idx];
-    const int vend = a.row[c.v + 1];
-    const int end = (a.chunk > 0 && vend - c.first > a.chunk) ? c.first + a.chunk : vend;
-    wg_expand<BFS>(a, c.first, end, a.dist[c.v], label, edges, out);
+    if (first >= H) break;
+    for (int idx = first; idx < min(first + kChunkGrab, H); ++idx) {
+      const Chunk c = a.hq[idx];
+      const int vend = a.row[c.v + 1];
+      const int end = (a.chunk > 0 && vend - c.first > a.chunk) ? c.first + a.chunk : vend;
+      wg_expand<BFS>(a, c.first, end, a.dist[c.v], label, edges, out);
+    }
   }
   log_round(a, r, -1, wave_sum(edges));
   flush_counters(a.cnt, edges, 0, out.pushes, 0, 0, 0, out);
@@ -557,6 +565,7 @@ int run_frontier(smdp_graph* g, int32_t src, int sol, bool bfs, smdp_stats* st) 
   a.rlog = ctx->profiling ? ctx->rlog : nullptr;
   if (a.rlog) SMDP_HIP(hipMemsetAsync(a.rlog, 0, sizeof(unsigned long long) * 2 * kRoundLogCap, ctx->stream));
   const int grid = grid_size(ctx, p);
+  a.small_q = 2 * grid * SMDP_WG;
   const int32_t inf = bfs ? SMDP_BFS_UNDEF : SMDP_SSSP_INF;
   const bool two_phase = sol == SMDP_SOL_GRID || tuned;
   hipStream_t s = ctx->stream;
