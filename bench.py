@@ -6,7 +6,10 @@ the HBM roofline of the dominant kernel, and batched vs plain SSSP.
 .57/.19/.19/.05, weights U[1,255], seed 3): reached out-arcs / wall time of one
 full single-source solve with the graph already resident in HBM.  A "step" is
 one such solve.  The PageRank (same graph, pull, d = 0.85, FP64 L1 < 1e-6 or
-100 iterations) and batched-SSSP figures ride along in the same JSON line.
+100 iterations), batched-SSSP, SpMV (RMAT-20), colouring (RMAT-23, symmetrised)
+and recursive-BFS (4M-node Zipf tree) figures ride along in the same JSON line,
+each with its HBM roofline, its parity check and a CPU time of the oracle, plus
+a measured device-to-device copy rate.
 
 The library call is synchronous (it returns after the device finished and the
 round counter was read back), so the wall clock around K calls is the device
@@ -89,7 +92,15 @@ def sssp_section(ctx, g, dg, args, smdp, orc, np):
         kernels.append({"kernel": k["kernel"], "ms_total": round(k["ms"], 4), "launches": k["launches"],
                         "achieved_GBs": round(a, 1), "frac": round(a / HBM_PEAK_GBS, 4)})
     whole = (bytes1 + bytes2) / (ms_step * 1e-3) / 1e9
+    inflation = st.edges_relaxed / max(1, st.m_reached)
+    roof["frac_min_bytes"] = round(roof["frac"] / max(inflation, 1e-9), 4)
+    # end to end: the drop-in call with host arrays (upload, solve, download)
+    e2e = ctx.sssp(g, src, sol)[1]
+    dev = sorted(x.ms_kernel for x in sts)
     info = {"source": int(src), "ms_per_solve": round(ms_step, 4), "ms_device": round(st.ms_kernel, 4),
+            "ms_device_median": round(dev[len(dev) // 2], 4),
+            "ms_end_to_end_dropin": round(e2e.ms_total, 3),
+            "gteps_end_to_end_dropin": round(st.m_reached / (e2e.ms_total * 1e6), 4),
             "rounds": st.rounds, "edges_relaxed": st.edges_relaxed, "m_reached": st.m_reached,
             "inflation": round(st.edges_relaxed / max(1, st.m_reached), 3),
             "vertices_expanded": st.vertices_expanded, "kernel_launches": st.kernel_launches,
@@ -98,23 +109,31 @@ def sssp_section(ctx, g, dg, args, smdp, orc, np):
     return gteps, ms_step, roof, info, st
 
 
-def cpu_baseline(g, src, orc, np, budget_s=12.0):
-    """The oracle's worklist Bellman-Ford (the reference algorithm, one thread)."""
-    runs = 0
-    t0 = time.perf_counter()
+def cpu_baseline(g, src, orc, np, budget_s=8.0):
+    """The oracle's worklist Bellman-Ford (the reference algorithm) on all host
+    threads, with the single-thread run and a binary-heap Dijkstra beside it."""
+    def timed(fn, budget, cap):
+        runs, t0, out = 0, time.perf_counter(), None
+        while True:
+            out = fn()
+            runs += 1
+            if time.perf_counter() - t0 >= budget or runs >= cap:
+                break
+        return (time.perf_counter() - t0) / runs, runs, out
     mr = None
-    while True:
-        d, rounds, edges = orc.sssp_worklist(g.row, g.col, g.w, src)
-        runs += 1
-        if mr is None:
-            mr = int(np.diff(g.row)[d != orc.INF].sum())
-        if time.perf_counter() - t0 >= budget_s or runs >= 20:
-            break
-    t = (time.perf_counter() - t0) / runs
-    return {"value": round(mr / t / 1e9, 5), "unit": "GTEPS", "cores": 1, "kind": "port",
-            "sample": f"{runs} full solves of worklist Bellman-Ford (sssp_kernel.cu:40-61 restated, "
-                      f"oracle/oracle.c orc_sssp_worklist) on the same RMAT-22 graph and source; "
-                      f"{t:.3f} s each, {rounds} rounds, {edges} arcs scanned"}
+    t_par, n_par, (d, rounds, edges, threads) = timed(lambda: orc.sssp_worklist_par(g.row, g.col, g.w, src), budget_s, 20)
+    mr = int(np.diff(g.row)[d != orc.INF].sum())
+    t_seq, n_seq, (_, _, _) = timed(lambda: orc.sssp_worklist(g.row, g.col, g.w, src), budget_s, 20)
+    t_dij, n_dij, _ = timed(lambda: orc.dijkstra(g.row, g.col, g.w, src), budget_s / 2, 5)
+    return {"value": round(mr / t_par / 1e9, 5), "unit": "GTEPS", "cores": threads, "kind": "port",
+            "sample": f"{n_par} full solves of worklist Bellman-Ford on {threads} threads (sssp_kernel.cu:40-61 "
+                      f"restated, oracle/oracle.c orc_sssp_worklist_par) on the same RMAT-22 graph and source; "
+                      f"{t_par:.3f} s each, {rounds} rounds, {edges} arcs scanned",
+            "nproc": os.cpu_count(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "single_thread": {"value": round(mr / t_seq / 1e9, 5), "cores": 1, "s_per_solve": round(t_seq, 3),
+                              "solves": n_seq},
+            "dijkstra_binary_heap": {"value": round(mr / t_dij / 1e9, 5), "cores": 1, "s_per_solve": round(t_dij, 3),
+                                     "solves": n_dij}}
 
 
 def pagerank_section(ctx, g, args, smdp, orc, np):
@@ -171,6 +190,9 @@ def pagerank_section(ctx, g, args, smdp, orc, np):
         ref, _, _ = orc.pagerank(ig.row, ig.col, od, d=0.85, T=1.0, fixed_iters=iters)
         out["l1_vs_fp64_iteration_matched"] = float(np.abs(r.astype(np.float64) - ref).sum())
         out["linf_vs_fp64"] = float(np.abs(r.astype(np.float64) - ref).max())
+        conv, it_conv, _ = orc.pagerank(ig.row, ig.col, od, d=0.85, T=1.0, tol=1e-9, max_iter=1000)
+        out["l1_vs_fp64_converged"] = float(np.abs(r.astype(np.float64) - conv).sum())
+        out["fp64_converged_iterations"] = it_conv
     return out
 
 
@@ -221,6 +243,106 @@ def batched_section(ctx, args, smdp, orc, np):
                           "ms_256_plain_with_upload": round(t_plain_upload * 1e3, 3),
                           "speedup": round(t_plain / t_batch, 2)}
     return out
+
+
+def _median_runs(fn, reps=5):
+    fn()
+    sts = [fn() for _ in range(reps)]
+    ms = sorted(x.ms_kernel for x in sts)
+    return ms[len(ms) // 2], sts[-1]
+
+
+def _roof(bytes_, ms):
+    ach = bytes_ / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "algorithmic_bytes": int(bytes_)}
+
+
+def spmv_section(ctx, smdp, orc, np):
+    """CSR SpMV y = A x, RMAT scale 20 (16.8M nnz), FP32 with compensated sums."""
+    g = smdp.gen_rmat(20, 16, seed=2)
+    val = smdp.gen_uniform_f32(g.m, -1.0, 1.0, 2, 6)
+    x = smdp.gen_uniform_f32(g.n, 0.0, 1.0, 2, 9)
+    with smdp.Graph(ctx, g) as dg:
+        dg.set_values(val)
+        dg.set_x(x)
+        ms, st = _median_runs(lambda: dg.spmv(smdp.SOL_TUNED))
+        y = dg.read(smdp.RES_Y, np.float32)
+    t0 = time.perf_counter()
+    y64 = orc.spmv(g.row, g.col, val, x)
+    t_cpu = time.perf_counter() - t0
+    return {"workload": "spmv_rmat20", "rows": g.n, "nnz": g.m, "ms_device_median": round(ms, 4),
+            "gflops": round(2 * g.m / (ms * 1e6), 2), "gteps": round(g.m / (ms * 1e6), 3),
+            "roofline": _roof(12 * g.m + 8 * g.n + 4, ms),
+            "linf_vs_fp64": float(np.abs(y.astype(np.float64) - y64).max()),
+            "cpu": {"ms": round(t_cpu * 1e3, 1), "cores": 1, "kind": "port",
+                    "sample": "one thread-per-row FP64 product (SpMV_kernel.cu:81-96 restated, orc_spmv_f64)"}}
+
+
+def color_section(ctx, smdp, orc, np):
+    """Colouring of RMAT scale 23, edge factor 8, symmetrised, random priorities."""
+    g = smdp.gen_rmat(23, 8, symmetrize=True, seed=5)
+    prio = smdp.gen_prio(g.n, 5)
+    with smdp.Graph(ctx, g) as dg:
+        dg.set_prio(prio)
+        ms, st = _median_runs(lambda: dg.color(True, smdp.SOL_TUNED), reps=3)
+        c = dg.read(smdp.RES_COLOR)
+    t0 = time.perf_counter()
+    ref, k = orc.color_closed(g.row, g.col, prio)
+    t_cpu = time.perf_counter() - t0
+    return {"workload": "color_rmat23_sym", "vertices": g.n, "arcs": g.m, "ms_device_median": round(ms, 4),
+            "rounds": int(st.rounds), "colors": int(c.max()), "exact_vs_oracle": bool(np.array_equal(c, ref)),
+            "scanned_arcs": int(st.edges_relaxed), "gteps": round(g.m / (ms * 1e6), 3),
+            "roofline": _roof(12 * st.edges_relaxed + 16 * st.vertices_expanded, ms),
+            "cpu": {"ms": round(t_cpu * 1e3, 1), "cores": 1, "kind": "port",
+                    "sample": "one pass of the closed form colour(v) = 1 + max over preceding neighbours "
+                              "(orc_color_closed; the round rule of graph_color_kernel.cu:28-49 gives the same colours)"}}
+
+
+def bfs_section(ctx, smdp, orc, np):
+    """Recursive-BFS workload: a 4M-node tree with Zipf(1.5) child counts capped at 4096."""
+    t = smdp.gen_zipf_tree(4194304, 1.5, 4096, 5)
+    with smdp.Graph(ctx, t) as dg:
+        ms, st = _median_runs(lambda: dg.bfs(0, smdp.SOL_TUNED))
+        lv = dg.read(smdp.RES_LEVEL)
+    t0 = time.perf_counter()
+    ref = orc.bfs(t.row, t.col, 0)
+    t_cpu = time.perf_counter() - t0
+    return {"workload": "bfs_zipf_tree_4m", "vertices": t.n, "arcs": t.m, "ms_device_median": round(ms, 4),
+            "rounds": int(st.rounds), "exact_vs_oracle": bool(np.array_equal(lv, ref)),
+            "gteps": round(t.m / (ms * 1e6), 3), "roofline": _roof(8 * st.edges_relaxed + 12 * t.n, ms),
+            "cpu": {"ms": round(t_cpu * 1e3, 1), "cores": 1, "kind": "port",
+                    "sample": "one FIFO BFS (bfs_rec.cpp:102-121 restated, orc_bfs_fifo)"}}
+
+
+def stream_copy_bandwidth(nbytes=1 << 29, reps=5):
+    """Measured device-to-device copy rate (read + write bytes per second)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+    hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+    hip.hipEventSynchronize.argtypes = [C.c_void_p]
+    hip.hipEventDestroy.argtypes = [C.c_void_p]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    a, b, e0, e1 = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipEventCreate(C.byref(e0)) == 0 and hip.hipEventCreate(C.byref(e1)) == 0
+    best = 1e30
+    for _ in range(reps):
+        assert hip.hipEventRecord(e0, None) == 0
+        assert hip.hipMemcpy(b, a, nbytes, 3) == 0  # hipMemcpyDeviceToDevice
+        assert hip.hipEventRecord(e1, None) == 0
+        assert hip.hipEventSynchronize(e1) == 0
+        ms = C.c_float()
+        assert hip.hipEventElapsedTime(C.byref(ms), e0, e1) == 0
+        best = min(best, ms.value)
+    for x in (a, b):
+        hip.hipFree(x)
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
+    return round(2 * nbytes / (best * 1e-3) / 1e9, 1)
 
 
 def copy_calibration(nbytes=1 << 28, reps=4):
@@ -298,6 +420,7 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-pagerank", action="store_true")
     ap.add_argument("--skip-batched", action="store_true")
+    ap.add_argument("--skip-extra", action="store_true", help="skip the SpMV, colouring and BFS sections")
     ap.add_argument("--layout", type=int, default=0,
                     help="PageRank arc layout: 0 auto, 1 plain CSR, 2 binned, 8 XCD blocks")
     ap.add_argument("--params", default="", help="profiling runs: JSON object of smdp_params fields to set")
@@ -355,6 +478,15 @@ def main():
             log(f"batched: {line['batched']}")
         except smdp.SmdpError as e:
             line["batched"] = {"error": str(e)}
+    if not args.skip_extra:
+        for name, fn in (("spmv", spmv_section), ("color", color_section), ("bfs", bfs_section)):
+            try:
+                line[name] = fn(ctx, smdp, orc, np)
+                log(f"{name}: {line[name]}")
+            except smdp.SmdpError as e:
+                line[name] = {"error": str(e)}
+    line["stream_copy_GBs"] = stream_copy_bandwidth()
+    log(f"stream copy: {line['stream_copy_GBs']} GB/s")
     ctx.close()
     print(json.dumps(line), flush=True)
 

@@ -15,6 +15,7 @@
  */
 #include <math.h>
 #include <stdint.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -144,6 +145,62 @@ long long orc_sssp_worklist(int n, const int* row, const int* col, const int* w,
   free(update);
   free(queue);
   if (edges) *edges = scanned;
+  return rounds;
+}
+
+/* The same rounds on all host threads (OpenMP): the flag scan and the arc
+ * loop are split over threads, and "if cost[v] > alt then cost[v] = alt" is
+ * an atomic minimum, so the distances are those of the sequential version.
+ * Timed as the multi-core CPU baseline.  *threads receives the thread count. */
+long long orc_sssp_worklist_par(int n, const int* row, const int* col, const int* w, int src, int* cost,
+                                long long* edges, int* threads) {
+  if (n <= 0 || src < 0 || src >= n) return -1;
+  char* update = (char*)calloc((size_t)n, 1);
+  int* queue = (int*)malloc(sizeof(int) * (size_t)n);
+  if (!update || !queue) {
+    free(update);
+    free(queue);
+    return -2;
+  }
+  for (int i = 0; i < n; ++i) cost[i] = ORC_INF;
+  cost[src] = 0;
+  update[src] = 1;
+  long long rounds = 0, scanned = 0;
+  int nt = 1;
+#pragma omp parallel
+  {
+#pragma omp single
+    nt = omp_get_num_threads();
+  }
+  for (;;) {
+    int qn = 0;
+#pragma omp parallel for schedule(static)
+    for (int v = 0; v < n; ++v)
+      if (update[v]) {
+        update[v] = 0;
+        queue[__atomic_fetch_add(&qn, 1, __ATOMIC_RELAXED)] = v;
+      }
+    if (qn == 0) break;
+    ++rounds;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : scanned)
+    for (int i = 0; i < qn; ++i) {
+      int u = queue[i];
+      int cu = __atomic_load_n(&cost[u], __ATOMIC_RELAXED);
+      for (int e = row[u]; e < row[u + 1]; ++e) {
+        int v = col[e];
+        int alt = cu + w[e];
+        int old = __atomic_load_n(&cost[v], __ATOMIC_RELAXED);
+        while (old > alt && !__atomic_compare_exchange_n(&cost[v], &old, alt, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+        }
+        if (old > alt) __atomic_store_n(&update[v], 1, __ATOMIC_RELAXED);
+      }
+      scanned += row[u + 1] - row[u];
+    }
+  }
+  free(update);
+  free(queue);
+  if (edges) *edges = scanned;
+  if (threads) *threads = nt;
   return rounds;
 }
 

@@ -34,6 +34,8 @@ def lib():
         L.orc_bellman_ford.argtypes = [i, vp, vp, vp, i, vp]
         L.orc_sssp_worklist.argtypes = [i, vp, vp, vp, i, vp, C.POINTER(C.c_longlong)]
         L.orc_sssp_worklist.restype = C.c_longlong
+        L.orc_sssp_worklist_par.argtypes = [i, vp, vp, vp, i, vp, C.POINTER(C.c_longlong), C.POINTER(C.c_int)]
+        L.orc_sssp_worklist_par.restype = C.c_longlong
         L.orc_bfs_fifo.argtypes = [i, vp, vp, i, vp]
         L.orc_color_rounds.argtypes = [i, vp, vp, vp, vp]
         L.orc_color_closed.argtypes = [i, vp, vp, vp, vp]
@@ -74,6 +76,18 @@ def sssp_worklist(row, col, w, src):
     rounds = lib().orc_sssp_worklist(len(row) - 1, _p(row), _p(col), _p(w), src, _p(d), C.byref(edges))
     assert rounds >= 0
     return d, int(rounds), int(edges.value)
+
+
+def sssp_worklist_par(row, col, w, src):
+    """The same rounds on all host threads; returns (dist, rounds, edges, threads)."""
+    row, col, w = _i32(row), _i32(col), _i32(w)
+    d = np.zeros(len(row) - 1, dtype=np.int32)
+    edges = C.c_longlong()
+    threads = C.c_int()
+    rounds = lib().orc_sssp_worklist_par(len(row) - 1, _p(row), _p(col), _p(w), src, _p(d), C.byref(edges),
+                                         C.byref(threads))
+    assert rounds >= 0
+    return d, int(rounds), int(edges.value), int(threads.value)
 
 
 def bfs(row, col, src):

@@ -349,6 +349,8 @@ __global__ __launch_bounds__(SMDP_WG) void k_expand_tile(FArgs a, int r) {
   const int L = cin->qlen;
   if (L == 0) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&a.cnt->rounds, (unsigned long long)(r + 1));
+  // workgroups past the tile count would only contend for the counter
+  if ((int)blockIdx.x * SMDP_WG * kTileGrab >= L) return;
   const int32_t* qin = (r & 1) ? a.q1 : a.q0;
   int32_t* qout = (r & 1) ? a.q0 : a.q1;
   const int label = r + 1;
@@ -442,7 +444,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_drain(FArgs a, int r) {
   Ctl* cin = a.ctl + (r & 3);
   Ctl* cout = a.ctl + ((r + 1) & 3);
   const int H = cin->hlen;
-  if (H == 0) return;
+  if ((int)blockIdx.x * kChunkGrab >= H) return;
   int32_t* qout = (r & 1) ? a.q0 : a.q1;
   const int label = r + 1;
   __shared__ int s_idx;

@@ -66,7 +66,7 @@ def test_color_tiny_thresholds(ctx):
     ref, _ = orc.color_closed(g.row, g.col, prio)
     old = ctx.get_params(smdp.APP_COLOR)
     try:
-        ctx.set_params(smdp.APP_COLOR, heavy_threshold=2, split_threshold=8, wg_per_cu=1)
+        ctx.set_params(smdp.APP_COLOR, heavy_threshold=2, split_threshold=8, chunk=64, wg_per_cu=1)
         for sol in SOLS:
             c, st = ctx.color(g, prio, sol)
             assert np.array_equal(c, ref), sol

@@ -66,12 +66,22 @@ def test_worklist_matches_dijkstra(seed):
     assert rounds >= 1 and edges >= np.diff(row)[3]
 
 
+@pytest.mark.parametrize("seed", range(3))
+def test_parallel_worklist_matches_dijkstra(seed):
+    # the multi-core CPU baseline: same rounds over all host threads, atomic minimum
+    row, col, w = random_graph(20000, 200000, 11 + seed, 1, 255)
+    d, rounds, edges, threads = orc.sssp_worklist_par(row, col, w, 5)
+    assert np.array_equal(d, orc.dijkstra(row, col, w, 5))
+    assert rounds >= 1 and edges >= np.diff(row)[5] and threads >= 1
+
+
 def test_sssp_unreachable_sentinel():
     row = np.array([0, 1, 1, 1], dtype=np.int32)
     col = np.array([1], dtype=np.int32)
     w = np.array([5], dtype=np.int32)
     assert orc.dijkstra(row, col, w, 0).tolist() == [0, 5, orc.INF]
     assert orc.sssp_worklist(row, col, w, 0)[0].tolist() == [0, 5, orc.INF]
+    assert orc.sssp_worklist_par(row, col, w, 0)[0].tolist() == [0, 5, orc.INF]
 
 
 # ---- BFS -------------------------------------------------------------------
