@@ -96,7 +96,7 @@ __global__ void k_ms_init(int32_t* mdist, int32_t* stamp, int32_t n, const int32
     stamp[i] = -1;
   if (blockIdx.x == 0) {
     int32_t* c = reinterpret_cast<int32_t*>(ctl);
-    for (int i = threadIdx.x; i < 4 * 32; i += blockDim.x) c[i] = 0;
+    for (int i = threadIdx.x; i < 4 * kCtlInts; i += blockDim.x) c[i] = 0;
     unsigned long long* p = reinterpret_cast<unsigned long long*>(cnt);
     for (int i = threadIdx.x; i < (int)(sizeof(Counters) / 8); i += blockDim.x) p[i] = 0;
   }
@@ -114,7 +114,7 @@ __global__ void k_ms_seed(int32_t* mdist, int32_t* stamp, const int32_t* __restr
 }
 
 __global__ __launch_bounds__(SMDP_WG) void k_ms_expand(MArgs a, int r) {
-  if (blockIdx.x == 0 && threadIdx.x < 32) reinterpret_cast<int32_t*>(a.ctl + ((r + 2) & 3))[threadIdx.x] = 0;
+  ctl_clear(a.ctl, (r + 2) & 3);
   Ctl* cin = a.ctl + (r & 3);
   Ctl* cout = a.ctl + ((r + 1) & 3);
   const int L = cin->qlen;
@@ -173,11 +173,15 @@ __global__ __launch_bounds__(SMDP_WG) void k_ms_drain(MArgs a, int r) {
   WaveOut<kMsOut> out;
   out.init(s_out + wave * kMsOut);
   unsigned long long edges = 0, wins = 0;
+  int part = 0;  // lane 0: claim cursor of this wave
   for (;;) {
-    int idx = 0;
-    if (lane == 0) idx = atomicAdd(&cin->chunk_next, 1);
+    int idx = -1;
+    if (lane == 0) {
+      int f = 0, e = 0;
+      if (claim_items(cin->chunks, H, 1, part, &f, &e)) idx = f;
+    }
     idx = __shfl(idx, 0);
-    if (idx >= H) break;
+    if (idx < 0) break;
     const Chunk c = a.hq[idx];
     const int e = min(a.row[c.v + 1], c.first + a.chunk);
     const int du = a.mdist[(int64_t)c.v * kLanes + lane];

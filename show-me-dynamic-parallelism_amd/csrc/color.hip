@@ -158,7 +158,7 @@ __global__ void k_color_init(int32_t* color, int32_t n, Ctl* ctl, Counters* cnt)
     color[i] = 0;
   if (blockIdx.x == 0) {
     int32_t* c = reinterpret_cast<int32_t*>(ctl);
-    for (int i = threadIdx.x; i < 4 * 32; i += blockDim.x) c[i] = 0;
+    for (int i = threadIdx.x; i < 4 * kCtlInts; i += blockDim.x) c[i] = 0;
     unsigned long long* k = reinterpret_cast<unsigned long long*>(cnt);
     for (int i = threadIdx.x; i < (int)(sizeof(Counters) / 8); i += blockDim.x) k[i] = 0;
     __syncthreads();
@@ -179,7 +179,7 @@ __device__ __forceinline__ bool settle(const CArgs& a, int v, int c, const ScanS
 
 template <int SOL>
 __global__ __launch_bounds__(SMDP_WG) void k_color(CArgs a, int r) {
-  if (blockIdx.x == 0 && threadIdx.x < 32) reinterpret_cast<int32_t*>(a.ctl + ((r + 2) & 3))[threadIdx.x] = 0;
+  ctl_clear(a.ctl, (r + 2) & 3);
   Ctl* cin = a.ctl + (r & 3);
   Ctl* cout = a.ctl + ((r + 1) & 3);
   const int L = cin->qlen;
@@ -320,12 +320,16 @@ __global__ __launch_bounds__(SMDP_WG) void k_color_grid(CArgs a, int r) {
   int32_t* qout = (r & 1) ? a.q0 : a.q1;
   __shared__ int s_idx, s_first, s_max;
   unsigned long long edges = 0, pushes = 0;
+  int part = 0;  // thread 0: claim cursor
   for (;;) {
-    if (threadIdx.x == 0) s_idx = atomicAdd(&cin->chunk_next, 1);
+    if (threadIdx.x == 0) {
+      int f = 0, e = 0;
+      s_idx = claim_items(cin->chunks, H, 1, part, &f, &e) ? f : -1;
+    }
     __syncthreads();
     const int idx = s_idx;
     __syncthreads();
-    if (idx >= H) break;
+    if (idx < 0) break;
     const int v = a.hq[idx];
     ScanState s{r == 0 ? a.row[v] : a.resume[v], r == 0 ? 0 : a.rmax[v]};
     const bool done = wg_scan(a, v, a.prio ? a.prio[v] : 0u, a.row[v + 1], c, s, edges, &s_first, &s_max);
@@ -387,7 +391,7 @@ __device__ __forceinline__ bool lprec(const LArgs& a, int u, uint32_t pu, int v)
 
 __global__ void k_layer_init(Ctl* ctl, Counters* cnt) {
   int32_t* c = reinterpret_cast<int32_t*>(ctl);
-  for (int i = threadIdx.x; i < 4 * 32; i += blockDim.x) c[i] = 0;
+  for (int i = threadIdx.x; i < 4 * kCtlInts; i += blockDim.x) c[i] = 0;
   unsigned long long* k = reinterpret_cast<unsigned long long*>(cnt);
   for (int i = threadIdx.x; i < (int)(sizeof(Counters) / 8); i += blockDim.x) k[i] = 0;
 }
@@ -463,12 +467,16 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_count_chunks(LArgs a) {
   if ((int)blockIdx.x >= H) return;
   __shared__ int s_idx;
   unsigned long long edges = 0;
+  int part = 0;  // thread 0: claim cursor
   for (;;) {
-    if (threadIdx.x == 0) s_idx = atomicAdd(&cs->chunk_next, 1);
+    if (threadIdx.x == 0) {
+      int f = 0, e = 0;
+      s_idx = claim_items(cs->chunks, H, 1, part, &f, &e) ? f : -1;
+    }
     __syncthreads();
     const int idx = s_idx;
     __syncthreads();
-    if (idx >= H) break;
+    if (idx < 0) break;
     const Chunk ch = a.hq[idx];
     const int e = min(a.row[ch.v + 1], ch.first + a.chunk);
     const int c = wave_sum(count_preceding(a, ch.v, a.prio ? a.prio[ch.v] : 0u, ch.first, e, SMDP_WG,
@@ -509,7 +517,7 @@ __device__ __forceinline__ bool notify(const LArgs& a, int u, uint32_t pu, int v
 // Round r: the queue holds the vertices of colour r+1.  Edge-balanced tiles
 // of 256 queue entries; transposed rows of >= split arcs go to grid chunks.
 __global__ __launch_bounds__(SMDP_WG) void k_layer_expand(LArgs a, int r) {
-  if (blockIdx.x == 0 && threadIdx.x < 32) reinterpret_cast<int32_t*>(a.ctl + ((r + 2) & 3))[threadIdx.x] = 0;
+  ctl_clear(a.ctl, (r + 2) & 3);
   Ctl* cin = a.ctl + (r & 3);
   Ctl* cout = a.ctl + ((r + 1) & 3);
   const int L = cin->qlen;
@@ -531,12 +539,17 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_expand(LArgs a, int r) {
   WaveOut<kCOut> out;
   out.init(s_out + (threadIdx.x / SMDP_WAVE) * kCOut);
   unsigned long long edges = 0, verts = 0, pushes = 0, pg = 0;
+  const int ntiles = (L + SMDP_WG - 1) / SMDP_WG;
+  int part = 0;  // thread 0: claim cursor
   for (;;) {
-    if (threadIdx.x == 0) s_tile = atomicAdd(&cin->tile_next, 1);
+    if (threadIdx.x == 0) {
+      int f = 0, e = 0;
+      s_tile = claim_items(cin->tiles, ntiles, 1, part, &f, &e) ? f : -1;
+    }
     __syncthreads();
     const int tile = s_tile;
     __syncthreads();
-    if (tile * SMDP_WG >= L) break;
+    if (tile < 0) break;
     const int i = tile * SMDP_WG + (int)threadIdx.x;
     const bool valid = i < L;
     int u = 0, s = 0, deg = 0;
@@ -609,12 +622,16 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_drain(LArgs a, int r) {
   WaveOut<kCOut> out;
   out.init(s_out + (threadIdx.x / SMDP_WAVE) * kCOut);
   unsigned long long edges = 0, pushes = 0;
+  int part = 0;  // thread 0: claim cursor
   for (;;) {
-    if (threadIdx.x == 0) s_idx = atomicAdd(&cin->chunk_next, 1);
+    if (threadIdx.x == 0) {
+      int f = 0, e = 0;
+      s_idx = claim_items(cin->chunks, H, 1, part, &f, &e) ? f : -1;
+    }
     __syncthreads();
     const int idx = s_idx;
     __syncthreads();
-    if (idx >= H) break;
+    if (idx < 0) break;
     const Chunk ch = a.hq[idx];
     const int e = min(a.trow[ch.v + 1], ch.first + a.chunk);
     const uint32_t pu = a.prio ? a.prio[ch.v] : 0u;

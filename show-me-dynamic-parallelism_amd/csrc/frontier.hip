@@ -92,29 +92,27 @@ __device__ __forceinline__ int arc_weight(const FArgs& a, int e) {
 // flight together); relax() reads it itself.
 // A plain read of dist[v] may be stale (larger) when another XCD lowered it;
 // the atomic decides, so a stale read only costs an extra atomic.
+// BFS: every lane of round r offers level r+1, so only the lane that finds v
+// unvisited lowers it; that atomic alone makes the claim unique and the stamp
+// is not needed.
+template <bool BFS>
 __device__ __forceinline__ bool relax_seen(const FArgs& a, int v, int alt, int seen, int label, Sink& out) {
   if (alt >= seen) return false;
   ++out.tries;
   int old = atomicMin(&a.dist[v], alt);
   if (alt >= old) return false;
   ++out.wins;
+  if (BFS) return true;
   if (a.stamp[v] == label) return false;
   return atomicExch(&a.stamp[v], label) != label;
 }
 
+template <bool BFS>
 __device__ __forceinline__ bool relax(const FArgs& a, int v, int alt, int label, Sink& out) {
-  if (alt >= a.dist[v]) return false;
-  ++out.tries;
-  int old = atomicMin(&a.dist[v], alt);
-  if (alt >= old) return false;
-  ++out.wins;
-  if (a.stamp[v] == label) return false;
-  return atomicExch(&a.stamp[v], label) != label;
+  return relax_seen<BFS>(a, v, alt, a.dist[v], label, out);
 }
 
-__device__ __forceinline__ void clear_slot(Ctl* ctl, int r) {
-  if (blockIdx.x == 0 && threadIdx.x < 32) reinterpret_cast<int32_t*>(ctl + ((r + 2) & 3))[threadIdx.x] = 0;
-}
+__device__ __forceinline__ void clear_slot(Ctl* ctl, int r) { ctl_clear(ctl, (r + 2) & 3); }
 
 __device__ __forceinline__ void flush_counters(Counters* cnt, unsigned long long edges, unsigned long long verts,
                                                unsigned long long p0, unsigned long long p1,
@@ -153,7 +151,7 @@ __device__ __forceinline__ void wave_expand(const FArgs& a, int s, int e, int du
     bool p = false;
     if (k < e) {
       v = a.col[k];
-      p = relax(a, v, du + arc_weight<BFS>(a, k), label, out);
+      p = relax<BFS>(a, v, du + arc_weight<BFS>(a, k), label, out);
       ++edges;
     }
     out.push(p, v);
@@ -178,7 +176,7 @@ __device__ __forceinline__ void wg_expand(const FArgs& a, int s, int e, int du, 
 #pragma unroll
     for (int t = 0; t < kIlp; ++t) {
       const bool ok = b + (int)threadIdx.x + t * SMDP_WG < e;
-      const bool p = ok && relax_seen(a, v[t], alt[t], seen[t], label, out);
+      const bool p = ok && relax_seen<BFS>(a, v[t], alt[t], seen[t], label, out);
       edges += ok;
       out.push(p, v[t]);
     }
@@ -193,7 +191,7 @@ __global__ void k_init_frontier(int32_t* dist, int32_t* stamp, int32_t n, int32_
   }
   if (blockIdx.x == 0) {
     int32_t* c = reinterpret_cast<int32_t*>(ctl);
-    for (int i = threadIdx.x; i < 4 * 32; i += blockDim.x) c[i] = 0;
+    for (int i = threadIdx.x; i < 4 * kCtlInts; i += blockDim.x) c[i] = 0;
     unsigned long long* k = reinterpret_cast<unsigned long long*>(cnt);
     for (int i = threadIdx.x; i < (int)(sizeof(Counters) / 8); i += blockDim.x) k[i] = 0;
     __syncthreads();
@@ -241,7 +239,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_expand_lane(FArgs a, int r) {
       u = qin[i];
       s = a.row[u];
       e = a.row[u + 1];
-      du = a.dist[u];
+      du = BFS ? r : a.dist[u];  // BFS: queue r holds exactly the vertices of level r
       ++verts;
     }
     const bool heavy = SOL != SMDP_SOL_FLAT && valid && (e - s) >= a.heavy;
@@ -269,7 +267,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_expand_lane(FArgs a, int r) {
         if (j < deg) {
           const int k = s + j;
           v = a.col[k];
-          p = relax(a, v, du + arc_weight<BFS>(a, k), label, out);
+          p = relax<BFS>(a, v, du + arc_weight<BFS>(a, k), label, out);
           ++edges;
         }
         out.push(p, v);
@@ -356,7 +354,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_expand_tile(FArgs a, int r) {
   const int label = r + 1;
   const int lane = lane_id();
 
-  __shared__ int s_tile;
+  __shared__ int s_tile, s_tend;
   __shared__ int s_start[SMDP_WG];
   __shared__ int s_du[SMDP_WG];
   __shared__ int s_off[SMDP_WG];
@@ -366,13 +364,19 @@ __global__ __launch_bounds__(SMDP_WG) void k_expand_tile(FArgs a, int r) {
   out.init(s_out + (threadIdx.x / SMDP_WAVE) * kOutCap, qout, &cout->qlen);
 
   unsigned long long edges = 0, verts = 0, pg = 0;
+  const int ntiles = (L + SMDP_WG - 1) / SMDP_WG;
+  int part = 0;  // thread 0: claim cursor
   for (;;) {
-    if (threadIdx.x == 0) s_tile = atomicAdd(&cin->tile_next, kTileGrab);
+    if (threadIdx.x == 0) {
+      int f = 0, e = 0;
+      s_tile = claim_items(cin->tiles, ntiles, kTileGrab, part, &f, &e) ? f : -1;
+      s_tend = e;
+    }
     __syncthreads();
-    const int first = s_tile;
+    const int first = s_tile, tend = s_tend;
     __syncthreads();
-    if (first * SMDP_WG >= L) break;
-    for (int tile = first; tile < first + kTileGrab && tile * SMDP_WG < L; ++tile) {
+    if (first < 0) break;
+    for (int tile = first; tile < tend; ++tile) {
     const int base = tile * SMDP_WG;
     const int i = base + (int)threadIdx.x;
     const bool valid = i < L;
@@ -381,7 +385,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_expand_tile(FArgs a, int r) {
       u = qin[i];
       s = a.row[u];
       deg = a.row[u + 1] - s;
-      du = a.dist[u];
+      du = BFS ? r : a.dist[u];  // BFS: queue r holds exactly the vertices of level r
       ++verts;
     }
     // A short queue has too few tiles to keep every workgroup busy, and its
@@ -425,7 +429,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_expand_tile(FArgs a, int r) {
 #pragma unroll
       for (int t = 0; t < kIlp; ++t) {
         const bool ok = jb + (int)threadIdx.x + t * SMDP_WG < tot;
-        const bool p = ok && relax_seen(a, v[t], alt[t], seen[t], label, out);
+        const bool p = ok && relax_seen<BFS>(a, v[t], alt[t], seen[t], label, out);
         edges += ok;
         out.push(p, v[t]);
       }
@@ -447,22 +451,27 @@ __global__ __launch_bounds__(SMDP_WG) void k_drain(FArgs a, int r) {
   if ((int)blockIdx.x * kChunkGrab >= H) return;
   int32_t* qout = (r & 1) ? a.q0 : a.q1;
   const int label = r + 1;
-  __shared__ int s_idx;
+  __shared__ int s_idx, s_iend;
   __shared__ int32_t s_out[SMDP_WAVES_PER_WG * kOutCap];
   Sink out;
   out.init(s_out + (threadIdx.x / SMDP_WAVE) * kOutCap, qout, &cout->qlen);
   unsigned long long edges = 0;
+  int part = 0;  // thread 0: claim cursor
   for (;;) {
-    if (threadIdx.x == 0) s_idx = atomicAdd(&cin->chunk_next, kChunkGrab);
+    if (threadIdx.x == 0) {
+      int f = 0, e = 0;
+      s_idx = claim_items(cin->chunks, H, kChunkGrab, part, &f, &e) ? f : -1;
+      s_iend = e;
+    }
     __syncthreads();
-    const int first = s_idx;
+    const int first = s_idx, iend = s_iend;
     __syncthreads();
-    if (first >= H) break;
-    for (int idx = first; idx < min(first + kChunkGrab, H); ++idx) {
+    if (first < 0) break;
+    for (int idx = first; idx < iend; ++idx) {
       const Chunk c = a.hq[idx];
       const int vend = a.row[c.v + 1];
       const int end = (a.chunk > 0 && vend - c.first > a.chunk) ? c.first + a.chunk : vend;
-      wg_expand<BFS>(a, c.first, end, a.dist[c.v], label, edges, out);
+      wg_expand<BFS>(a, c.first, end, BFS ? r : a.dist[c.v], label, edges, out);
     }
   }
   log_round(a, r, -1, wave_sum(edges));

@@ -13,6 +13,11 @@ namespace smdp {
 // Per-round control block.  Rounds use slots r&3: round r reads slot r&3,
 // appends to slot (r+1)&3 and clears slot (r+2)&3, which no kernel of round r
 // touches.  Padded to a cache line so slots do not share lines.
+// Work is claimed from kClaimParts counters, one per 128-byte line: one
+// counter serialises about 85M claims/s on MI355X, eight about 500M/s.
+constexpr int kClaimParts = 8;
+constexpr int kClaimStride = 32;  // ints between two counters
+
 struct Ctl {
   int32_t qlen;        // frontier entries of this round
   int32_t hlen;        // grid-queue entries pushed during this round
@@ -20,8 +25,42 @@ struct Ctl {
   int32_t chunk_next;  // dynamic grid-queue counter (phase 2)
   int32_t ticket;      // workgroups finished (last-workgroup reductions)
   int32_t pad[27];
+  int32_t tiles[kClaimParts * kClaimStride];   // partitioned tile counters (phase 1)
+  int32_t chunks[kClaimParts * kClaimStride];  // partitioned grid-queue counters (phase 2)
 };
-static_assert(sizeof(Ctl) == 128, "Ctl must be one 128-byte line");
+static_assert(sizeof(Ctl) % 128 == 0, "Ctl is made of 128-byte lines");
+constexpr int kCtlInts = (int)(sizeof(Ctl) / sizeof(int32_t));
+
+// Items [0, total) in kClaimParts equal ranges, one counter each (ctr[p *
+// kClaimStride]).  A workgroup takes up to `grab` items at a time from range
+// (blockIdx.x + k) % kClaimParts, starting at k = 0 (workgroups of one XCD
+// share a counter first) and moving on when a range is exhausted; a plain read
+// skips exhausted ranges without an atomic.  Called by one thread; returns
+// false when every range is exhausted, else the items [*first, *end).
+__device__ __forceinline__ bool claim_items(int32_t* ctr, int total, int grab, int& k, int* first, int* end) {
+  for (; k < kClaimParts; ++k) {
+    const int p = ((int)blockIdx.x + k) % kClaimParts;
+    const int lo = (int)((int64_t)total * p / kClaimParts);
+    const int size = (int)((int64_t)total * (p + 1) / kClaimParts) - lo;
+    int32_t* c = ctr + p * kClaimStride;
+    if (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= size) continue;
+    const int t = atomicAdd(c, grab);
+    if (t < size) {
+      *first = lo + t;
+      *end = lo + min(t + grab, size);
+      return true;
+    }
+  }
+  return false;
+}
+
+// Zeroes control slot `slot` (block 0 only; every thread of it calls this).
+__device__ __forceinline__ void ctl_clear(Ctl* ctl, int slot) {
+  if (blockIdx.x != 0) return;
+  int32_t* c = reinterpret_cast<int32_t*>(ctl + slot);
+  for (int i = threadIdx.x; i < kCtlInts; i += blockDim.x) c[i] = 0;
+}
+
 
 // Work counters, accumulated with one atomic per wave.
 struct Counters {
