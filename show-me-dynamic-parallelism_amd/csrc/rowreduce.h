@@ -163,7 +163,25 @@ __device__ __forceinline__ CSum wg_csum(CSum v, float2* lds) {
 template <bool PR>
 __device__ __forceinline__ CSum wave_row(const RArgs& a, int s, int e) {
   CSum acc;
-  for (int k = s + lane_id(); k < e; k += SMDP_WAVE) add_term<PR>(a, acc, k);
+  int k = s + lane_id();
+  // four steps' loads are issued before their terms are added (same order)
+  for (; k + 3 * SMDP_WAVE < e; k += 4 * SMDP_WAVE) {
+    int c[4];
+    float g[4], va[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) c[t] = stream_load(a.col + k + t * SMDP_WAVE);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      g[t] = a.x[c[t]];
+      va[t] = PR ? 0.f : stream_load(a.val + k + t * SMDP_WAVE);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      if (PR) acc.add(g[t]);
+      else acc.add_prod(va[t], g[t]);
+    }
+  }
+  for (; k < e; k += SMDP_WAVE) add_term<PR>(a, acc, k);
   acc = wave_csum(acc);
   CSum out;
   out.s = __shfl(acc.s, 0);
