@@ -95,11 +95,31 @@ int prof_collect(smdp_ctx* ctx, smdp_stats* st) {
   return SMDP_OK;
 }
 
+int read_counters(const Counters* dev, hipStream_t s, Counters* out) {
+  Counters parts[kCntParts];
+  SMDP_HIP(hipMemcpyAsync(parts, dev, sizeof parts, hipMemcpyDeviceToHost, s));
+  SMDP_HIP(hipStreamSynchronize(s));
+  Counters c;
+  memset(&c, 0, sizeof c);
+  for (const Counters& p : parts) {
+    c.edges += p.edges;
+    c.verts += p.verts;
+    for (int i = 0; i < 4; ++i) c.push[i] += p.push[i];
+    c.rounds = std::max(c.rounds, p.rounds);
+    c.reached += p.reached;
+    c.done = std::max(c.done, p.done);
+    c.edges2 += p.edges2;
+    c.atom_try += p.atom_try;
+    c.atom_win += p.atom_win;
+  }
+  *out = c;
+  return SMDP_OK;
+}
+
 int fill_stats(smdp_graph* g, smdp_stats* st, double ms_kernel, uint64_t launches) {
   if (!st) return SMDP_OK;
   Counters c;
-  SMDP_HIP(hipMemcpyAsync(&c, g->cnt, sizeof c, hipMemcpyDeviceToHost, g->ctx->stream));
-  SMDP_HIP(hipStreamSynchronize(g->ctx->stream));
+  SMDP_CHECK(read_counters(g->cnt, g->ctx->stream, &c));
   st->ms_kernel = ms_kernel;
   st->edges_relaxed = c.edges;
   st->vertices_expanded = c.verts;
@@ -262,7 +282,7 @@ int smdp_graph_create(smdp_ctx* ctx, const smdp_csr* h, smdp_graph** out) {
   if (rc == SMDP_OK) rc = up(&g->col, h->col, h->m);
   if (rc == SMDP_OK && h->w) rc = up(&g->w, h->w, h->m);
   if (rc == SMDP_OK) rc = alloc_once(&g->ctl, 4);
-  if (rc == SMDP_OK) rc = alloc_once(&g->cnt, 1);
+  if (rc == SMDP_OK) rc = alloc_once(&g->cnt, kCntParts);
   if (rc != SMDP_OK) {
     smdp_graph_destroy(g);
     return rc;

@@ -98,7 +98,7 @@ __global__ void k_ms_init(int32_t* mdist, int32_t* stamp, int32_t n, const int32
     int32_t* c = reinterpret_cast<int32_t*>(ctl);
     for (int i = threadIdx.x; i < 4 * kCtlInts; i += blockDim.x) c[i] = 0;
     unsigned long long* p = reinterpret_cast<unsigned long long*>(cnt);
-    for (int i = threadIdx.x; i < (int)(sizeof(Counters) / 8); i += blockDim.x) p[i] = 0;
+    for (int i = threadIdx.x; i < (int)(kCntParts * sizeof(Counters) / 8); i += blockDim.x) p[i] = 0;
   }
 }
 
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_ms_expand(MArgs a, int r) {
   Ctl* cout = a.ctl + ((r + 1) & 3);
   const int L = cin->qlen;
   if (L == 0) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&a.cnt->rounds, (unsigned long long)(r + 1));
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&cnt_of(a.cnt)->rounds, (unsigned long long)(r + 1));
   const int32_t* qin = (r & 1) ? a.q1 : a.q0;
   int32_t* qout = (r & 1) ? a.q0 : a.q1;
   const int label = r + 1;
@@ -153,10 +153,10 @@ __global__ __launch_bounds__(SMDP_WG) void k_ms_expand(MArgs a, int r) {
   wins = wave_sum(wins);
   pg = wave_sum(pg);
   if (lane == 0) {
-    if (edges) atomicAdd(&a.cnt->edges, edges);
-    if (verts) atomicAdd(&a.cnt->verts, verts);
-    if (wins) atomicAdd(&a.cnt->atom_win, wins);
-    if (pg) atomicAdd(&a.cnt->push[3], pg);
+    if (edges) atomicAdd(&cnt_of(a.cnt)->edges, edges);
+    if (verts) atomicAdd(&cnt_of(a.cnt)->verts, verts);
+    if (wins) atomicAdd(&cnt_of(a.cnt)->atom_win, wins);
+    if (pg) atomicAdd(&cnt_of(a.cnt)->push[3], pg);
   }
 }
 
@@ -192,10 +192,10 @@ __global__ __launch_bounds__(SMDP_WG) void k_ms_drain(MArgs a, int r) {
   wins = wave_sum(wins);
   if (lane == 0) {
     if (edges) {
-      atomicAdd(&a.cnt->edges, edges);
-      atomicAdd(&a.cnt->edges2, edges);
+      atomicAdd(&cnt_of(a.cnt)->edges, edges);
+      atomicAdd(&cnt_of(a.cnt)->edges2, edges);
     }
-    if (wins) atomicAdd(&a.cnt->atom_win, wins);
+    if (wins) atomicAdd(&cnt_of(a.cnt)->atom_win, wins);
   }
 }
 
@@ -240,7 +240,7 @@ constexpr int kMgOut = 256;
 __global__ __launch_bounds__(SMDP_WG) void k_mg_reset(int32_t* next, Counters* cnt) {
   if (threadIdx.x == 0) *next = 0;
   unsigned long long* p = reinterpret_cast<unsigned long long*>(cnt);
-  for (int i = threadIdx.x; i < (int)(sizeof(Counters) / 8); i += blockDim.x) p[i] = 0;
+  for (int i = threadIdx.x; i < (int)(kCntParts * sizeof(Counters) / 8); i += blockDim.x) p[i] = 0;
 }
 
 // One workgroup of NT threads solves whole graphs, one after another.  Each
@@ -356,8 +356,8 @@ __global__ __launch_bounds__(NT) void k_mg(GArgs a) {
   edges = wave_sum(edges);
   verts = wave_sum(verts);
   if (lane_id() == 0) {
-    if (edges) atomicAdd(&a.cnt->edges, edges);
-    if (verts) atomicAdd(&a.cnt->verts, verts);
+    if (edges) atomicAdd(&cnt_of(a.cnt)->edges, edges);
+    if (verts) atomicAdd(&cnt_of(a.cnt)->verts, verts);
   }
 }
 
@@ -454,7 +454,7 @@ int run_multi_source(smdp_graph* g, const int32_t* srcs, int32_t k, int32_t* hos
     }
     ms_total += ms;
     Counters c;
-    if (hipMemcpy(&c, g->cnt, sizeof c, hipMemcpyDeviceToHost) != hipSuccess) {
+    if (read_counters(g->cnt, s, &c) != SMDP_OK) {
       rc = SMDP_ERR_DEVICE;
       break;
     }
@@ -613,7 +613,7 @@ int smdp_batch_create(smdp_ctx* ctx, const smdp_csr* graphs, int32_t ng, const i
   up32(&b->d_qb, nullptr, vtot);
   up32(&b->d_next, nullptr, 1);
   up32(&b->d_status, nullptr, ng);
-  if (rc == SMDP_OK && hipMalloc((void**)&b->d_cnt, sizeof(Counters)) != hipSuccess) rc = SMDP_ERR_NOMEM;
+  if (rc == SMDP_OK && hipMalloc((void**)&b->d_cnt, kCntParts * sizeof(Counters)) != hipSuccess) rc = SMDP_ERR_NOMEM;
   if (rc != SMDP_OK) {
     smdp_batch_destroy(b);
     return rc;
@@ -673,7 +673,7 @@ int smdp_batch_run(smdp_batch* b, smdp_stats* st) {
   std::vector<int32_t> status(b->ng);
   SMDP_HIP(hipMemcpy(status.data(), b->d_status, sizeof(int32_t) * b->ng, hipMemcpyDeviceToHost));
   Counters c;
-  SMDP_HIP(hipMemcpy(&c, b->d_cnt, sizeof c, hipMemcpyDeviceToHost));
+  SMDP_CHECK(read_counters(b->d_cnt, s, &c));
   int rc = SMDP_OK;
   uint64_t rounds = 0;
   for (int32_t v : status) {

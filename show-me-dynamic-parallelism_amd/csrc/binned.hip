@@ -359,7 +359,7 @@ __global__ __launch_bounds__(kBinNT, 2) void k_bin_reduce(RArgs a, BArgs b, int 
       }
       epilogue<true>(a, b.wrow[i], h.value(), base, res, dang);
     }
-    if (threadIdx.x == 0 && b.nwide) atomicAdd(&a.cnt->verts, (unsigned long long)b.nwide);
+    if (threadIdx.x == 0 && b.nwide) atomicAdd(&cnt_of(a.cnt)->verts, (unsigned long long)b.nwide);
     finish_pass<true, kBinNT>(a, k, make_double2(res, dang), s_red2);
   }
 }

@@ -160,7 +160,7 @@ __global__ void k_color_init(int32_t* color, int32_t n, Ctl* ctl, Counters* cnt)
     int32_t* c = reinterpret_cast<int32_t*>(ctl);
     for (int i = threadIdx.x; i < 4 * kCtlInts; i += blockDim.x) c[i] = 0;
     unsigned long long* k = reinterpret_cast<unsigned long long*>(cnt);
-    for (int i = threadIdx.x; i < (int)(sizeof(Counters) / 8); i += blockDim.x) k[i] = 0;
+    for (int i = threadIdx.x; i < (int)(kCntParts * sizeof(Counters) / 8); i += blockDim.x) k[i] = 0;
     __syncthreads();
     if (threadIdx.x == 0) ctl[0].qlen = n;  // round 0 takes every vertex, in id order
   }
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_color(CArgs a, int r) {
   Ctl* cout = a.ctl + ((r + 1) & 3);
   const int L = cin->qlen;
   if (L == 0) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&a.cnt->rounds, (unsigned long long)(r + 1));
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&cnt_of(a.cnt)->rounds, (unsigned long long)(r + 1));
   const int c = r + 1;
   const int32_t* qin = (r & 1) ? a.q1 : a.q0;
   int32_t* qout = (r & 1) ? a.q0 : a.q1;
@@ -300,12 +300,12 @@ __global__ __launch_bounds__(SMDP_WG) void k_color(CArgs a, int r) {
   pb = wave_sum(pb);
   pg = wave_sum(pg);
   if (lane == 0) {
-    if (edges) atomicAdd(&a.cnt->edges, edges);
-    if (verts) atomicAdd(&a.cnt->verts, verts);
-    if (pushes) atomicAdd(&a.cnt->push[0], pushes);
-    if (pw) atomicAdd(&a.cnt->push[1], pw);
-    if (pb) atomicAdd(&a.cnt->push[2], pb);
-    if (pg) atomicAdd(&a.cnt->push[3], pg);
+    if (edges) atomicAdd(&cnt_of(a.cnt)->edges, edges);
+    if (verts) atomicAdd(&cnt_of(a.cnt)->verts, verts);
+    if (pushes) atomicAdd(&cnt_of(a.cnt)->push[0], pushes);
+    if (pw) atomicAdd(&cnt_of(a.cnt)->push[1], pw);
+    if (pb) atomicAdd(&cnt_of(a.cnt)->push[2], pb);
+    if (pg) atomicAdd(&cnt_of(a.cnt)->push[3], pg);
   }
 }
 
@@ -344,10 +344,10 @@ __global__ __launch_bounds__(SMDP_WG) void k_color_grid(CArgs a, int r) {
   pushes = wave_sum(pushes);
   if (lane_id() == 0) {
     if (edges) {
-      atomicAdd(&a.cnt->edges, edges);
-      atomicAdd(&a.cnt->edges2, edges);
+      atomicAdd(&cnt_of(a.cnt)->edges, edges);
+      atomicAdd(&cnt_of(a.cnt)->edges2, edges);
     }
-    if (pushes) atomicAdd(&a.cnt->push[0], pushes);
+    if (pushes) atomicAdd(&cnt_of(a.cnt)->push[0], pushes);
   }
 }
 
@@ -393,7 +393,7 @@ __global__ void k_layer_init(Ctl* ctl, Counters* cnt) {
   int32_t* c = reinterpret_cast<int32_t*>(ctl);
   for (int i = threadIdx.x; i < 4 * kCtlInts; i += blockDim.x) c[i] = 0;
   unsigned long long* k = reinterpret_cast<unsigned long long*>(cnt);
-  for (int i = threadIdx.x; i < (int)(sizeof(Counters) / 8); i += blockDim.x) k[i] = 0;
+  for (int i = threadIdx.x; i < (int)(kCntParts * sizeof(Counters) / 8); i += blockDim.x) k[i] = 0;
 }
 
 // Arcs of row(v) [s, e) pointing to a vertex that precedes v.
@@ -458,7 +458,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_count(LArgs a) {
     }
   }
   edges = wave_sum(edges);
-  if (lane == 0 && edges) atomicAdd(&a.cnt->edges, edges);
+  if (lane == 0 && edges) atomicAdd(&cnt_of(a.cnt)->edges, edges);
 }
 
 __global__ __launch_bounds__(SMDP_WG) void k_layer_count_chunks(LArgs a) {
@@ -485,7 +485,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_count_chunks(LArgs a) {
     if (threadIdx.x == 0) edges += e - ch.first;
   }
   edges = wave_sum(edges);
-  if (lane_id() == 0 && edges) atomicAdd(&a.cnt->edges, edges);
+  if (lane_id() == 0 && edges) atomicAdd(&cnt_of(a.cnt)->edges, edges);
 }
 
 // Colour 1 for the vertices without preceding neighbours: the queue of round 0.
@@ -505,26 +505,70 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_seed(LArgs a) {
   out.flush(a.q0, &a.ctl[0].qlen);
 }
 
-// One arc u -> v of the transposed CSR: the lane that brings pend[v] to zero
-// colours v and queues it.
-__device__ __forceinline__ bool notify(const LArgs& a, int u, uint32_t pu, int v, int newc) {
-  if (v == u || !lprec(a, u, pu, v)) return false;
-  if (atomicSub(&a.pend[v], 1) != 1) return false;
-  a.color[v] = newc;
-  return true;
+// Arcs u -> v[t] of the transposed CSR (ok[t]: the arc exists).  All
+// priorities are gathered before the first atomic; the lane that brings
+// pend[v] to zero colours v and queues it (won[t]).
+template <int K>
+__device__ __forceinline__ void notify(const LArgs& a, const int* u, const uint32_t* pu, const int* v,
+                                       const bool* ok, int newc, bool* won) {
+  bool want[K];
+  if (a.prio) {
+    uint32_t pv[K];
+#pragma unroll
+    for (int t = 0; t < K; ++t) pv[t] = ok[t] ? a.prio[v[t]] : 0u;
+#pragma unroll
+    for (int t = 0; t < K; ++t) want[t] = ok[t] && (pu[t] < pv[t] || (pu[t] == pv[t] && u[t] < v[t]));
+  } else {
+#pragma unroll
+    for (int t = 0; t < K; ++t) want[t] = ok[t] && u[t] < v[t];
+  }
+  int old[K];
+#pragma unroll
+  for (int t = 0; t < K; ++t) old[t] = want[t] ? atomicSub(&a.pend[v[t]], 1) : 0;
+#pragma unroll
+  for (int t = 0; t < K; ++t) {
+    won[t] = old[t] == 1;
+    if (won[t]) a.color[v[t]] = newc;
+  }
+}
+
+// Shape of round r's work, from its queue length L and the grid (the expand
+// and drain kernels of a round run on the same grid, so both derive the same
+// values).  Tiles hold 256 queue entries, halved while the tiles still number
+// at most one per workgroup: a short layer (RMAT-23 has ~2300 layers of about
+// 1200 vertices) then spreads over the whole grid instead of a few
+// workgroups.  In such short rounds rows are cut into chunks of at most one
+// pass of the workgroup (kLayerIlp * 256 arcs).
+struct LayerShape {
+  int tile, split, chunk;
+};
+constexpr int kLayerShortChunk = kLayerIlp * SMDP_WG;
+__device__ __forceinline__ LayerShape layer_shape(const LArgs& a, int L, int grid) {
+  LayerShape sh;
+  sh.tile = SMDP_WG;
+  while (sh.tile > 1 && (L + sh.tile / 2 - 1) / (sh.tile / 2) <= grid) sh.tile >>= 1;
+  const bool short_round = sh.tile < SMDP_WG;
+  sh.split = short_round ? min(a.split, kLayerShortChunk) : a.split;
+  sh.chunk = short_round ? min(a.chunk, kLayerShortChunk) : a.chunk;
+  return sh;
 }
 
 // Round r: the queue holds the vertices of colour r+1.  Edge-balanced tiles
-// of 256 queue entries; transposed rows of >= split arcs go to grid chunks.
+// of queue entries (layer_shape); transposed rows of >= split arcs go to grid
+// chunks.
 __global__ __launch_bounds__(SMDP_WG) void k_layer_expand(LArgs a, int r) {
   ctl_clear(a.ctl, (r + 2) & 3);
   Ctl* cin = a.ctl + (r & 3);
   Ctl* cout = a.ctl + ((r + 1) & 3);
   const int L = cin->qlen;
   if (L == 0) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&a.cnt->rounds, (unsigned long long)(r + 1));
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&cnt_of(a.cnt)->rounds, (unsigned long long)(r + 1));
+  const LayerShape sh = layer_shape(a, L, (int)gridDim.x);
+  const int ntiles = (L + sh.tile - 1) / sh.tile;
   // workgroups past the tile count would only contend for the counter
-  if ((int)blockIdx.x * SMDP_WG >= L) return;
+  if ((int)blockIdx.x >= ntiles) return;
+  // at most one tile per workgroup: tile = workgroup, no claims
+  const bool one_each = ntiles <= (int)gridDim.x;
   const int32_t* qin = (r & 1) ? a.q1 : a.q0;
   int32_t* qout = (r & 1) ? a.q0 : a.q1;
   const int newc = r + 2;
@@ -539,19 +583,24 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_expand(LArgs a, int r) {
   WaveOut<kCOut> out;
   out.init(s_out + (threadIdx.x / SMDP_WAVE) * kCOut);
   unsigned long long edges = 0, verts = 0, pushes = 0, pg = 0;
-  const int ntiles = (L + SMDP_WG - 1) / SMDP_WG;
   int part = 0;  // thread 0: claim cursor
-  for (;;) {
-    if (threadIdx.x == 0) {
-      int f = 0, e = 0;
-      s_tile = claim_items(cin->tiles, ntiles, 1, part, &f, &e) ? f : -1;
+  for (int pass = 0;; ++pass) {
+    int tile;
+    if (one_each) {
+      if (pass) break;
+      tile = blockIdx.x;
+    } else {
+      if (threadIdx.x == 0) {
+        int f = 0, e = 0;
+        s_tile = claim_items(cin->tiles, ntiles, 1, part, &f, &e) ? f : -1;
+      }
+      __syncthreads();
+      tile = s_tile;
+      __syncthreads();
+      if (tile < 0) break;
     }
-    __syncthreads();
-    const int tile = s_tile;
-    __syncthreads();
-    if (tile < 0) break;
-    const int i = tile * SMDP_WG + (int)threadIdx.x;
-    const bool valid = i < L;
+    const int i = tile * sh.tile + (int)threadIdx.x;
+    const bool valid = (int)threadIdx.x < sh.tile && i < L;
     int u = 0, s = 0, deg = 0;
     uint32_t pu = 0;
     if (valid) {
@@ -561,15 +610,15 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_expand(LArgs a, int r) {
       if (a.prio) pu = a.prio[u];
       ++verts;
     }
-    const bool big = deg >= a.split;
-    const int nch = big ? (deg + a.chunk - 1) / a.chunk : 0;
+    const bool big = deg >= sh.split;
+    const int nch = big ? (deg + sh.chunk - 1) / sh.chunk : 0;
     const int incl = wave_inclusive_scan(nch);
     const int wtot = __shfl(incl, SMDP_WAVE - 1);
     if (wtot > 0) {
       int b0 = 0;
       if (lane == 0) b0 = atomicAdd(&cin->hlen, wtot);
       b0 = __shfl(b0, 0);
-      for (int k = 0; k < nch; ++k) a.hq[b0 + incl - nch + k] = Chunk{u, s + k * a.chunk};
+      for (int k = 0; k < nch; ++k) a.hq[b0 + incl - nch + k] = Chunk{u, s + k * sh.chunk};
       pg += nch;
     }
     s_start[threadIdx.x] = s;
@@ -580,19 +629,23 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_expand(LArgs a, int r) {
     s_off[threadIdx.x] = ex;
     __syncthreads();
     for (int jb = 0; jb < tot; jb += kLayerIlp * SMDP_WG) {
-      int j[kLayerIlp], own[kLayerIlp], v[kLayerIlp];
+      int j[kLayerIlp], own[kLayerIlp], v[kLayerIlp], uu[kLayerIlp];
+      uint32_t pu[kLayerIlp];
+      bool ok[kLayerIlp], won[kLayerIlp];
 #pragma unroll
       for (int t = 0; t < kLayerIlp; ++t) j[t] = min(jb + (int)threadIdx.x + t * SMDP_WG, tot - 1);
       tile_owners<kLayerIlp>(s_off, j, own);
 #pragma unroll
-      for (int t = 0; t < kLayerIlp; ++t) v[t] = stream_load(a.tcol + s_start[own[t]] + (j[t] - s_off[own[t]]));
-#pragma unroll
       for (int t = 0; t < kLayerIlp; ++t) {
-        const bool ok = jb + (int)threadIdx.x + t * SMDP_WG < tot;
-        const bool p = ok && notify(a, s_u[own[t]], s_pu[own[t]], v[t], newc);
-        edges += ok;
-        pushes += out.push(p, v[t], qout, &cout->qlen);
+        v[t] = stream_load(a.tcol + s_start[own[t]] + (j[t] - s_off[own[t]]));
+        uu[t] = s_u[own[t]];
+        pu[t] = s_pu[own[t]];
+        ok[t] = jb + (int)threadIdx.x + t * SMDP_WG < tot && v[t] != uu[t];
+        edges += jb + (int)threadIdx.x + t * SMDP_WG < tot;
       }
+      notify<kLayerIlp>(a, uu, pu, v, ok, newc, won);
+#pragma unroll
+      for (int t = 0; t < kLayerIlp; ++t) pushes += out.push(won[t], v[t], qout, &cout->qlen);
     }
     __syncthreads();
   }
@@ -602,10 +655,10 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_expand(LArgs a, int r) {
   pushes = wave_sum(pushes);
   pg = wave_sum(pg);
   if (lane == 0) {
-    if (edges) atomicAdd(&a.cnt->edges, edges);
-    if (verts) atomicAdd(&a.cnt->verts, verts);
-    if (pushes) atomicAdd(&a.cnt->push[0], pushes);
-    if (pg) atomicAdd(&a.cnt->push[3], pg);
+    if (edges) atomicAdd(&cnt_of(a.cnt)->edges, edges);
+    if (verts) atomicAdd(&cnt_of(a.cnt)->verts, verts);
+    if (pushes) atomicAdd(&cnt_of(a.cnt)->push[0], pushes);
+    if (pg) atomicAdd(&cnt_of(a.cnt)->push[3], pg);
   }
 }
 
@@ -615,6 +668,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_drain(LArgs a, int r) {
   Ctl* cout = a.ctl + ((r + 1) & 3);
   const int H = cin->hlen;
   if ((int)blockIdx.x >= H) return;
+  const int chunk = layer_shape(a, cin->qlen, (int)gridDim.x).chunk;
   int32_t* qout = (r & 1) ? a.q0 : a.q1;
   const int newc = r + 2;
   __shared__ int s_idx;
@@ -633,18 +687,24 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_drain(LArgs a, int r) {
     __syncthreads();
     if (idx < 0) break;
     const Chunk ch = a.hq[idx];
-    const int e = min(a.trow[ch.v + 1], ch.first + a.chunk);
-    const uint32_t pu = a.prio ? a.prio[ch.v] : 0u;
-    for (int b = ch.first; b < e; b += SMDP_WG) {
-      const int k = b + (int)threadIdx.x;
-      int v = 0;
-      bool p = false;
-      if (k < e) {
-        v = stream_load(a.tcol + k);
-        p = notify(a, ch.v, pu, v, newc);
-        ++edges;
+    const int e = min(a.trow[ch.v + 1], ch.first + chunk);
+    const uint32_t pu1 = a.prio ? a.prio[ch.v] : 0u;
+    for (int b = ch.first; b < e; b += kLayerIlp * SMDP_WG) {
+      int v[kLayerIlp], uu[kLayerIlp];
+      uint32_t pu[kLayerIlp];
+      bool ok[kLayerIlp], won[kLayerIlp];
+#pragma unroll
+      for (int t = 0; t < kLayerIlp; ++t) {
+        const int k = b + (int)threadIdx.x + t * SMDP_WG;
+        v[t] = k < e ? stream_load(a.tcol + k) : ch.v;
+        uu[t] = ch.v;
+        pu[t] = pu1;
+        ok[t] = k < e && v[t] != ch.v;
+        edges += k < e;
       }
-      pushes += out.push(p, v, qout, &cout->qlen);
+      notify<kLayerIlp>(a, uu, pu, v, ok, newc, won);
+#pragma unroll
+      for (int t = 0; t < kLayerIlp; ++t) pushes += out.push(won[t], v[t], qout, &cout->qlen);
     }
   }
   out.flush(qout, &cout->qlen);
@@ -652,10 +712,10 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_drain(LArgs a, int r) {
   pushes = wave_sum(pushes);
   if (lane_id() == 0) {
     if (edges) {
-      atomicAdd(&a.cnt->edges, edges);
-      atomicAdd(&a.cnt->edges2, edges);
+      atomicAdd(&cnt_of(a.cnt)->edges, edges);
+      atomicAdd(&cnt_of(a.cnt)->edges2, edges);
     }
-    if (pushes) atomicAdd(&a.cnt->push[0], pushes);
+    if (pushes) atomicAdd(&cnt_of(a.cnt)->push[0], pushes);
   }
 }
 
@@ -744,8 +804,10 @@ static int run_color_layers(smdp_graph* g, int use_prio, smdp_stats* st) {
   SMDP_CHECK(alloc_once(&g->q[0], n1));
   SMDP_CHECK(alloc_once(&g->q[1], n1));
   SMDP_CHECK(alloc_once(&b->pend, n1));
-  // chunks of one round (or of the count) never exceed n + m/chunk
-  SMDP_CHECK(ensure(&g->hq, &g->hq_cap, n1 + (int64_t)g->m / std::max(1, p.chunk) + 64));
+  // chunks of one round (or of the count) never exceed n + m/chunk, with the
+  // chunk of short rounds at most kLayerShortChunk arcs
+  SMDP_CHECK(ensure(&g->hq, &g->hq_cap,
+                    n1 + (int64_t)g->m / std::max(1, std::min(p.chunk, kLayerShortChunk)) + 64));
   SMDP_CHECK(ensure_transpose(g, b));
   LArgs a;
   a.row = g->row;

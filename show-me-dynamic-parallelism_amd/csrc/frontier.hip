@@ -121,8 +121,8 @@ __device__ __forceinline__ void flush_counters(Counters* cnt, unsigned long long
   out.flush();
   unsigned long long tries = wave_sum(out.tries), wins = wave_sum(out.wins);
   if (lane_id() == 0) {
-    if (tries) atomicAdd(&cnt->atom_try, tries);
-    if (wins) atomicAdd(&cnt->atom_win, wins);
+    if (tries) atomicAdd(&cnt_of(cnt)->atom_try, tries);
+    if (wins) atomicAdd(&cnt_of(cnt)->atom_win, wins);
   }
   edges = wave_sum(edges);
   verts = wave_sum(verts);
@@ -131,12 +131,12 @@ __device__ __forceinline__ void flush_counters(Counters* cnt, unsigned long long
   p2 = wave_sum(p2);
   p3 = wave_sum(p3);
   if (lane_id() == 0) {
-    if (edges) atomicAdd(&cnt->edges, edges);
-    if (verts) atomicAdd(&cnt->verts, verts);
-    if (p0) atomicAdd(&cnt->push[0], p0);
-    if (p1) atomicAdd(&cnt->push[1], p1);
-    if (p2) atomicAdd(&cnt->push[2], p2);
-    if (p3) atomicAdd(&cnt->push[3], p3);
+    if (edges) atomicAdd(&cnt_of(cnt)->edges, edges);
+    if (verts) atomicAdd(&cnt_of(cnt)->verts, verts);
+    if (p0) atomicAdd(&cnt_of(cnt)->push[0], p0);
+    if (p1) atomicAdd(&cnt_of(cnt)->push[1], p1);
+    if (p2) atomicAdd(&cnt_of(cnt)->push[2], p2);
+    if (p3) atomicAdd(&cnt_of(cnt)->push[3], p3);
   }
 }
 
@@ -193,7 +193,7 @@ __global__ void k_init_frontier(int32_t* dist, int32_t* stamp, int32_t n, int32_
     int32_t* c = reinterpret_cast<int32_t*>(ctl);
     for (int i = threadIdx.x; i < 4 * kCtlInts; i += blockDim.x) c[i] = 0;
     unsigned long long* k = reinterpret_cast<unsigned long long*>(cnt);
-    for (int i = threadIdx.x; i < (int)(sizeof(Counters) / 8); i += blockDim.x) k[i] = 0;
+    for (int i = threadIdx.x; i < (int)(kCntParts * sizeof(Counters) / 8); i += blockDim.x) k[i] = 0;
     __syncthreads();
     if (threadIdx.x == 0) {
       q0[0] = src;
@@ -210,7 +210,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_expand_lane(FArgs a, int r) {
   Ctl* cout = a.ctl + ((r + 1) & 3);
   const int L = cin->qlen;
   if (L == 0) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&a.cnt->rounds, (unsigned long long)(r + 1));
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&cnt_of(a.cnt)->rounds, (unsigned long long)(r + 1));
   const int32_t* qin = (r & 1) ? a.q1 : a.q0;
   int32_t* qout = (r & 1) ? a.q0 : a.q1;
   const int label = r + 1;
@@ -346,7 +346,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_expand_tile(FArgs a, int r) {
   Ctl* cout = a.ctl + ((r + 1) & 3);
   const int L = cin->qlen;
   if (L == 0) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&a.cnt->rounds, (unsigned long long)(r + 1));
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&cnt_of(a.cnt)->rounds, (unsigned long long)(r + 1));
   // workgroups past the tile count would only contend for the counter
   if ((int)blockIdx.x * SMDP_WG * kTileGrab >= L) return;
   const int32_t* qin = (r & 1) ? a.q1 : a.q0;
@@ -477,7 +477,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_drain(FArgs a, int r) {
   log_round(a, r, -1, wave_sum(edges));
   flush_counters(a.cnt, edges, 0, out.pushes, 0, 0, 0, out);
   unsigned long long e2 = wave_sum(edges);
-  if (lane_id() == 0 && e2) atomicAdd(&a.cnt->edges2, e2);
+  if (lane_id() == 0 && e2) atomicAdd(&cnt_of(a.cnt)->edges2, e2);
 }
 
 // Out-arcs of reached vertices (the GTEPS numerator).
@@ -487,7 +487,7 @@ __global__ void k_reached(const int32_t* __restrict__ row, const int32_t* __rest
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     if (dist[i] != inf) acc += (unsigned long long)(row[i + 1] - row[i]);
   acc = wave_sum(acc);
-  if (lane_id() == 0 && acc) atomicAdd(&cnt->reached, acc);
+  if (lane_id() == 0 && acc) atomicAdd(&cnt_of(cnt)->reached, acc);
 }
 
 template <bool BFS>

@@ -254,7 +254,7 @@ __device__ void finish_pass(const RArgs& a, int k, double2 extra, double2* lds2)
     st->hlen = 0;
     st->hnext = 0;
     for (int i = 0; i < 8; ++i) st->bwork[i * 32] = 0;
-    atomicMax(&a.cnt->rounds, (unsigned long long)(k + 1));
+    atomicMax(&cnt_of(a.cnt)->rounds, (unsigned long long)(k + 1));
   }
 }
 
@@ -264,9 +264,9 @@ __device__ __forceinline__ void count_work(Counters* cnt, unsigned long long edg
   rows = wave_sum(rows);
   pushes = wave_sum(pushes);
   if (lane_id() == 0) {
-    if (edges) atomicAdd(&cnt->edges, edges);
-    if (rows) atomicAdd(&cnt->verts, rows);
-    if (pushes) atomicAdd(&cnt->push[1], pushes);
+    if (edges) atomicAdd(&cnt_of(cnt)->edges, edges);
+    if (rows) atomicAdd(&cnt_of(cnt)->verts, rows);
+    if (pushes) atomicAdd(&cnt_of(cnt)->push[1], pushes);
   }
 }
 

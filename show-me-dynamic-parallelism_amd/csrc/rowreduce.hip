@@ -559,7 +559,7 @@ __global__ void k_pr_init(float* rank, float* contrib, const int32_t* __restrict
     int32_t* p = reinterpret_cast<int32_t*>(st);
     for (int i = threadIdx.x; i < (int)(sizeof(RState) / 4); i += blockDim.x) p[i] = 0;
     unsigned long long* c = reinterpret_cast<unsigned long long*>(cnt);
-    for (int i = threadIdx.x; i < (int)(sizeof(Counters) / 8); i += blockDim.x) c[i] = 0;
+    for (int i = threadIdx.x; i < (int)(kCntParts * sizeof(Counters) / 8); i += blockDim.x) c[i] = 0;
     __syncthreads();
     if (threadIdx.x == 0) st->dang[0] = dang0;
   }
