@@ -37,7 +37,7 @@ constexpr int kOutCap = 512;  // per-wave LDS staging of next-queue entries
 constexpr int kIlp = 4;       // arcs per thread per step in the tuned kernels
 constexpr int kTileGrab = 1;  // frontier tiles claimed per atomic
 constexpr int kSmallSplit = 32;  // split degree of the tuned solution when the queue is short
-constexpr int kChunkGrab = 4;    // grid-queue chunks claimed per atomic
+constexpr int kDrainTile = 8;    // grid-queue entries per drain tile (a power of two, at most 64)
 
 // Next-queue output of one wave: LDS staging + the global queue it flushes to.
 struct Sink {
@@ -441,17 +441,28 @@ __global__ __launch_bounds__(SMDP_WG) void k_expand_tile(FArgs a, int r) {
   flush_counters(a.cnt, edges, verts, out.pushes, 0, 0, pg, out);
 }
 
-// Phase 2 of the grid solutions: workgroups take grid-queue entries from a
-// shared counter and expand them with all SMDP_WG threads.
+// Phase 2 of the grid solutions: workgroups claim tiles of kDrainTile
+// grid-queue entries and expand them edge-balanced, like k_expand_tile: wave
+// 0 scans the entries' lengths and every thread takes kIlp arcs per step.  A
+// short round's entries are whole vertices of a few dozen arcs, so one
+// workgroup pass then covers several of them instead of one each.
 template <bool BFS>
 __global__ __launch_bounds__(SMDP_WG) void k_drain(FArgs a, int r) {
   Ctl* cin = a.ctl + (r & 3);
   Ctl* cout = a.ctl + ((r + 1) & 3);
   const int H = cin->hlen;
-  if ((int)blockIdx.x * kChunkGrab >= H) return;
+  // entries per tile: kDrainTile, halved while the tiles still number at most
+  // one per workgroup (a few hub chunks then go to as many workgroups)
+  int ept = kDrainTile;
+  while (ept > 1 && (H + ept / 2 - 1) / (ept / 2) <= (int)gridDim.x) ept >>= 1;
+  const int ntiles = (H + ept - 1) / ept;
+  if ((int)blockIdx.x >= ntiles) return;
   int32_t* qout = (r & 1) ? a.q0 : a.q1;
   const int label = r + 1;
-  __shared__ int s_idx, s_iend;
+  __shared__ int s_tile, s_tot;
+  __shared__ int s_start[kDrainTile];
+  __shared__ int s_du[kDrainTile];
+  __shared__ int s_off[kDrainTile];
   __shared__ int32_t s_out[SMDP_WAVES_PER_WG * kOutCap];
   Sink out;
   out.init(s_out + (threadIdx.x / SMDP_WAVE) * kOutCap, qout, &cout->qlen);
@@ -460,19 +471,55 @@ __global__ __launch_bounds__(SMDP_WG) void k_drain(FArgs a, int r) {
   for (;;) {
     if (threadIdx.x == 0) {
       int f = 0, e = 0;
-      s_idx = claim_items(cin->chunks, H, kChunkGrab, part, &f, &e) ? f : -1;
-      s_iend = e;
+      s_tile = claim_items(cin->chunks, ntiles, 1, part, &f, &e) ? f : -1;
     }
     __syncthreads();
-    const int first = s_idx, iend = s_iend;
-    __syncthreads();
-    if (first < 0) break;
-    for (int idx = first; idx < iend; ++idx) {
-      const Chunk c = a.hq[idx];
-      const int vend = a.row[c.v + 1];
-      const int end = (a.chunk > 0 && vend - c.first > a.chunk) ? c.first + a.chunk : vend;
-      wg_expand<BFS>(a, c.first, end, BFS ? r : a.dist[c.v], label, edges, out);
+    const int tile = s_tile;
+    if (tile < 0) break;
+    if (threadIdx.x < SMDP_WAVE) {
+      const int idx = tile * ept + (int)threadIdx.x;
+      int len = 0, st = 0, du = 0;
+      if ((int)threadIdx.x < ept && idx < H) {
+        const Chunk c = a.hq[idx];
+        const int vend = a.row[c.v + 1];
+        const int end = (a.chunk > 0 && vend - c.first > a.chunk) ? c.first + a.chunk : vend;
+        len = end - c.first;
+        st = c.first;
+        du = BFS ? r : a.dist[c.v];
+      }
+      const int incl = wave_inclusive_scan(len);
+      if ((int)threadIdx.x < kDrainTile) {
+        s_off[threadIdx.x] = incl - len;
+        s_start[threadIdx.x] = st;
+        s_du[threadIdx.x] = du;
+      }
+      if (threadIdx.x == kDrainTile - 1) s_tot = incl;
     }
+    __syncthreads();
+    const int tot = s_tot;
+    for (int jb = 0; jb < tot; jb += kIlp * SMDP_WG) {
+      int j[kIlp], own[kIlp], k[kIlp], v[kIlp], alt[kIlp];
+#pragma unroll
+      for (int t = 0; t < kIlp; ++t) j[t] = min(jb + (int)threadIdx.x + t * SMDP_WG, tot - 1);
+      tile_owners<kIlp, kDrainTile>(s_off, j, own);
+#pragma unroll
+      for (int t = 0; t < kIlp; ++t) k[t] = s_start[own[t]] + (j[t] - s_off[own[t]]);
+#pragma unroll
+      for (int t = 0; t < kIlp; ++t) v[t] = stream_load(a.col + k[t]);
+#pragma unroll
+      for (int t = 0; t < kIlp; ++t) alt[t] = s_du[own[t]] + arc_weight<BFS>(a, k[t]);
+      int seen[kIlp];
+#pragma unroll
+      for (int t = 0; t < kIlp; ++t) seen[t] = a.dist[v[t]];
+#pragma unroll
+      for (int t = 0; t < kIlp; ++t) {
+        const bool ok = jb + (int)threadIdx.x + t * SMDP_WG < tot;
+        const bool p = ok && relax_seen<BFS>(a, v[t], alt[t], seen[t], label, out);
+        edges += ok;
+        out.push(p, v[t]);
+      }
+    }
+    __syncthreads();
   }
   log_round(a, r, -1, wave_sum(edges));
   flush_counters(a.cnt, edges, 0, out.pushes, 0, 0, 0, out);
@@ -576,7 +623,7 @@ int run_frontier(smdp_graph* g, int32_t src, int sol, bool bfs, smdp_stats* st) 
   a.rlog = ctx->profiling ? ctx->rlog : nullptr;
   if (a.rlog) SMDP_HIP(hipMemsetAsync(a.rlog, 0, sizeof(unsigned long long) * 2 * kRoundLogCap, ctx->stream));
   const int grid = grid_size(ctx, p);
-  a.small_q = 2 * grid * SMDP_WG;
+  a.small_q = grid * SMDP_WG;
   const int32_t inf = bfs ? SMDP_BFS_UNDEF : SMDP_SSSP_INF;
   const bool two_phase = sol == SMDP_SOL_GRID || tuned;
   hipStream_t s = ctx->stream;
