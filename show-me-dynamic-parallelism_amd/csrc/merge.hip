@@ -1,0 +1,273 @@
+// Merge-path CSR SpMV, the tuned solution of y = A x.
+//
+// The rows' end offsets and the nonzeros form one merged sequence of n + m
+// items (a row end comes before nonzero j when the row ends at or before j).
+// Every workgroup takes kMpItems consecutive items of it, so the work per
+// workgroup is the same whatever the row lengths: no row is too long for a
+// workgroup and no workgroup waits on one long row.  This is the published
+// merge-based SpMV (Merrill and Garland, "Merge-based parallel sparse matrix-
+// vector multiplication", SC16), built here for 64-lane waves:
+//
+//  * the tile's nonzeros are multiplied with all x gathers of a thread in
+//    flight and the products (with their rounding errors) staged in LDS,
+//  * each thread walks kMpT items of the tile with a compensated sum,
+//  * a row crossing threads is finished by the thread that reaches its end,
+//    which merges the partial sums of the threads before it in thread order,
+//  * a row crossing tiles is finished by k_merge_fixup, which merges the tile
+//    partials in tile order.
+//
+// Every row is summed in an order fixed by the matrix alone, so results are
+// identical from run to run.  Replaces csr_spmv_* (SpMV_kernel.cu:81-403) for
+// the tuned solution.
+
+#include "rowreduce.h"
+
+namespace smdp {
+namespace {
+
+constexpr int kMpNT = 256;                // threads per workgroup
+constexpr int kMpT = 8;                   // merge items per thread
+constexpr int kMpItems = kMpNT * kMpT;    // merge items per tile
+
+struct MergePlan {
+  int32_t* tstart = nullptr;     // ntiles + 1: row ends before each tile's first item
+  float2* carry = nullptr;       // per tile: partial sum of the row open at its end
+  int32_t* carry_row = nullptr;  // per tile: that row
+  float2* head = nullptr;        // per tile: its part of a row begun in an earlier tile and ended in it
+  int32_t* head_row = nullptr;   // per tile: that row, or -1
+  int32_t ntiles = 0;
+  int32_t n = -1;
+  int32_t m = -1;
+};
+
+struct MArgs {
+  const int32_t* __restrict__ row;
+  const int32_t* __restrict__ col;
+  const float* __restrict__ val;
+  const float* __restrict__ x;
+  float* y;
+  const int32_t* __restrict__ tstart;
+  float2* carry;
+  int32_t* carry_row;
+  float2* head;
+  int32_t* head_row;
+  Counters* cnt;
+  int32_t n, m, ntiles;
+};
+
+// Number of row ends among the first d merge items: the smallest i with
+// end(i) > d - i - 1, searched in [max(0, d - nz), min(d, nr)].
+template <typename End>
+__device__ __forceinline__ int64_t merge_search(End end, int64_t d, int64_t nr, int64_t nz) {
+  int64_t lo = d - nz > 0 ? d - nz : 0;
+  int64_t hi = d < nr ? d : nr;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (end(mid) <= d - mid - 1) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void k_merge_starts(const int32_t* __restrict__ row, int32_t n, int32_t m, int32_t ntiles,
+                               int32_t* tstart) {
+  const int64_t total = (int64_t)n + m;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t <= ntiles; t += gridDim.x * blockDim.x) {
+    const int64_t d = (int64_t)t * kMpItems < total ? (int64_t)t * kMpItems : total;
+    tstart[t] = (int32_t)merge_search([&](int64_t i) { return (int64_t)row[i + 1]; }, d, n, m);
+  }
+}
+
+__global__ __launch_bounds__(kMpNT) void k_spmv_merge(MArgs a) {
+  __shared__ int32_t s_end[kMpItems];  // end of each row ending in the tile, in tile nonzero positions
+  __shared__ float s_p[kMpItems];      // products
+  __shared__ float s_e[kMpItems];      // their rounding errors
+  __shared__ int32_t s_crow[kMpNT];    // per thread: the row open at the end of its items (tile index)
+  __shared__ float2 s_cs[kMpNT];       // per thread: its partial sum of that row
+  const int tid = threadIdx.x;
+  const int t = blockIdx.x;
+  const int64_t d0 = (int64_t)t * kMpItems;
+  const int64_t dn = (int64_t)a.n + a.m;
+  const int64_t d1 = d0 + kMpItems < dn ? d0 + kMpItems : dn;
+  const int i0 = a.tstart[t], i1 = a.tstart[t + 1];
+  const int j0 = (int)(d0 - i0), j1 = (int)(d1 - i1);
+  const int nr = i1 - i0, nz = j1 - j0;
+  // the tile's first row began in an earlier tile
+  const bool mid_row = i0 < a.n && a.row[i0] < j0;
+  if (tid == 0) {
+    a.head_row[t] = (mid_row && nr > 0) ? i0 : -1;
+    Counters* c = cnt_of(a.cnt);
+    if (nz) atomicAdd(&c->edges, (unsigned long long)nz);
+    if (nr) atomicAdd(&c->verts, (unsigned long long)nr);
+  }
+  {
+    // every load of the tile is issued before the first one is used
+    int c[kMpT], re[kMpT];
+    float v[kMpT], xv[kMpT];
+#pragma unroll
+    for (int u = 0; u < kMpT; ++u) {
+      const int k = tid + u * kMpNT;
+      c[u] = k < nz ? stream_load(a.col + j0 + k) : 0;
+      v[u] = k < nz ? stream_load(a.val + j0 + k) : 0.f;
+      re[u] = k < nr ? a.row[i0 + k + 1] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kMpT; ++u) xv[u] = tid + u * kMpNT < nz ? a.x[c[u]] : 0.f;
+#pragma unroll
+    for (int u = 0; u < kMpT; ++u) {
+      const int k = tid + u * kMpNT;
+      if (k < nr) s_end[k] = re[u] - j0;
+    }
+#pragma unroll
+    for (int u = 0; u < kMpT; ++u) {
+      const int k = tid + u * kMpNT;
+      if (k < nz) {
+#pragma clang fp contract(off)
+        const float p = v[u] * xv[u];
+        s_p[k] = p;
+        s_e[k] = __builtin_fmaf(v[u], xv[u], -p);
+      }
+    }
+  }
+  __syncthreads();
+
+  const int dl = min(tid * kMpT, nr + nz), dr = min(dl + kMpT, nr + nz);
+  int ri = (int)merge_search([&](int64_t i) { return (int64_t)s_end[i]; }, dl, nr, nz);
+  int zi = dl - ri;
+  const int rfirst = ri;
+  CSum acc, fpart;
+  bool fdone = false;
+  for (int q = dl; q < dr; ++q) {
+    if (ri < nr && s_end[ri] <= zi) {
+      if (ri == rfirst) {
+        fpart = acc;
+        fdone = true;
+      } else {
+        a.y[i0 + ri] = acc.value();
+      }
+      acc = CSum();
+      ++ri;
+    } else {
+      acc.add(s_p[zi]);
+      acc.c += s_e[zi];
+      ++zi;
+    }
+  }
+  s_crow[tid] = ri;
+  s_cs[tid] = make_float2(acc.s, acc.c);
+  __syncthreads();
+
+  if (fdone) {
+    // the threads before this one that ended inside rfirst, merged in order
+    int k = tid;
+    while (k > 0 && s_crow[k - 1] == rfirst) --k;
+    CSum tot;
+    for (int q = k; q < tid; ++q) {
+      CSum o;
+      o.s = s_cs[q].x;
+      o.c = s_cs[q].y;
+      tot.merge(o);
+    }
+    tot.merge(fpart);
+    if (k == 0 && rfirst == 0 && mid_row) a.head[t] = make_float2(tot.s, tot.c);
+    else a.y[i0 + rfirst] = tot.value();
+  }
+  if (tid == kMpNT - 1) {
+    int k = kMpNT - 1;
+    while (k > 0 && s_crow[k - 1] == nr) --k;
+    CSum tot;
+    for (int q = k; q < kMpNT; ++q) {
+      CSum o;
+      o.s = s_cs[q].x;
+      o.c = s_cs[q].y;
+      tot.merge(o);
+    }
+    a.carry[t] = make_float2(tot.s, tot.c);
+    a.carry_row[t] = i1;
+  }
+}
+
+// Rows that began in an earlier tile: the partials of the tiles they cross,
+// merged in tile order, then the part of the tile where they end.
+__global__ void k_merge_fixup(MArgs a) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < a.ntiles; t += gridDim.x * blockDim.x) {
+    const int r = a.head_row[t];
+    if (r < 0) continue;
+    int k = t;
+    while (k > 0 && a.carry_row[k - 1] == r) --k;
+    CSum tot;
+    for (int q = k; q < t; ++q) {
+      CSum o;
+      o.s = a.carry[q].x;
+      o.c = a.carry[q].y;
+      tot.merge(o);
+    }
+    CSum h;
+    h.s = a.head[t].x;
+    h.c = a.head[t].y;
+    tot.merge(h);
+    a.y[r] = tot.value();
+  }
+}
+
+}  // namespace
+
+void merge_free(void* p) {
+  MergePlan* mp = static_cast<MergePlan*>(p);
+  if (!mp) return;
+  void* ptrs[] = {mp->tstart, mp->carry, mp->carry_row, mp->head, mp->head_row};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  delete mp;
+}
+
+int merge_spmv(smdp_graph* g, void** slot, hipStream_t s, int* launches) {
+  const int64_t total = (int64_t)g->n + g->m;
+  if (g->n <= 0 || total > (int64_t)INT32_MAX - kMpItems) return SMDP_ERR_ARG;
+  MergePlan* mp = static_cast<MergePlan*>(*slot);
+  if (!mp || mp->n != g->n || mp->m != g->m) {
+    merge_free(mp);
+    *slot = nullptr;
+    mp = new MergePlan();
+    *slot = mp;
+    mp->ntiles = (int32_t)((total + kMpItems - 1) / kMpItems);
+    const int64_t nt = std::max<int64_t>(1, mp->ntiles);
+    SMDP_CHECK(alloc_once(&mp->tstart, nt + 1));
+    SMDP_CHECK(alloc_once(&mp->carry, nt));
+    SMDP_CHECK(alloc_once(&mp->carry_row, nt));
+    SMDP_CHECK(alloc_once(&mp->head, nt));
+    SMDP_CHECK(alloc_once(&mp->head_row, nt));
+    hipLaunchKernelGGL(k_merge_starts, dim3((mp->ntiles + 256) / 256), dim3(256), 0, s, g->row, g->n, g->m,
+                       mp->ntiles, mp->tstart);
+    SMDP_HIP(hipGetLastError());
+    mp->n = g->n;
+    mp->m = g->m;
+  }
+  MArgs a;
+  a.row = g->row;
+  a.col = g->col;
+  a.val = g->val;
+  a.x = g->x;
+  a.y = g->y;
+  a.tstart = mp->tstart;
+  a.carry = mp->carry;
+  a.carry_row = mp->carry_row;
+  a.head = mp->head;
+  a.head_row = mp->head_row;
+  a.cnt = g->cnt;
+  a.n = g->n;
+  a.m = g->m;
+  a.ntiles = mp->ntiles;
+  if (mp->ntiles == 0) return SMDP_OK;
+  prof_mark(g->ctx, 0);
+  hipLaunchKernelGGL(k_spmv_merge, dim3(mp->ntiles), dim3(kMpNT), 0, s, a);
+  prof_mark(g->ctx, 0);
+  prof_mark(g->ctx, 1);
+  hipLaunchKernelGGL(k_merge_fixup, dim3((mp->ntiles + 255) / 256), dim3(256), 0, s, a);
+  prof_mark(g->ctx, 1);
+  *launches += 2;
+  SMDP_HIP(hipGetLastError());
+  return SMDP_OK;
+}
+
+}  // namespace smdp

@@ -275,4 +275,9 @@ int binned_ensure(smdp_graph* g, void** slot, BArgs* out);
 void binned_free(void* plan);
 void binned_pass(smdp_ctx* ctx, hipStream_t s, const RArgs& a, const BArgs& b, int k, int* launches);
 
+// Merge-path SpMV (merge.hip): y = A x over g's CSR, values and x, with the
+// per-graph plan kept in *slot (built on first use).
+int merge_spmv(smdp_graph* g, void** slot, hipStream_t s, int* launches);
+void merge_free(void* plan);
+
 }  // namespace smdp

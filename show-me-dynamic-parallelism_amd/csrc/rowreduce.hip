@@ -609,6 +609,7 @@ struct RowBufs {
   float* contrib2 = nullptr;
   int64_t ndang = -1;
   void* binned = nullptr;  // binned layout of PageRank (binned.hip)
+  void* merge = nullptr;   // merge-path SpMV plan (merge.hip)
 };
 }  // namespace
 
@@ -631,6 +632,7 @@ void rowbufs_free(smdp_graph* g) {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   binned_free(b->binned);
+  merge_free(b->merge);
   delete b;
   g->rowbufs = nullptr;
 }
@@ -901,7 +903,9 @@ int run_spmv(smdp_graph* g, int sol, smdp_stats* st) {
   hipLaunchKernelGGL(k_pr_init, dim3(1), dim3(256), 0, s, (float*)nullptr, (float*)nullptr, (const int32_t*)nullptr, 0,
                      0.f, b->st, 0.0, g->cnt);
   ++launches;
-  if (g->n > 0) launch_pass<false>(sol, grid, s, a, 0, &launches, ctx);
+  // tuned solution on the plain CSR: merge-path SpMV (merge.hip)
+  if (g->n > 0 && sol == SMDP_SOL_TUNED && !a.browp) SMDP_CHECK(merge_spmv(g, &b->merge, s, &launches));
+  else if (g->n > 0) launch_pass<false>(sol, grid, s, a, 0, &launches, ctx);
   SMDP_HIP(hipGetLastError());
   SMDP_HIP(hipEventRecord(ctx->ev_end, s));
   SMDP_HIP(hipEventSynchronize(ctx->ev_end));

@@ -97,6 +97,51 @@ def test_spmv_edge_cases(ctx):
             assert np.allclose(y, orc.spmv(g.row, g.col, val, x), rtol=1e-6, atol=1e-7)
 
 
+def _merge_mats():
+    """Row layouts around the tuned SpMV's tiles of 2048 merged items (row ends + nonzeros)."""
+    rng = np.random.default_rng(11)
+    out = []
+    deg = np.zeros(20000, np.int64)
+    deg[7] = 60000                       # crosses ~30 tiles
+    deg[5000] = 4095
+    deg[12000:12010] = 1500
+    deg[15000:] = rng.integers(0, 5, 5000)
+    out.append(deg)                      # long empty runs between
+    out.append(np.full(3 * 1024, 1))     # every tile ends on a row end
+    out.append(np.full(6000, 2047))      # rows just under a tile
+    out.append(np.zeros(10000, np.int64))  # no nonzeros at all
+    out.append(rng.geometric(0.02, 4000) - 1)
+    mats = []
+    for i, d in enumerate(out):
+        row = np.concatenate([[0], np.cumsum(d)]).astype(np.int32)
+        n = len(d)
+        col = rng.integers(0, n, int(row[-1])).astype(np.int32)
+        mats.append(smdp.HostGraph(row, col))
+    return mats
+
+
+def test_spmv_tuned_tile_boundaries_exact(ctx):
+    """Small integer values make every sum exact: a partial dropped or added twice changes y."""
+    for i, g in enumerate(_merge_mats()):
+        print(f"merge case {i}: n={g.n} m={g.m}", flush=True)
+        val = ((np.arange(g.m) % 7) - 3).astype(np.float32)
+        x = ((np.arange(g.n) % 5) + 1).astype(np.float32)
+        exact = np.zeros(g.n, np.int64)
+        np.add.at(exact, np.repeat(np.arange(g.n), np.diff(g.row)),
+                  val.astype(np.int64) * x.astype(np.int64)[g.col])
+        assert np.abs(exact).max(initial=0) < 2 ** 24
+        y, st = ctx.spmv(g, val, x, smdp.SOL_TUNED)
+        assert np.array_equal(y.astype(np.int64), exact)
+        assert st.edges_relaxed == g.m and st.vertices_expanded == g.n
+        rv = smdp.gen_uniform_f32(g.m, -1.0, 1.0, 4, 6)
+        rx = smdp.gen_uniform_f32(g.n, 0.0, 1.0, 4, 9)
+        y1, _ = ctx.spmv(g, rv, rx, smdp.SOL_TUNED)
+        y2, _ = ctx.spmv(g, rv, rx, smdp.SOL_TUNED)
+        y64 = orc.spmv(g.row, g.col, rv, rx)
+        assert np.all(np.abs(y1 - y64) <= _spmv_bound(g, rv, rx, y64))
+        assert np.array_equal(y1.view(np.uint32), y2.view(np.uint32))
+
+
 def _pr_graph(name):
     g = MATS[name]
     ig = smdp.invert(g)
