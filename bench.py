@@ -403,6 +403,27 @@ def only_section(args, smdp, orc, np):
                 log(f"pagerank: {st.ms_kernel:.3f} ms, {st.rounds} iterations, kernels {pst.ms_dominant:.3f} ms "
                     f"({pst.dominant_launches} launches) + {pst.ms_phase2:.3f} ms ({pst.phase2_launches}), "
                     f"params {ctx.get_params(smdp.APP_PAGERANK).as_dict()}, {time.time() - t0:.1f} s with set-up")
+    elif args.only == "spmv":
+        g = smdp.gen_rmat(20, 16, seed=2)
+        with smdp.Graph(ctx, g) as dg:
+            dg.set_values(smdp.gen_uniform_f32(g.m, -1.0, 1.0, 2, 6))
+            dg.set_x(smdp.gen_uniform_f32(g.n, 0.0, 1.0, 2, 9))
+            for _ in range(args.warmup + args.steps):
+                st = dg.spmv(smdp.SOL_TUNED)
+            log(f"spmv rmat-20: {st.ms_kernel:.4f} ms, {st.edges_relaxed} nnz")
+    elif args.only == "color":
+        g = smdp.gen_rmat(23, 8, symmetrize=True, seed=5)
+        with smdp.Graph(ctx, g) as dg:
+            dg.set_prio(smdp.gen_prio(g.n, 5))
+            for _ in range(args.warmup + args.steps):
+                st = dg.color(True, smdp.SOL_TUNED)
+            log(f"colour rmat-23: {st.ms_kernel:.3f} ms, {st.rounds} rounds")
+    elif args.only == "bfs":
+        t = smdp.gen_zipf_tree(4194304, 1.5, 4096, 5)
+        with smdp.Graph(ctx, t) as dg:
+            for _ in range(args.warmup + args.steps):
+                st = dg.bfs(0, smdp.SOL_TUNED)
+            log(f"bfs zipf tree 4M: {st.ms_kernel:.3f} ms, {st.rounds} rounds")
     else:
         log(json.dumps(batched_section(ctx, args, smdp, orc, np)))
     ctx.close()
@@ -425,7 +446,7 @@ def main():
                     help="PageRank arc layout: 0 auto, 1 plain CSR, 2 binned, 8 XCD blocks")
     ap.add_argument("--params", default="", help="profiling runs: JSON object of smdp_params fields to set")
     ap.add_argument("--round-log", action="store_true", help="profiling runs: print SSSP per-round log")
-    ap.add_argument("--only", choices=["sssp", "pagerank", "batched", "copy"],
+    ap.add_argument("--only", choices=["sssp", "pagerank", "batched", "copy", "spmv", "color", "bfs"],
                     help="profiling runs: one section only, no JSON contract line")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -505,9 +505,13 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_seed(LArgs a) {
   out.flush(a.q0, &a.ctl[0].qlen);
 }
 
-// Arcs u -> v[t] of the transposed CSR (ok[t]: the arc exists).  All
-// priorities are gathered before the first atomic; the lane that brings
-// pend[v] to zero colours v and queues it (won[t]).
+// Arcs u -> v[t] of the transposed CSR (ok[t]: the arc exists; v[t] is a
+// valid vertex for every t).  The priorities are gathered unconditionally, K
+// at a time (a load under a per-lane condition makes the compiler wait for it
+// before the next one).  The atomics stay conditional: unconditional ones
+// (subtracting 0) were measured 38x slower, since the padded slots of a
+// step's last pass all hit one vertex.  The lane that brings pend[v] to zero
+// colours v and queues it (won[t]).
 template <int K>
 __device__ __forceinline__ void notify(const LArgs& a, const int* u, const uint32_t* pu, const int* v,
                                        const bool* ok, int newc, bool* won) {
@@ -515,7 +519,7 @@ __device__ __forceinline__ void notify(const LArgs& a, const int* u, const uint3
   if (a.prio) {
     uint32_t pv[K];
 #pragma unroll
-    for (int t = 0; t < K; ++t) pv[t] = ok[t] ? a.prio[v[t]] : 0u;
+    for (int t = 0; t < K; ++t) pv[t] = a.prio[v[t]];
 #pragma unroll
     for (int t = 0; t < K; ++t) want[t] = ok[t] && (pu[t] < pv[t] || (pu[t] == pv[t] && u[t] < v[t]));
   } else {
@@ -527,7 +531,7 @@ __device__ __forceinline__ void notify(const LArgs& a, const int* u, const uint3
   for (int t = 0; t < K; ++t) old[t] = want[t] ? atomicSub(&a.pend[v[t]], 1) : 0;
 #pragma unroll
   for (int t = 0; t < K; ++t) {
-    won[t] = old[t] == 1;
+    won[t] = want[t] && old[t] == 1;
     if (won[t]) a.color[v[t]] = newc;
   }
 }
@@ -696,7 +700,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_drain(LArgs a, int r) {
 #pragma unroll
       for (int t = 0; t < kLayerIlp; ++t) {
         const int k = b + (int)threadIdx.x + t * SMDP_WG;
-        v[t] = k < e ? stream_load(a.tcol + k) : ch.v;
+        v[t] = stream_load(a.tcol + min(k, e - 1));
         uu[t] = ch.v;
         pu[t] = pu1;
         ok[t] = k < e && v[t] != ch.v;

@@ -82,8 +82,9 @@ __global__ __launch_bounds__(kMpNT) void k_spmv_merge(MArgs a) {
   __shared__ int32_t s_end[kMpItems];  // end of each row ending in the tile, in tile nonzero positions
   __shared__ float s_p[kMpItems];      // products
   __shared__ float s_e[kMpItems];      // their rounding errors
-  __shared__ int32_t s_crow[kMpNT];    // per thread: the row open at the end of its items (tile index)
-  __shared__ float2 s_cs[kMpNT];       // per thread: its partial sum of that row
+  __shared__ float2 s_cs[kMpNT];       // per thread: partial sum of its open row, scanned over the row's threads
+  __shared__ int32_t s_hk[kMpNT / SMDP_WAVE], s_tk[kMpNT / SMDP_WAVE];  // per wave: open row of lanes 0 and 63
+  __shared__ float2 s_tv[kMpNT / SMDP_WAVE];                           // per wave: scanned sum of lane 63
   const int tid = threadIdx.x;
   const int t = blockIdx.x;
   const int64_t d0 = (int64_t)t * kMpItems;
@@ -100,23 +101,32 @@ __global__ __launch_bounds__(kMpNT) void k_spmv_merge(MArgs a) {
     if (nz) atomicAdd(&c->edges, (unsigned long long)nz);
     if (nr) atomicAdd(&c->verts, (unsigned long long)nr);
   }
-  {
-    // every load of the tile is issued before the first one is used
+  // Every load of the tile is issued before the first one is used.  Loads go
+  // to clamped positions inside the tile (the branches on nz and nr are the
+  // same for the whole workgroup): a load under a per-lane condition would
+  // make the compiler wait for it before the other lanes' default value is
+  // written to the same register.
+  if (nz > 0) {
     int c[kMpT], re[kMpT];
     float v[kMpT], xv[kMpT];
 #pragma unroll
     for (int u = 0; u < kMpT; ++u) {
-      const int k = tid + u * kMpNT;
-      c[u] = k < nz ? stream_load(a.col + j0 + k) : 0;
-      v[u] = k < nz ? stream_load(a.val + j0 + k) : 0.f;
-      re[u] = k < nr ? a.row[i0 + k + 1] : 0;
+      const int k = min(tid + u * kMpNT, nz - 1);
+      c[u] = stream_load(a.col + j0 + k);
+      v[u] = stream_load(a.val + j0 + k);
+    }
+    if (nr > 0) {
+#pragma unroll
+      for (int u = 0; u < kMpT; ++u) re[u] = a.row[i0 + min(tid + u * kMpNT, nr - 1) + 1];
     }
 #pragma unroll
-    for (int u = 0; u < kMpT; ++u) xv[u] = tid + u * kMpNT < nz ? a.x[c[u]] : 0.f;
+    for (int u = 0; u < kMpT; ++u) xv[u] = a.x[c[u]];
+    if (nr > 0) {
 #pragma unroll
-    for (int u = 0; u < kMpT; ++u) {
-      const int k = tid + u * kMpNT;
-      if (k < nr) s_end[k] = re[u] - j0;
+      for (int u = 0; u < kMpT; ++u) {
+        const int k = tid + u * kMpNT;
+        if (k < nr) s_end[k] = re[u] - j0;
+      }
     }
 #pragma unroll
     for (int u = 0; u < kMpT; ++u) {
@@ -128,11 +138,29 @@ __global__ __launch_bounds__(kMpNT) void k_spmv_merge(MArgs a) {
         s_e[k] = __builtin_fmaf(v[u], xv[u], -p);
       }
     }
+  } else if (nr > 0) {
+    int re[kMpT];
+#pragma unroll
+    for (int u = 0; u < kMpT; ++u) re[u] = a.row[i0 + min(tid + u * kMpNT, nr - 1) + 1];
+#pragma unroll
+    for (int u = 0; u < kMpT; ++u) {
+      const int k = tid + u * kMpNT;
+      if (k < nr) s_end[k] = re[u] - j0;
+    }
   }
   __syncthreads();
 
   const int dl = min(tid * kMpT, nr + nz), dr = min(dl + kMpT, nr + nz);
-  int ri = (int)merge_search([&](int64_t i) { return (int64_t)s_end[i]; }, dl, nr, nz);
+  int ri;
+  {
+    int lo = max(0, dl - nz), hi = min(dl, nr);
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (s_end[mid] <= dl - mid - 1) lo = mid + 1;
+      else hi = mid;
+    }
+    ri = lo;
+  }
   int zi = dl - ri;
   const int rfirst = ri;
   CSum acc, fpart;
@@ -153,36 +181,66 @@ __global__ __launch_bounds__(kMpNT) void k_spmv_merge(MArgs a) {
       ++zi;
     }
   }
-  s_crow[tid] = ri;
-  s_cs[tid] = make_float2(acc.s, acc.c);
+  // The open rows (keys) do not decrease from thread to thread, so the
+  // threads ending inside one row are consecutive.  An inclusive segmented
+  // scan gives every thread the sum of its row's partials from the first of
+  // those threads up to itself: six shuffle steps in the wave, then the tails
+  // of at most three earlier waves.  The combination order is fixed.
+  const int key = ri;
+  const int lane = lane_id(), wave = tid / SMDP_WAVE;
+  CSum sc = acc;
+#pragma unroll
+  for (int d = 1; d < SMDP_WAVE; d <<= 1) {
+    const int k2 = __shfl_up(key, d);
+    CSum o;
+    o.s = __shfl_up(sc.s, d);
+    o.c = __shfl_up(sc.c, d);
+    if (lane >= d && k2 == key) {
+      o.merge(sc);
+      sc = o;
+    }
+  }
+  if (lane == 0) s_hk[wave] = key;
+  if (lane == SMDP_WAVE - 1) {
+    s_tk[wave] = key;
+    s_tv[wave] = make_float2(sc.s, sc.c);
+  }
+  __syncthreads();
+  if (key == s_hk[wave]) {
+    int v = wave;
+    while (v > 0 && s_tk[v - 1] == key) {
+      --v;
+      if (s_hk[v] != key) break;
+    }
+    if (v < wave) {
+      CSum pre;
+      for (int u = v; u < wave; ++u) {
+        CSum o;
+        o.s = s_tv[u].x;
+        o.c = s_tv[u].y;
+        pre.merge(o);
+      }
+      pre.merge(sc);
+      sc = pre;
+    }
+  }
+  s_cs[tid] = make_float2(sc.s, sc.c);
   __syncthreads();
 
   if (fdone) {
-    // the threads before this one that ended inside rfirst, merged in order
-    int k = tid;
-    while (k > 0 && s_crow[k - 1] == rfirst) --k;
+    // thread tid - 1 ended inside rfirst: its scanned sum holds every earlier
+    // thread's part of the row
     CSum tot;
-    for (int q = k; q < tid; ++q) {
-      CSum o;
-      o.s = s_cs[q].x;
-      o.c = s_cs[q].y;
-      tot.merge(o);
+    if (tid > 0) {
+      tot.s = s_cs[tid - 1].x;
+      tot.c = s_cs[tid - 1].y;
     }
     tot.merge(fpart);
-    if (k == 0 && rfirst == 0 && mid_row) a.head[t] = make_float2(tot.s, tot.c);
+    if (rfirst == 0 && mid_row) a.head[t] = make_float2(tot.s, tot.c);
     else a.y[i0 + rfirst] = tot.value();
   }
   if (tid == kMpNT - 1) {
-    int k = kMpNT - 1;
-    while (k > 0 && s_crow[k - 1] == nr) --k;
-    CSum tot;
-    for (int q = k; q < kMpNT; ++q) {
-      CSum o;
-      o.s = s_cs[q].x;
-      o.c = s_cs[q].y;
-      tot.merge(o);
-    }
-    a.carry[t] = make_float2(tot.s, tot.c);
+    a.carry[t] = make_float2(sc.s, sc.c);
     a.carry_row[t] = i1;
   }
 }
