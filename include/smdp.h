@@ -99,6 +99,9 @@ typedef struct {
 #define SMDP_LAYOUT_CSR     1  /* gather from the in-CSR as given */
 #define SMDP_LAYOUT_BINNED  2  /* PageRank only: arcs regrouped once per graph by (source chunk, row bin); each
                                   iteration is two streaming passes through LDS instead of random gathers */
+#define SMDP_LAYOUT_HOT     3  /* SSSP/BFS tuned: the solve runs on a copy of the graph with vertices renumbered by
+                                  falling in-degree (built once per resident graph; results are reported in the
+                                  caller's numbering).  Automatic picks it for resident graphs of >= 2^20 arcs. */
 #define SMDP_LAYOUT_XCD8    8  /* sources split into 8 blocks, one per XCD, so each XCD gathers from a slice
                                   that fits its L2 */
 

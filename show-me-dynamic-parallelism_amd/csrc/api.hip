@@ -214,9 +214,10 @@ int smdp_set_params(smdp_ctx* ctx, int app, const smdp_params* p) {
   if (q.max_rounds <= 0) q.max_rounds = d.max_rounds;
   if (q.wg_per_cu > 64 || q.chunk < 64) return SMDP_ERR_ARG;
   if (q.layout != SMDP_LAYOUT_AUTO && q.layout != SMDP_LAYOUT_CSR && q.layout != SMDP_LAYOUT_BINNED &&
-      q.layout != SMDP_LAYOUT_XCD8)
+      q.layout != SMDP_LAYOUT_HOT && q.layout != SMDP_LAYOUT_XCD8)
     return SMDP_ERR_ARG;
   if (q.layout == SMDP_LAYOUT_BINNED && app != SMDP_APP_PAGERANK) return SMDP_ERR_ARG;
+  if (q.layout == SMDP_LAYOUT_HOT && app != SMDP_APP_SSSP && app != SMDP_APP_BFS) return SMDP_ERR_ARG;
   ctx->params[app] = q;
   return SMDP_OK;
 }
@@ -302,6 +303,7 @@ void smdp_graph_destroy(smdp_graph* g) {
     if (p) (void)hipFree(p);
   rowbufs_free(g);
   colorbufs_free(g);
+  relabel_free(g);
   delete g;
 }
 
@@ -369,6 +371,11 @@ int smdp_graph_read(smdp_graph* g, int what, void* out, int64_t count) {
   // Only the result of the last run is defined (dist/level/colour share a buffer).
   if (what != g->last_res) return SMDP_ERR_ARG;
   if (!src || count > avail) return SMDP_ERR_ARG;
+  if ((what == SMDP_RES_DIST || what == SMDP_RES_LEVEL) && g->dist_perm) {
+    // the solve ran on the renumbered copy: back to the caller's numbering
+    SMDP_CHECK(relabel_unpermute(g, g->q[1], count));
+    src = g->q[1];
+  }
   SMDP_HIP(hipMemcpyAsync(out, src, esz * (size_t)count, hipMemcpyDeviceToHost, g->ctx->stream));
   SMDP_HIP(hipStreamSynchronize(g->ctx->stream));
   return SMDP_OK;

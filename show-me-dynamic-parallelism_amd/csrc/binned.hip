@@ -384,16 +384,6 @@ struct BinnedPlan {
   BArgs args{};
 };
 
-// Runs body(lo, hi) over [0, count) split into contiguous ranges, one per thread.
-template <typename F>
-void parallel_ranges(int64_t count, F body) {
-  const int64_t nt = std::max<int64_t>(1, std::min<int64_t>({(int64_t)std::thread::hardware_concurrency(), 16,
-                                                             (count + 255) / 256}));
-  std::vector<std::thread> th;
-  for (int64_t t = 0; t < nt; ++t) th.emplace_back([=] { body(count * t / nt, count * (t + 1) / nt); });
-  for (auto& x : th) x.join();
-}
-
 }  // namespace
 
 void binned_free(void* plan) {

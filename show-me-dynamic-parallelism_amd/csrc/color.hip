@@ -864,6 +864,7 @@ static int run_color_layers(smdp_graph* g, int use_prio, smdp_stats* st) {
   float ms = 0;
   SMDP_HIP(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end));
   g->last_res = SMDP_RES_COLOR;
+  g->dist_perm = false;
   if (st) {
     SMDP_CHECK(fill_stats(g, st, ms, launches));
     st->m_reached = g->m;
@@ -950,6 +951,7 @@ int run_color(smdp_graph* g, int use_prio, int sol, smdp_stats* st) {
   float ms = 0;
   SMDP_HIP(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end));
   g->last_res = SMDP_RES_COLOR;
+  g->dist_perm = false;
   if (st) {
     SMDP_CHECK(fill_stats(g, st, ms, launches));
     st->m_reached = g->m;

@@ -4,6 +4,8 @@
 #include <stdint.h>
 
 #include <string>
+#include <algorithm>
+#include <thread>
 #include <vector>
 
 #include "../../include/smdp.h"
@@ -155,6 +157,9 @@ struct smdp_graph {
   int val_version = 0;      // bumped by smdp_graph_set_values
   void* rowbufs = nullptr;  // row-reduce family buffers (rowreduce.hip)
   void* colorbufs = nullptr; // coloring buffers (color.hip)
+  void* relabel = nullptr;   // renumbered copy for the tuned frontier (frontier.hip)
+  bool dist_perm = false;    // dist holds the renumbered copy's result
+  bool transient = false;    // created by a drop-in call for one solve
 };
 
 namespace smdp {
@@ -210,6 +215,21 @@ void prof_reset(smdp_ctx* ctx);
 // Records one event; call before and after each launch of kernel `kind`
 // (0: phase-1 / dominant kernel, 1: phase-2 kernel).
 int prof_mark(smdp_ctx* ctx, int kind);
+// Runs body(lo, hi) over [0, count) split into contiguous ranges, one per
+// host thread (at most 16).
+template <typename F>
+void parallel_ranges(int64_t count, F body) {
+  const int64_t nt = std::max<int64_t>(1, std::min<int64_t>({(int64_t)std::thread::hardware_concurrency(), 16,
+                                                             (count + 255) / 256}));
+  std::vector<std::thread> th;
+  for (int64_t t = 0; t < nt; ++t) th.emplace_back([=] { body(count * t / nt, count * (t + 1) / nt); });
+  for (auto& x : th) x.join();
+}
+
+// Renumbered frontier copy (frontier.hip): freed with the graph; the
+// un-permuting gather writes the first `count` results in caller numbering.
+void relabel_free(smdp_graph* g);
+int relabel_unpermute(smdp_graph* g, int32_t* out_dev, int64_t count);
 int prof_collect(smdp_ctx* ctx, smdp_stats* st);
 
 void rowbufs_free(smdp_graph* g);

@@ -19,7 +19,7 @@ ERR_ARG, ERR_NOMEM, ERR_DEVICE, ERR_SOLUTION, ERR_OVERFLOW, ERR_IO, ERR_FORMAT, 
 SOL_FLAT, SOL_NAIVE, SOL_WAVE, SOL_BLOCK, SOL_GRID, SOL_HIER, SOL_TUNED = range(7)
 SOLUTIONS = {"flat": 0, "naive": 1, "wave": 2, "block": 3, "grid": 4, "hier": 5, "tuned": 6}
 APP_SSSP, APP_BFS, APP_SPMV, APP_PAGERANK, APP_COLOR, APP_MSSSP = range(6)
-LAYOUT_AUTO, LAYOUT_CSR, LAYOUT_BINNED, LAYOUT_XCD8 = 0, 1, 2, 8
+LAYOUT_AUTO, LAYOUT_CSR, LAYOUT_BINNED, LAYOUT_HOT, LAYOUT_XCD8 = 0, 1, 2, 3, 8
 RES_DIST, RES_LEVEL, RES_Y, RES_RANK, RES_COLOR, RES_MDIST = range(6)
 FMT_DIMACS9, FMT_DIMACS10, FMT_SLNDC = range(3)
 
@@ -444,9 +444,10 @@ class Graph:
                "smdp_sssp_multi_source_run")
         return st
 
-    def read(self, what, dtype=np.int32):
-        out = np.zeros(self.n, dtype=dtype)
-        _check(lib().smdp_graph_read(self.h, what, _ptr(out), self.n), "smdp_graph_read")
+    def read(self, what, dtype=np.int32, count=None):
+        k = self.n if count is None else int(count)
+        out = np.zeros(k, dtype=dtype)
+        _check(lib().smdp_graph_read(self.h, what, _ptr(out), k), "smdp_graph_read")
         return out
 
     def autotune(self, app, src=0, cache_path=None):

@@ -140,3 +140,49 @@ def test_resident_graph_repeat_determinism(ctx):
         assert np.array_equal(dg.read(smdp.RES_DIST), ref)
         with pytest.raises(smdp.SmdpError):
             dg.read(smdp.RES_LEVEL)  # the last run was SSSP
+
+
+@pytest.mark.parametrize("name", ["rmat12", "random_sparse", "uniform16k"])
+def test_hot_layout_renumbered_copy(ctx, name):
+    """SSSP/BFS tuned on the in-degree-renumbered copy report results in the caller's numbering."""
+    g = GRAPHS[name]
+    print(f"hot layout {name}", flush=True)
+    old_s, old_b = ctx.get_params(smdp.APP_SSSP), ctx.get_params(smdp.APP_BFS)
+    try:
+        ctx.set_params(smdp.APP_SSSP, layout=smdp.LAYOUT_HOT)
+        ctx.set_params(smdp.APP_BFS, layout=smdp.LAYOUT_HOT)
+        srcs = [int(s) for s in smdp.gen_sources(g, 3, 4)] + [g.source]
+        with smdp.Graph(ctx, g) as dg:
+            dg.set_prio(smdp.gen_prio(g.n, 3))
+            for s in srcs:
+                ref = orc.dijkstra(g.row, g.col, g.w, s)
+                st = dg.sssp(s, smdp.SOL_TUNED)
+                assert np.array_equal(dg.read(smdp.RES_DIST), ref)
+                assert np.array_equal(dg.read(smdp.RES_DIST, count=g.n // 3), ref[: g.n // 3])
+                assert st.m_reached == int(np.diff(g.row)[ref != orc.INF].sum())
+                lv = orc.bfs(g.row, g.col, s)
+                dg.bfs(s, smdp.SOL_TUNED)
+                assert np.array_equal(dg.read(smdp.RES_LEVEL), lv)
+            # a colouring run shares the result buffer and is not renumbered
+            dg.color(False, smdp.SOL_TUNED)
+            cref, _ = orc.color_closed(g.row, g.col)
+            assert np.array_equal(dg.read(smdp.RES_COLOR), cref)
+            # the plain layout on the same resident graph
+            ctx.set_params(smdp.APP_SSSP, layout=smdp.LAYOUT_CSR)
+            dg.sssp(g.source, smdp.SOL_TUNED)
+            assert np.array_equal(dg.read(smdp.RES_DIST), orc.dijkstra(g.row, g.col, g.w, g.source))
+        # the drop-in call honours an explicit HOT layout too
+        ctx.set_params(smdp.APP_SSSP, layout=smdp.LAYOUT_HOT)
+        d, _ = ctx.sssp(g, g.source, smdp.SOL_TUNED)
+        assert np.array_equal(d, orc.dijkstra(g.row, g.col, g.w, g.source))
+    finally:
+        ctx.set_params(smdp.APP_SSSP, **old_s.as_dict())
+        ctx.set_params(smdp.APP_BFS, **old_b.as_dict())
+
+
+def test_hot_layout_is_frontier_only(ctx):
+    for app in (smdp.APP_SPMV, smdp.APP_PAGERANK, smdp.APP_COLOR):
+        old = ctx.get_params(app)
+        with pytest.raises(smdp.SmdpError):
+            ctx.set_params(app, layout=smdp.LAYOUT_HOT)
+        assert ctx.get_params(app).as_dict() == old.as_dict()
