@@ -252,6 +252,11 @@ def _median_runs(fn, reps=5):
     return ms[len(ms) // 2], sts[-1]
 
 
+def _sum_traffic(*kernels):
+    t = [traffic_for(k) for k in kernels]
+    return None if any(x is None for x in t) else int(sum(t))
+
+
 def _roof(bytes_, ms):
     ach = bytes_ / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     return {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -273,7 +278,8 @@ def spmv_section(ctx, smdp, orc, np):
     t_cpu = time.perf_counter() - t0
     return {"workload": "spmv_rmat20", "rows": g.n, "nnz": g.m, "ms_device_median": round(ms, 4),
             "gflops": round(2 * g.m / (ms * 1e6), 2), "gteps": round(g.m / (ms * 1e6), 3),
-            "roofline": _roof(12 * g.m + 8 * g.n + 4, ms),
+            "roofline": dict(_roof(12 * g.m + 8 * g.n + 4, ms), kernel="k_spmv_merge + k_merge_fixup",
+                             traffic=_sum_traffic("k_spmv_merge", "k_merge_fixup")),
             "linf_vs_fp64": float(np.abs(y.astype(np.float64) - y64).max()),
             "cpu": {"ms": round(t_cpu * 1e3, 1), "cores": 1, "kind": "port",
                     "sample": "one thread-per-row FP64 product (SpMV_kernel.cu:81-96 restated, orc_spmv_f64)"}}
