@@ -79,9 +79,11 @@ __global__ void k_merge_starts(const int32_t* __restrict__ row, int32_t n, int32
 }
 
 __global__ __launch_bounds__(kMpNT) void k_spmv_merge(MArgs a) {
-  __shared__ int32_t s_end[kMpItems];  // end of each row ending in the tile, in tile nonzero positions
-  __shared__ float s_p[kMpItems];      // products
-  __shared__ float s_e[kMpItems];      // their rounding errors
+  // The tile's nr row ends and nz nonzeros fill its kMpItems merge items, so
+  // one array holds both: slot k < nr the end of row i0 + k (in tile nonzero
+  // positions, as int bits in .x), slot nr + j the product of nonzero j and
+  // its rounding error.  19 KB of LDS per workgroup: 8 workgroups per CU.
+  __shared__ float2 s_item[kMpItems];
   __shared__ float2 s_cs[kMpNT];       // per thread: partial sum of its open row, scanned over the row's threads
   __shared__ int32_t s_hk[kMpNT / SMDP_WAVE], s_tk[kMpNT / SMDP_WAVE];  // per wave: open row of lanes 0 and 63
   __shared__ float2 s_tv[kMpNT / SMDP_WAVE];                           // per wave: scanned sum of lane 63
@@ -125,7 +127,7 @@ __global__ __launch_bounds__(kMpNT) void k_spmv_merge(MArgs a) {
 #pragma unroll
       for (int u = 0; u < kMpT; ++u) {
         const int k = tid + u * kMpNT;
-        if (k < nr) s_end[k] = re[u] - j0;
+        if (k < nr) s_item[k].x = __int_as_float(re[u] - j0);
       }
     }
 #pragma unroll
@@ -134,8 +136,7 @@ __global__ __launch_bounds__(kMpNT) void k_spmv_merge(MArgs a) {
       if (k < nz) {
 #pragma clang fp contract(off)
         const float p = v[u] * xv[u];
-        s_p[k] = p;
-        s_e[k] = __builtin_fmaf(v[u], xv[u], -p);
+        s_item[nr + k] = make_float2(p, __builtin_fmaf(v[u], xv[u], -p));
       }
     }
   } else if (nr > 0) {
@@ -145,7 +146,7 @@ __global__ __launch_bounds__(kMpNT) void k_spmv_merge(MArgs a) {
 #pragma unroll
     for (int u = 0; u < kMpT; ++u) {
       const int k = tid + u * kMpNT;
-      if (k < nr) s_end[k] = re[u] - j0;
+      if (k < nr) s_item[k].x = __int_as_float(re[u] - j0);
     }
   }
   __syncthreads();
@@ -156,7 +157,7 @@ __global__ __launch_bounds__(kMpNT) void k_spmv_merge(MArgs a) {
     int lo = max(0, dl - nz), hi = min(dl, nr);
     while (lo < hi) {
       const int mid = (lo + hi) >> 1;
-      if (s_end[mid] <= dl - mid - 1) lo = mid + 1;
+      if (__float_as_int(s_item[mid].x) <= dl - mid - 1) lo = mid + 1;
       else hi = mid;
     }
     ri = lo;
@@ -166,7 +167,7 @@ __global__ __launch_bounds__(kMpNT) void k_spmv_merge(MArgs a) {
   CSum acc, fpart;
   bool fdone = false;
   for (int q = dl; q < dr; ++q) {
-    if (ri < nr && s_end[ri] <= zi) {
+    if (ri < nr && __float_as_int(s_item[ri].x) <= zi) {
       if (ri == rfirst) {
         fpart = acc;
         fdone = true;
@@ -176,8 +177,9 @@ __global__ __launch_bounds__(kMpNT) void k_spmv_merge(MArgs a) {
       acc = CSum();
       ++ri;
     } else {
-      acc.add(s_p[zi]);
-      acc.c += s_e[zi];
+      const float2 pe = s_item[nr + zi];
+      acc.add(pe.x);
+      acc.c += pe.y;
       ++zi;
     }
   }
