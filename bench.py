@@ -187,7 +187,15 @@ def pagerank_section(ctx, g, args, smdp, orc, np):
                        {"kernel": "k_bin_scatter" if binned else "phase 2", "ms_total": round(pst.ms_phase2, 4),
                         "launches": pst.phase2_launches}]}
     if not args.skip_check:
+        t0 = time.perf_counter()
         ref, _, _ = orc.pagerank(ig.row, ig.col, od, d=0.85, T=1.0, fixed_iters=iters)
+        t_seq = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        _, _, nt = orc.pagerank_par(ig.row, ig.col, od, iters, d=0.85, T=1.0)
+        t_par = time.perf_counter() - t0
+        out["cpu"] = _cpu_entry(t_seq, t_par, nt, g.m * iters,
+                                f"{iters} FP64 pull iterations (pagerank_kernel.cu:95-120 restated, "
+                                "orc_pagerank_f64 / orc_pagerank_f64_par) on the same in-CSR")
         out["l1_vs_fp64_iteration_matched"] = float(np.abs(r.astype(np.float64) - ref).sum())
         out["linf_vs_fp64"] = float(np.abs(r.astype(np.float64) - ref).max())
         conv, it_conv, _ = orc.pagerank(ig.row, ig.col, od, d=0.85, T=1.0, tol=1e-9, max_iter=1000)
@@ -252,6 +260,14 @@ def _median_runs(fn, reps=5):
     return ms[len(ms) // 2], sts[-1]
 
 
+def _cpu_entry(t_seq, t_par, threads, arcs, sample):
+    """CPU baseline of one section: the oracle on one thread and on all host threads."""
+    return {"ms": round(t_seq * 1e3, 1), "cores": 1, "kind": "port",
+            "ms_parallel": round(t_par * 1e3, 1), "threads": threads,
+            "gteps": round(arcs / t_seq / 1e9, 4), "gteps_parallel": round(arcs / t_par / 1e9, 4),
+            "nproc": os.cpu_count(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "sample": sample}
+
+
 def _sum_traffic(*kernels):
     t = [traffic_for(k) for k in kernels]
     return None if any(x is None for x in t) else int(sum(t))
@@ -276,13 +292,16 @@ def spmv_section(ctx, smdp, orc, np):
     t0 = time.perf_counter()
     y64 = orc.spmv(g.row, g.col, val, x)
     t_cpu = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, nt = orc.spmv_par(g.row, g.col, val, x)
+    t_par = time.perf_counter() - t0
     return {"workload": "spmv_rmat20", "rows": g.n, "nnz": g.m, "ms_device_median": round(ms, 4),
             "gflops": round(2 * g.m / (ms * 1e6), 2), "gteps": round(g.m / (ms * 1e6), 3),
             "roofline": dict(_roof(12 * g.m + 8 * g.n + 4, ms), kernel="k_spmv_merge + k_merge_fixup",
                              traffic=_sum_traffic("k_spmv_merge", "k_merge_fixup")),
             "linf_vs_fp64": float(np.abs(y.astype(np.float64) - y64).max()),
-            "cpu": {"ms": round(t_cpu * 1e3, 1), "cores": 1, "kind": "port",
-                    "sample": "one thread-per-row FP64 product (SpMV_kernel.cu:81-96 restated, orc_spmv_f64)"}}
+            "cpu": _cpu_entry(t_cpu, t_par, nt, g.m, "one thread-per-row FP64 product (SpMV_kernel.cu:81-96 "
+                              "restated, orc_spmv_f64 / orc_spmv_f64_par)")}
 
 
 def color_section(ctx, smdp, orc, np):
@@ -296,13 +315,17 @@ def color_section(ctx, smdp, orc, np):
     t0 = time.perf_counter()
     ref, k = orc.color_closed(g.row, g.col, prio)
     t_cpu = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, _, nt = orc.color_layers_par(g.row, g.col, prio)
+    t_par = time.perf_counter() - t0
     return {"workload": "color_rmat23_sym", "vertices": g.n, "arcs": g.m, "ms_device_median": round(ms, 4),
             "rounds": int(st.rounds), "colors": int(c.max()), "exact_vs_oracle": bool(np.array_equal(c, ref)),
             "scanned_arcs": int(st.edges_relaxed), "gteps": round(g.m / (ms * 1e6), 3),
             "roofline": _roof(12 * st.edges_relaxed + 16 * st.vertices_expanded, ms),
-            "cpu": {"ms": round(t_cpu * 1e3, 1), "cores": 1, "kind": "port",
-                    "sample": "one pass of the closed form colour(v) = 1 + max over preceding neighbours "
-                              "(orc_color_closed; the round rule of graph_color_kernel.cu:28-49 gives the same colours)"}}
+            "cpu": _cpu_entry(t_cpu, t_par, nt, g.m,
+                              "the closed form colour(v) = 1 + max over preceding neighbours: one pass in "
+                              "precedence order (orc_color_closed) / by layers (orc_color_layers_par); the round "
+                              "rule of graph_color_kernel.cu:28-49 gives the same colours")}
 
 
 def bfs_section(ctx, smdp, orc, np):
@@ -314,11 +337,14 @@ def bfs_section(ctx, smdp, orc, np):
     t0 = time.perf_counter()
     ref = orc.bfs(t.row, t.col, 0)
     t_cpu = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, nt = orc.bfs_par(t.row, t.col, 0)
+    t_par = time.perf_counter() - t0
     return {"workload": "bfs_zipf_tree_4m", "vertices": t.n, "arcs": t.m, "ms_device_median": round(ms, 4),
             "rounds": int(st.rounds), "exact_vs_oracle": bool(np.array_equal(lv, ref)),
             "gteps": round(t.m / (ms * 1e6), 3), "roofline": _roof(8 * st.edges_relaxed + 12 * t.n, ms),
-            "cpu": {"ms": round(t_cpu * 1e3, 1), "cores": 1, "kind": "port",
-                    "sample": "one FIFO BFS (bfs_rec.cpp:102-121 restated, orc_bfs_fifo)"}}
+            "cpu": _cpu_entry(t_cpu, t_par, nt, t.m, "one BFS from the root: FIFO (bfs_rec.cpp:102-121 restated, "
+                              "orc_bfs_fifo) / level-synchronous (orc_bfs_par)")}
 
 
 def stream_copy_bandwidth(nbytes=1 << 29, reps=5):

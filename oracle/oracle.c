@@ -371,3 +371,210 @@ int orc_pagerank_f64(int n, const int* in_row, const int* in_col, const int* out
   if (residual) *residual = res;
   return 0;
 }
+
+/* ------------------------------------------- OpenMP CPU baselines ---------*/
+/* Parallel twins of the functions above, used only to time the CPU baseline
+ * on all host cores.  Each returns the number of threads it used (< 0 on
+ * error) and is checked against its sequential twin in tests/test_oracle.py. */
+
+/* orc_spmv_f64 with rows split over threads: every row is summed exactly as
+ * in the sequential version, so y is bit-identical. */
+int orc_spmv_f64_par(int n, const int* row, const int* col, const float* val, const float* x, double* y) {
+  int nt = 1;
+#pragma omp parallel
+  {
+#pragma omp single
+    nt = omp_get_num_threads();
+#pragma omp for schedule(dynamic, 4096)
+    for (int r = 0; r < n; ++r) {
+      long double acc = 0.0L;
+      for (int e = row[r]; e < row[r + 1]; ++e) acc += (long double)((double)val[e] * (double)x[col[e]]);
+      y[r] = (double)acc;
+    }
+  }
+  return nt;
+}
+
+/* orc_pagerank_f64 with fixed_iters iterations, rows split over threads.  The
+ * dangling mass and the residual are OpenMP reductions, so they may differ
+ * from the sequential sums in the last bits. */
+int orc_pagerank_f64_par(int n, const int* in_row, const int* in_col, const int* outdeg, double d, double T,
+                         int init_zero, int fixed_iters, double* rank, double* residual) {
+  if (n <= 0 || fixed_iters <= 0) return -1;
+  double* r = rank;
+  double* nr = (double*)malloc(sizeof(double) * (size_t)n);
+  double* contrib = (double*)malloc(sizeof(double) * (size_t)n);
+  if (!nr || !contrib) {
+    free(nr);
+    free(contrib);
+    return -2;
+  }
+  int nt = 1;
+  double res = 0.0;
+#pragma omp parallel
+  {
+#pragma omp single
+    nt = omp_get_num_threads();
+  }
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) r[i] = init_zero ? 0.0 : T / n;
+  for (int it = 0; it < fixed_iters; ++it) {
+    double dang = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : dang)
+    for (int u = 0; u < n; ++u) {
+      if (outdeg[u] == 0) dang += r[u];
+      contrib[u] = outdeg[u] ? r[u] / outdeg[u] : 0.0;
+    }
+    double base = (1.0 - d) * T / n + d * dang / n;
+    res = 0.0;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : res)
+    for (int v = 0; v < n; ++v) {
+      double s = 0.0;
+      for (int e = in_row[v]; e < in_row[v + 1]; ++e) s += contrib[in_col[e]];
+      nr[v] = d * s + base;
+      res += fabs(nr[v] - r[v]);
+    }
+    double* t = r;
+    r = nr;
+    nr = t;
+  }
+  if (r != rank) {
+    memcpy(rank, r, sizeof(double) * (size_t)n);
+    nr = r;
+  }
+  free(nr);
+  free(contrib);
+  if (residual) *residual = res;
+  return nt;
+}
+
+/* Level-synchronous BFS: the frontier of level k is expanded by all threads,
+ * a vertex joins level k+1 when a compare-and-swap claims it.  Levels equal
+ * orc_bfs_fifo's (shortest hop counts). */
+int orc_bfs_par(int n, const int* row, const int* col, int src, int* level) {
+  if (n <= 0 || src < 0 || src >= n) return -1;
+  int* cur = (int*)malloc(sizeof(int) * (size_t)n);
+  int* nxt = (int*)malloc(sizeof(int) * (size_t)n);
+  if (!cur || !nxt) {
+    free(cur);
+    free(nxt);
+    return -2;
+  }
+  int nt = 1;
+#pragma omp parallel
+  {
+#pragma omp single
+    nt = omp_get_num_threads();
+#pragma omp for schedule(static)
+    for (int i = 0; i < n; ++i) level[i] = ORC_UNDEF;
+  }
+  level[src] = 0;
+  cur[0] = src;
+  int64_t nc = 1;
+  for (int lv = 0; nc > 0; ++lv) {
+    int64_t nn = 0;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < nc; ++i) {
+      const int u = cur[i];
+      for (int e = row[u]; e < row[u + 1]; ++e) {
+        const int v = col[e];
+        int exp = ORC_UNDEF;
+        if (__atomic_load_n(&level[v], __ATOMIC_RELAXED) == ORC_UNDEF &&
+            __atomic_compare_exchange_n(&level[v], &exp, lv + 1, 0, __ATOMIC_RELAXED, __ATOMIC_RELAXED))
+          nxt[__atomic_fetch_add(&nn, 1, __ATOMIC_RELAXED)] = v;
+      }
+    }
+    int* t = cur;
+    cur = nxt;
+    nxt = t;
+    nc = nn;
+  }
+  free(cur);
+  free(nxt);
+  return nt;
+}
+
+/* orc_color_closed by layers (Kahn's algorithm): pend[v] counts the
+ * preceding row neighbours of v; the vertices with none form layer 1, and
+ * the vertices whose count reaches zero while layer c is expanded form layer
+ * c+1.  Expanding v walks the transposed CSR (built here, in parallel where
+ * it can be).  The layer index is 1 + the longest chain of preceding
+ * neighbours, which is the closed form's colour.  Returns the thread count;
+ * *ncolors receives the number of colours. */
+int orc_color_layers_par(int n, const int* row, const int* col, const unsigned* prio, int* color, int* ncolors) {
+  if (n <= 0) return -1;
+  const int64_t m = row[n];
+  int* tdeg = (int*)calloc((size_t)n + 1, sizeof(int));
+  int* tcol = (int*)malloc(sizeof(int) * (size_t)(m > 0 ? m : 1));
+  int* pend = (int*)malloc(sizeof(int) * (size_t)n);
+  int* cur = (int*)malloc(sizeof(int) * (size_t)n);
+  int* nxt = (int*)malloc(sizeof(int) * (size_t)n);
+  if (!tdeg || !tcol || !pend || !cur || !nxt) {
+    free(tdeg);
+    free(tcol);
+    free(pend);
+    free(cur);
+    free(nxt);
+    return -2;
+  }
+  int nt = 1;
+#pragma omp parallel
+  {
+#pragma omp single
+    nt = omp_get_num_threads();
+  }
+  /* transposed CSR: tcol lists, for each u, the rows v that contain u */
+#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < m; ++e) __atomic_fetch_add(&tdeg[col[e] + 1], 1, __ATOMIC_RELAXED);
+  for (int v = 0; v < n; ++v) tdeg[v + 1] += tdeg[v];
+  int* fill = (int*)malloc(sizeof(int) * (size_t)n);
+  if (!fill) {
+    free(tdeg);
+    free(tcol);
+    free(pend);
+    free(cur);
+    free(nxt);
+    return -2;
+  }
+  memcpy(fill, tdeg, sizeof(int) * (size_t)n);
+  /* order inside a transposed row does not matter to the layers */
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int v = 0; v < n; ++v)
+    for (int e = row[v]; e < row[v + 1]; ++e) tcol[__atomic_fetch_add(&fill[col[e]], 1, __ATOMIC_RELAXED)] = v;
+  free(fill);
+  int64_t nc = 0;
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int v = 0; v < n; ++v) {
+    int c = 0;
+    for (int e = row[v]; e < row[v + 1]; ++e) c += col[e] != v && precedes(prio, col[e], v);
+    pend[v] = c;
+    color[v] = 0;
+    if (c == 0) cur[__atomic_fetch_add(&nc, 1, __ATOMIC_RELAXED)] = v;
+  }
+  int c = 0;
+  while (nc > 0) {
+    ++c;
+    int64_t nn = 0;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < nc; ++i) {
+      const int u = cur[i];
+      color[u] = c;
+      for (int e = tdeg[u]; e < tdeg[u + 1]; ++e) {
+        const int v = tcol[e];
+        if (v != u && precedes(prio, u, v) && __atomic_sub_fetch(&pend[v], 1, __ATOMIC_RELAXED) == 0)
+          nxt[__atomic_fetch_add(&nn, 1, __ATOMIC_RELAXED)] = v;
+      }
+    }
+    int* t = cur;
+    cur = nxt;
+    nxt = t;
+    nc = nn;
+  }
+  free(tdeg);
+  free(tcol);
+  free(pend);
+  free(cur);
+  free(nxt);
+  if (ncolors) *ncolors = c;
+  return nt;
+}

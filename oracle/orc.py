@@ -40,6 +40,10 @@ def lib():
         L.orc_color_rounds.argtypes = [i, vp, vp, vp, vp]
         L.orc_color_closed.argtypes = [i, vp, vp, vp, vp]
         L.orc_spmv_f64.argtypes = [i, vp, vp, vp, vp, vp]
+        L.orc_spmv_f64_par.argtypes = [i, vp, vp, vp, vp, vp]
+        L.orc_pagerank_f64_par.argtypes = [i, vp, vp, vp, C.c_double, C.c_double, i, i, vp, C.POINTER(C.c_double)]
+        L.orc_bfs_par.argtypes = [i, vp, vp, i, vp]
+        L.orc_color_layers_par.argtypes = [i, vp, vp, vp, vp, C.POINTER(C.c_int)]
         L.orc_pagerank_f64.argtypes = [i, vp, vp, vp, C.c_double, C.c_double, i, C.c_double, i, i, vp,
                                        C.POINTER(C.c_int), C.POINTER(C.c_double)]
         _lib = L
@@ -134,6 +138,51 @@ def pagerank(in_row, in_col, outdeg, d=0.85, T=1.0, init_zero=False, tol=1e-6, m
     assert lib().orc_pagerank_f64(n, _p(in_row), _p(in_col), _p(outdeg), d, T, int(init_zero), tol, max_iter,
                                   fixed_iters, _p(r), C.byref(it), C.byref(res)) == 0
     return r, it.value, res.value
+
+
+# ---- OpenMP twins, for timing the CPU baseline on all host cores -----------
+
+def spmv_par(row, col, val, x):
+    """orc_spmv_f64 on all host threads; returns (y, threads)."""
+    row, col = _i32(row), _i32(col)
+    val = np.ascontiguousarray(val, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.zeros(len(row) - 1, dtype=np.float64)
+    nt = lib().orc_spmv_f64_par(len(row) - 1, _p(row), _p(col), _p(val), _p(x), _p(y))
+    assert nt > 0
+    return y, nt
+
+
+def pagerank_par(in_row, in_col, outdeg, iters, d=0.85, T=1.0, init_zero=False):
+    """`iters` FP64 pull iterations on all host threads; returns (rank, last residual, threads)."""
+    in_row, in_col, outdeg = _i32(in_row), _i32(in_col), _i32(outdeg)
+    n = len(in_row) - 1
+    r = np.zeros(n, dtype=np.float64)
+    res = C.c_double()
+    nt = lib().orc_pagerank_f64_par(n, _p(in_row), _p(in_col), _p(outdeg), d, T, int(init_zero), iters, _p(r),
+                                    C.byref(res))
+    assert nt > 0
+    return r, res.value, nt
+
+
+def bfs_par(row, col, src):
+    """Level-synchronous BFS on all host threads; returns (level, threads)."""
+    row, col = _i32(row), _i32(col)
+    lv = np.zeros(len(row) - 1, dtype=np.int32)
+    nt = lib().orc_bfs_par(len(row) - 1, _p(row), _p(col), src, _p(lv))
+    assert nt > 0
+    return lv, nt
+
+
+def color_layers_par(row, col, prio=None):
+    """The closed-form colouring by layers on all host threads; returns (color, ncolors, threads)."""
+    row, col = _i32(row), _i32(col)
+    prio = None if prio is None else np.ascontiguousarray(prio, dtype=np.uint32)
+    c = np.zeros(len(row) - 1, dtype=np.int32)
+    k = C.c_int()
+    nt = lib().orc_color_layers_par(len(row) - 1, _p(row), _p(col), _p(prio), _p(c), C.byref(k))
+    assert nt > 0
+    return c, k.value, nt
 
 
 # ---- the reference's own host graph library (built from its sources) --------

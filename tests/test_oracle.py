@@ -223,3 +223,50 @@ def test_pagerank_iteration_matched_and_init():
     assert np.allclose(r1, 0.5 * 1000.0 / 50, rtol=0, atol=1e-12) and it1 == 1
     r2, it2, res2 = orc.pagerank(ig.row, ig.col, od, tol=1e-6, max_iter=100)
     assert it2 < 100 and res2 < 1e-6
+
+
+# ---- OpenMP twins (CPU baseline timing) -----------------------------------
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_parallel_twins_match_sequential(seed):
+    rng = np.random.default_rng(40 + seed)
+    row, col, w = random_graph(5000, 40000, 50 + seed, 1, 9)
+    n = len(row) - 1
+    val = rng.uniform(-1, 1, len(col)).astype(np.float32)
+    x = rng.uniform(0, 1, n).astype(np.float32)
+    y, nt = orc.spmv_par(row, col, val, x)
+    assert nt >= 1 and np.array_equal(y, orc.spmv(row, col, val, x))  # same per-row sums
+    lv, _ = orc.bfs_par(row, col, 3)
+    assert np.array_equal(lv, orc.bfs(row, col, 3))
+    # colouring: symmetric and directed inputs, id and random priorities (with ties)
+    sym_r, sym_c = _symmetrize(row, col)
+    for r_, c_ in ((row, col), (sym_r, sym_c)):
+        for prio in (None, rng.integers(0, 50, n).astype(np.uint32)):
+            c, k, _ = orc.color_layers_par(r_, c_, prio)
+            ref, kref = orc.color_closed(r_, c_, prio)
+            assert np.array_equal(c, ref) and k == kref
+    # PageRank: same iterations, reductions in another order
+    od = np.diff(row).astype(np.int32)
+    inv_r, inv_c = _invert(row, col)
+    ref, it, res = orc.pagerank(inv_r, inv_c, od, fixed_iters=12)
+    r, res_p, _ = orc.pagerank_par(inv_r, inv_c, od, 12)
+    assert np.abs(r - ref).max() <= 1e-14 and abs(res_p - res) <= 1e-12
+
+
+def _symmetrize(row, col):
+    n = len(row) - 1
+    src = np.repeat(np.arange(n), np.diff(row))
+    s = np.concatenate([src, col]); d = np.concatenate([col, src])
+    o = np.lexsort((d, s))
+    r = np.zeros(n + 1, np.int64)
+    np.add.at(r, s[o] + 1, 1)
+    return np.cumsum(r).astype(np.int32), d[o].astype(np.int32)
+
+
+def _invert(row, col):
+    n = len(row) - 1
+    src = np.repeat(np.arange(n), np.diff(row))
+    o = np.lexsort((src, col))
+    r = np.zeros(n + 1, np.int64)
+    np.add.at(r, col[o] + 1, 1)
+    return np.cumsum(r).astype(np.int32), src[o].astype(np.int32)
