@@ -557,9 +557,18 @@ __device__ __forceinline__ LayerShape layer_shape(const LArgs& a, int L, int gri
   return sh;
 }
 
-// Round r: the queue holds the vertices of colour r+1.  Edge-balanced tiles
-// of queue entries (layer_shape); transposed rows of >= split arcs go to grid
-// chunks.
+// Layers of at most kShortL vertices (most of RMAT-23's 2373) are expanded
+// arc-balanced: G = min(grid, max(L / kShortPerWG, kShortMinWG)) workgroups
+// each stage the whole layer (vertex, first arc, arc offset) in LDS and take
+// an equal share of its arcs, so a layer's hubs are split over the group
+// without grid chunks and the round needs no drain.
+constexpr int kShortL = 2048;
+constexpr int kShortPerWG = 8;
+constexpr int kShortMinWG = 128;
+
+// Round r: the queue holds the vertices of colour r+1.  Short layers as
+// above; longer ones in edge-balanced tiles of queue entries (layer_shape),
+// with transposed rows of >= split arcs cut into grid chunks.
 __global__ __launch_bounds__(SMDP_WG) void k_layer_expand(LArgs a, int r) {
   ctl_clear(a.ctl, (r + 2) & 3);
   Ctl* cin = a.ctl + (r & 3);
@@ -567,26 +576,89 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_expand(LArgs a, int r) {
   const int L = cin->qlen;
   if (L == 0) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&cnt_of(a.cnt)->rounds, (unsigned long long)(r + 1));
+  const int32_t* qin = (r & 1) ? a.q1 : a.q0;
+  int32_t* qout = (r & 1) ? a.q0 : a.q1;
+  const int newc = r + 2;
+  const int lane = lane_id();
+  __shared__ int32_t s_raw[3 * kShortL];  // short layers: vertex, first arc, arc offset; tiles: see below
+  __shared__ int s_tile;
+  __shared__ int s_scan[SMDP_WAVES_PER_WG];
+  __shared__ int32_t s_out[SMDP_WAVES_PER_WG * kCOut];
+  WaveOut<kCOut> out;
+  out.init(s_out + (threadIdx.x / SMDP_WAVE) * kCOut);
+  unsigned long long edges = 0, verts = 0, pushes = 0, pg = 0;
+
+  if (L <= kShortL) {
+    const int G = min((int)gridDim.x, max((L + kShortPerWG - 1) / kShortPerWG, kShortMinWG));
+    if ((int)blockIdx.x >= G) return;
+    int* s_v = s_raw;
+    int* s_st = s_raw + kShortL;
+    int* s_of = s_raw + 2 * kShortL;
+    constexpr int E = kShortL / SMDP_WG;  // consecutive entries staged per thread
+    int vv[E], st[E], en[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) vv[k] = qin[min((int)threadIdx.x * E + k, L - 1)];
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      st[k] = a.trow[vv[k]];
+      en[k] = a.trow[vv[k] + 1];
+    }
+    int sum = 0;
+#pragma unroll
+    for (int k = 0; k < E; ++k) sum += (int)threadIdx.x * E + k < L ? en[k] - st[k] : 0;
+    int A;
+    int run = wg_exclusive_scan(sum, s_scan, &A);
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int i = (int)threadIdx.x * E + k;
+      if (i < L) {
+        s_v[i] = vv[k];
+        s_st[i] = st[k];
+        s_of[i] = run;
+        run += en[k] - st[k];
+      } else {
+        s_of[i] = A;  // never the owner of an arc j < A
+      }
+    }
+    __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x == 0) verts = L;
+    const int a0 = (int)((int64_t)A * blockIdx.x / G), a1 = (int)((int64_t)A * (blockIdx.x + 1) / G);
+    for (int jb = a0; jb < a1; jb += kLayerIlp * SMDP_WG) {
+      int j[kLayerIlp], own[kLayerIlp], v[kLayerIlp], uu[kLayerIlp];
+      uint32_t pu[kLayerIlp];
+      bool ok[kLayerIlp], won[kLayerIlp];
+#pragma unroll
+      for (int t = 0; t < kLayerIlp; ++t) j[t] = min(jb + (int)threadIdx.x + t * SMDP_WG, a1 - 1);
+      tile_owners<kLayerIlp, kShortL>(s_of, j, own);
+#pragma unroll
+      for (int t = 0; t < kLayerIlp; ++t) {
+        uu[t] = s_v[own[t]];
+        v[t] = stream_load(a.tcol + s_st[own[t]] + (j[t] - s_of[own[t]]));
+        ok[t] = jb + (int)threadIdx.x + t * SMDP_WG < a1 && v[t] != uu[t];
+        edges += jb + (int)threadIdx.x + t * SMDP_WG < a1;
+      }
+      if (a.prio) {
+#pragma unroll
+        for (int t = 0; t < kLayerIlp; ++t) pu[t] = a.prio[uu[t]];
+      } else {
+#pragma unroll
+        for (int t = 0; t < kLayerIlp; ++t) pu[t] = 0u;
+      }
+      notify<kLayerIlp>(a, uu, pu, v, ok, newc, won);
+#pragma unroll
+      for (int t = 0; t < kLayerIlp; ++t) pushes += out.push(won[t], v[t], qout, &cout->qlen);
+    }
+  } else {
   const LayerShape sh = layer_shape(a, L, (int)gridDim.x);
   const int ntiles = (L + sh.tile - 1) / sh.tile;
   // workgroups past the tile count would only contend for the counter
   if ((int)blockIdx.x >= ntiles) return;
   // at most one tile per workgroup: tile = workgroup, no claims
   const bool one_each = ntiles <= (int)gridDim.x;
-  const int32_t* qin = (r & 1) ? a.q1 : a.q0;
-  int32_t* qout = (r & 1) ? a.q0 : a.q1;
-  const int newc = r + 2;
-  const int lane = lane_id();
-  __shared__ int s_tile;
-  __shared__ int s_start[SMDP_WG];
-  __shared__ int s_u[SMDP_WG];
-  __shared__ uint32_t s_pu[SMDP_WG];
-  __shared__ int s_off[SMDP_WG];
-  __shared__ int s_scan[SMDP_WAVES_PER_WG];
-  __shared__ int32_t s_out[SMDP_WAVES_PER_WG * kCOut];
-  WaveOut<kCOut> out;
-  out.init(s_out + (threadIdx.x / SMDP_WAVE) * kCOut);
-  unsigned long long edges = 0, verts = 0, pushes = 0, pg = 0;
+  int* s_start = s_raw;
+  int* s_u = s_raw + SMDP_WG;
+  uint32_t* s_pu = reinterpret_cast<uint32_t*>(s_raw + 2 * SMDP_WG);
+  int* s_off = s_raw + 3 * SMDP_WG;
   int part = 0;  // thread 0: claim cursor
   for (int pass = 0;; ++pass) {
     int tile;
@@ -652,6 +724,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_expand(LArgs a, int r) {
       for (int t = 0; t < kLayerIlp; ++t) pushes += out.push(won[t], v[t], qout, &cout->qlen);
     }
     __syncthreads();
+  }
   }
   out.flush(qout, &cout->qlen);
   edges = wave_sum(edges);

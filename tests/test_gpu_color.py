@@ -82,3 +82,16 @@ def test_color_edge_cases(ctx):
         for sol in SOLS:
             c, _ = ctx.color(g, None, sol)
             assert np.array_equal(c, ref)
+
+
+@pytest.mark.parametrize("use_prio", [False, True])
+def test_color_layers_long_and_short(ctx, use_prio):
+    """RMAT-15 symmetrised: the first layers exceed 2048 vertices (edge-balanced tiles with
+    grid chunks for hubs), the later ones are expanded arc-balanced without chunks."""
+    g = smdp.gen_rmat(15, 8, symmetrize=True, seed=9)
+    prio = smdp.gen_prio(g.n, 9) if use_prio else None
+    ref, k = orc.color_closed(g.row, g.col, prio)
+    assert int((ref == 1).sum()) > 2048 and int(np.bincount(ref).max()) > 2048
+    c, st = ctx.color(g, prio, smdp.SOL_TUNED)
+    assert np.array_equal(c, ref)
+    assert st.rounds == k
