@@ -33,8 +33,10 @@ namespace {
 using smdp::GlibcRand;
 using smdp::rng_at;
 
+// Host worker threads: the hardware count, at most 16, or SMDP_HOST_THREADS.
 unsigned worker_count() {
   unsigned t = std::thread::hardware_concurrency();
+  if (const char* e = getenv("SMDP_HOST_THREADS")) t = (unsigned)std::max(1, atoi(e));
   if (t == 0) t = 1;
   if (t > 16) t = 16;
   return t;
@@ -72,31 +74,100 @@ smdp_hgraph* hg_alloc(int32_t n, int32_t m, bool weights) {
 
 // Arcs in file order -> CSR whose rows hold the arcs in reverse file order
 // (convertCSR pops the back of each adjacency list, common/global.cpp:215-221).
+// Counting sort of the arc list [0, m) by key(e) into rows of `row` (n + 1
+// offsets, filled here), on T threads that each own a contiguous slice of
+// the arcs.  place(e, p) stores arc e at position p.  Within a row the arcs
+// keep their list order (reverse = false) or the reverse of it (reverse =
+// true): thread t's arcs of row v take the block after (or before) those of
+// threads t' < t, so the result equals the sequential fill exactly.
+template <typename Key, typename Place>
+void parallel_row_fill(int32_t n, int64_t m, int32_t* row, bool reverse, Key key, Place place) {
+  // per-thread counts: T x n ints, T kept so that they stay under 1 GiB
+  int64_t T = std::min<int64_t>(worker_count(), std::max<int64_t>(1, ((int64_t)1 << 28) / std::max(1, n)));
+  if (m < ((int64_t)1 << 16)) T = 1;
+  std::vector<int32_t> cnt((size_t)(T * n), 0);
+  auto slice = [&](int64_t t, int64_t* b, int64_t* e) {
+    *b = m * t / T;
+    *e = m * (t + 1) / T;
+  };
+  auto run = [&](const std::function<void(int64_t)>& fn) {
+    std::vector<std::thread> pool;
+    for (int64_t t = 1; t < T; ++t) pool.emplace_back([=, &fn] { fn(t); });
+    fn(0);
+    for (auto& th : pool) th.join();
+  };
+  run([&](int64_t t) {
+    int64_t b, e;
+    slice(t, &b, &e);
+    int32_t* c = cnt.data() + t * n;
+    for (int64_t k = b; k < e; ++k) ++c[key(k)];
+  });
+  // rows (totals in parallel, prefix in order), then each thread's start
+  // inside each row
+  parallel_slices(n, [&](int64_t b, int64_t e) {
+    for (int64_t v = b; v < e; ++v) {
+      int32_t tot = 0;
+      for (int64_t t = 0; t < T; ++t) tot += cnt[(size_t)(t * n + v)];
+      row[v + 1] = tot;
+    }
+  });
+  row[0] = 0;
+  for (int32_t v = 0; v < n; ++v) row[v + 1] += row[v];
+  parallel_slices(n, [&](int64_t b, int64_t e) {
+    for (int64_t v = b; v < e; ++v) {
+      int32_t at = row[v];
+      for (int64_t i = 0; i < T; ++i) {
+        const int64_t t = reverse ? T - 1 - i : i;
+        const int32_t c = cnt[(size_t)(t * n + v)];
+        cnt[(size_t)(t * n + v)] = reverse ? at + c : at;  // reverse fills its block from the end
+        at += c;
+      }
+    }
+  });
+  run([&](int64_t t) {
+    int64_t b, e;
+    slice(t, &b, &e);
+    int32_t* c = cnt.data() + t * n;
+    if (reverse) {
+      for (int64_t k = b; k < e; ++k) place(k, --c[key(k)]);
+    } else {
+      for (int64_t k = b; k < e; ++k) place(k, c[key(k)]++);
+    }
+  });
+}
+
 int build_csr_reversed(int32_t n, const std::vector<int32_t>& tail, const std::vector<int32_t>& head,
                        const std::vector<int32_t>* wt, smdp_hgraph** out) {
   int64_t m = (int64_t)tail.size();
   if (m > INT32_MAX) return SMDP_ERR_IO;
   smdp_hgraph* g = hg_alloc(n, (int32_t)m, wt != nullptr);
   if (!g) return SMDP_ERR_NOMEM;
-  std::vector<int32_t> cnt((size_t)n + 1, 0);
-  for (int64_t e = 0; e < m; ++e) cnt[tail[e]]++;
-  g->row[0] = 0;
-  for (int32_t v = 0; v < n; ++v) g->row[v + 1] = g->row[v] + cnt[v];
-  // Fill each row from its end: the first arc read lands last.
-  std::vector<int32_t> pos(g->row + 1, g->row + n + 1);
-  for (int64_t e = 0; e < m; ++e) {
-    int32_t p = --pos[tail[e]];
-    g->col[p] = head[e];
-    if (wt) g->w[p] = (*wt)[e];
-  }
+  // Each row holds its arcs in reverse read order: the first arc read lands last.
+  parallel_row_fill(n, m, g->row, true, [&](int64_t e) { return tail[(size_t)e]; },
+                    [&](int64_t e, int32_t p) {
+                      g->col[p] = head[(size_t)e];
+                      if (wt) g->w[p] = (*wt)[(size_t)e];
+                    });
   *out = g;
   return SMDP_OK;
 }
 
-// Whole-file reader.
+// Whole-file reader: one read of the file's size when it is known.
 bool slurp(const char* path, std::string* s) {
   FILE* f = fopen(path, "rb");
   if (!f) return false;
+  if (fseek(f, 0, SEEK_END) == 0) {
+    const long size = ftell(f);
+    if (size > 0 && fseek(f, 0, SEEK_SET) == 0) {
+      s->resize((size_t)size);
+      const size_t k = fread(&(*s)[0], 1, (size_t)size, f);
+      s->resize(k);
+    } else {
+      rewind(f);
+    }
+  } else {
+    rewind(f);
+  }
   char buf[1 << 16];
   size_t k;
   while ((k = fread(buf, 1, sizeof buf, f)) > 0) s->append(buf, k);
@@ -128,8 +199,130 @@ void split_spaces(const char* b, const char* e, std::vector<std::pair<const char
   }
 }
 
+// The DIMACS9 rules of read_dimacs9 on all host threads.  The text is cut at
+// line starts; each thread parses its slice ('p', 's' and 'a' lines) into
+// arrays of its own, which are joined in file order.  A file that is not one
+// 'p' line followed by the arcs (an 'a' before the 'p' line, or several 'p'
+// lines, which the sequential rules handle line by line) returns 1 so that
+// the caller uses the sequential reader.
+int read_dimacs9_parallel(const std::string& s, smdp_hgraph** out) {
+  const int64_t len = (int64_t)s.size();
+  const int64_t T = std::min<int64_t>(worker_count(), std::max<int64_t>(1, len >> 20));
+  if (T < 2) return 1;
+  const char* base = s.data();
+  std::vector<int64_t> cut((size_t)T + 1);
+  cut[0] = 0;
+  cut[(size_t)T] = len;
+  for (int64_t t = 1; t < T; ++t) {
+    int64_t c = std::max(cut[(size_t)t - 1], len * t / T);
+    while (c < len && c > 0 && base[c - 1] != '\n') ++c;
+    cut[(size_t)t] = c;
+  }
+  struct Part {
+    std::vector<int32_t> tail, head, wt;
+    int64_t first_a = -1;         // offset of the first 'a' line
+    std::vector<int64_t> p_at;    // offsets of 'p' lines
+    int32_t n = -1, mdecl = 0;    // from the last 'p' line of the slice
+    int64_t s_at = -1;            // offset of the last 's' line
+    int32_t src = 0;
+    bool bad = false;
+  };
+  std::vector<Part> parts((size_t)T);
+  std::vector<std::thread> pool;
+  auto work = [&](int64_t t) {
+    Part& P = parts[(size_t)t];
+    std::vector<std::pair<const char*, const char*>> toks;
+    const char* p = base + cut[(size_t)t];
+    const char* end = base + cut[(size_t)t + 1];
+    const char* fend = base + len;
+    P.tail.reserve((size_t)((end - p) / 12));
+    P.head.reserve((size_t)((end - p) / 12));
+    P.wt.reserve((size_t)((end - p) / 12));
+    while (p < end) {
+      const char* eol = (const char*)memchr(p, '\n', fend - p);
+      const char* le = eol ? eol + 1 : fend;
+      const char c = *p;
+      if (c == 'p') {
+        if (le - p < 4) { P.bad = true; break; }
+        split_spaces(p + 4, le, &toks);
+        if (toks.size() < 2) { P.bad = true; break; }
+        P.p_at.push_back(p - base);
+        P.n = atoi_span(toks[0].first, toks[0].second);
+        P.mdecl = atoi_span(toks[1].first, toks[1].second);
+      } else if (c == 's') {
+        split_spaces(p + 1, le, &toks);
+        if (toks.empty()) { P.bad = true; break; }
+        P.s_at = p - base;
+        P.src = atoi_span(toks[0].first, toks[0].second) - 1;
+      } else if (c == 'a') {
+        split_spaces(p + 1, le, &toks);
+        if (toks.size() < 3) { P.bad = true; break; }
+        if (P.first_a < 0) P.first_a = p - base;
+        P.tail.push_back(atoi_span(toks[0].first, toks[0].second) - 1);
+        P.head.push_back(atoi_span(toks[1].first, toks[1].second) - 1);
+        P.wt.push_back(atoi_span(toks[2].first, toks[2].second));
+      }
+      p = le;
+    }
+  };
+  for (int64_t t = 1; t < T; ++t) pool.emplace_back(work, t);
+  work(0);
+  for (auto& th : pool) th.join();
+  // one 'p' line, before every 'a' line; otherwise the sequential rules decide
+  int64_t np = 0, p_at = -1, first_a = -1, s_at = -1;
+  int32_t n = -1, mdecl = 0, src = 0;
+  bool bad = false;
+  for (const Part& P : parts) {
+    np += (int64_t)P.p_at.size();
+    if (!P.p_at.empty()) {
+      p_at = P.p_at.back();
+      n = P.n;
+      mdecl = P.mdecl;
+    }
+    if (first_a < 0 && P.first_a >= 0) first_a = P.first_a;
+    if (P.s_at >= 0) {
+      s_at = P.s_at;
+      src = P.src;
+    }
+    bad = bad || P.bad;
+  }
+  if (np != 1 || (first_a >= 0 && first_a < p_at)) return 1;
+  (void)s_at;
+  if (bad || n < 0 || mdecl < 0) return SMDP_ERR_IO;
+  std::vector<int64_t> off((size_t)T + 1, 0);
+  for (int64_t t = 0; t < T; ++t) off[(size_t)t + 1] = off[(size_t)t] + (int64_t)parts[(size_t)t].tail.size();
+  const int64_t m = off[(size_t)T];
+  if (m != mdecl) return SMDP_ERR_IO;
+  if (n > 0 && (src < 0 || src >= n)) return SMDP_ERR_IO;
+  std::vector<int32_t> tail((size_t)m), head((size_t)m), wt((size_t)m);
+  std::vector<char> range_ok((size_t)T, 1);
+  pool.clear();
+  auto join = [&](int64_t t) {
+    const Part& P = parts[(size_t)t];
+    const size_t o = (size_t)off[(size_t)t];
+    for (size_t k = 0; k < P.tail.size(); ++k) {
+      const int32_t u = P.tail[k], v = P.head[k];
+      if (u < 0 || u >= n || v < 0 || v >= n) range_ok[(size_t)t] = 0;
+      tail[o + k] = u;
+      head[o + k] = v;
+      wt[o + k] = P.wt[k];
+    }
+  };
+  for (int64_t t = 1; t < T; ++t) pool.emplace_back(join, t);
+  join(0);
+  for (auto& th : pool) th.join();
+  for (char ok : range_ok)
+    if (!ok) return SMDP_ERR_IO;
+  parts.clear();
+  int rc = build_csr_reversed(n, tail, head, &wt, out);
+  if (rc == SMDP_OK) (*out)->source = src;
+  return rc;
+}
+
 int read_dimacs9(const std::string& s, smdp_hgraph** out) {
   // common/global.cpp:29-64
+  const int prc = read_dimacs9_parallel(s, out);
+  if (prc != 1) return prc;
   int32_t n = -1, mdecl = 0, src = 0;
   std::vector<int32_t> tail, head, wt;
   std::vector<std::pair<const char*, const char*>> toks;
@@ -305,16 +498,11 @@ int csr_from_arcs(int32_t n, const std::vector<int32_t>& tail, const std::vector
   if (m > INT32_MAX) return SMDP_ERR_ARG;
   smdp_hgraph* g = hg_alloc(n, (int32_t)m, wt != nullptr);
   if (!g) return SMDP_ERR_NOMEM;
-  std::vector<int32_t> cnt((size_t)n, 0);
-  for (int64_t e = 0; e < m; ++e) cnt[tail[e]]++;
-  g->row[0] = 0;
-  for (int32_t v = 0; v < n; ++v) g->row[v + 1] = g->row[v] + cnt[v];
-  std::vector<int32_t> pos(g->row, g->row + n);
-  for (int64_t e = 0; e < m; ++e) {
-    int32_t p = pos[tail[e]]++;
-    g->col[p] = head[e];
-    if (wt) g->w[p] = (*wt)[e];
-  }
+  parallel_row_fill(n, m, g->row, false, [&](int64_t e) { return tail[(size_t)e]; },
+                    [&](int64_t e, int32_t p) {
+                      g->col[p] = head[(size_t)e];
+                      if (wt) g->w[p] = (*wt)[(size_t)e];
+                    });
   *out = g;
   return SMDP_OK;
 }
@@ -349,13 +537,15 @@ int smdp_invert(const smdp_hgraph* g, smdp_hgraph** out) {
   if (!g || !out) return SMDP_ERR_ARG;
   smdp_hgraph* r = hg_alloc(g->n, g->m, false);
   if (!r) return SMDP_ERR_NOMEM;
-  std::vector<int32_t> cnt((size_t)g->n, 0);
-  for (int32_t e = 0; e < g->m; ++e) cnt[g->col[e]]++;
-  r->row[0] = 0;
-  for (int32_t v = 0; v < g->n; ++v) r->row[v + 1] = r->row[v] + cnt[v];
-  std::vector<int32_t> pos(r->row + 1, r->row + g->n + 1);
-  for (int32_t u = 0; u < g->n; ++u)
-    for (int32_t e = g->row[u]; e < g->row[u + 1]; ++e) r->col[--pos[g->col[e]]] = u;
+  // the source of arc e (arcs are in row order, so reversing the list order
+  // gives descending sources in every in-row)
+  std::vector<int32_t> src((size_t)g->m);
+  parallel_slices(g->n, [&](int64_t b, int64_t e) {
+    for (int64_t u = b; u < e; ++u)
+      for (int32_t k = g->row[u]; k < g->row[u + 1]; ++k) src[(size_t)k] = (int32_t)u;
+  });
+  parallel_row_fill(g->n, g->m, r->row, true, [&](int64_t e) { return g->col[e]; },
+                    [&](int64_t e, int32_t p) { r->col[p] = src[(size_t)e]; });
   r->source = g->source;
   *out = r;
   return SMDP_OK;

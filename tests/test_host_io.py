@@ -190,3 +190,84 @@ def test_spmv_reference_inputs():
     data, x = smdp.gen_spmv_reference_inputs(20, 30)
     assert data.tolist() == [i % 13 + 0.5 for i in range(30)]
     assert x.tolist() == [i % 17 + 0.5 for i in range(20)]
+
+
+# ---- parallel ingest -------------------------------------------------------
+
+def _with_threads(k, fn):
+    old = os.environ.get("SMDP_HOST_THREADS")
+    os.environ["SMDP_HOST_THREADS"] = str(k)
+    try:
+        return fn()
+    finally:
+        if old is None:
+            del os.environ["SMDP_HOST_THREADS"]
+        else:
+            os.environ["SMDP_HOST_THREADS"] = old
+
+
+def _same(a, b):
+    assert a.n == b.n and a.m == b.m and a.source == b.source
+    assert np.array_equal(a.row, b.row) and np.array_equal(a.col, b.col)
+    assert (a.w is None and b.w is None) or np.array_equal(a.w, b.w)
+
+
+def test_parallel_ingest_matches_one_thread(tmp_path):
+    """A DIMACS9 file of several MB is parsed by 8 threads and its CSR built and inverted in parallel:
+    the result must equal the one-thread run bit for bit."""
+    g = smdp.gen_rmat(16, 16, wlo=1, whi=999, seed=12)
+    path = str(tmp_path / "big.gr")
+    smdp.write_dimacs9(path, g)
+    assert os.path.getsize(path) > 8 << 20
+    one = _with_threads(1, lambda: smdp.read_graph(path, smdp.FMT_DIMACS9))
+    many = _with_threads(8, lambda: smdp.read_graph(path, smdp.FMT_DIMACS9))
+    _same(one, many)
+    # the writer emits each row's arcs last first, so the read reproduces the CSR
+    assert np.array_equal(many.row, g.row) and np.array_equal(many.col, g.col) and np.array_equal(many.w, g.w)
+    _same(_with_threads(1, lambda: smdp.invert(many)), _with_threads(8, lambda: smdp.invert(many)))
+    # generated graphs use the stable (non-reversed) parallel fill
+    _same(_with_threads(1, lambda: smdp.gen_rmat(15, 8, seed=3)), _with_threads(8, lambda: smdp.gen_rmat(15, 8, seed=3)))
+
+
+@needs_ref
+def test_parallel_ingest_matches_reference(tmp_path):
+    g = smdp.gen_uniform(200000, 8, 1, 1000, 5)
+    path = str(tmp_path / "u.gr")
+    smdp.write_dimacs9(path, g)
+    ours = _with_threads(8, lambda: smdp.read_graph(path, smdp.FMT_DIMACS9))
+    ref = orc.ref_read(path, 0, invert=True)
+    assert ref["n"] == ours.n and ref["m"] == ours.m and ref["source"] == ours.source
+    assert np.array_equal(ref["row"], ours.row) and np.array_equal(ref["col"], ours.col)
+    assert np.array_equal(ref["w"], ours.w)
+    inv = _with_threads(8, lambda: smdp.invert(ours))
+    assert np.array_equal(ref["in_row"], inv.row) and np.array_equal(ref["in_col"], inv.col)
+
+
+def test_parallel_ingest_line_rules(tmp_path):
+    """Comments, several 's' lines (the last wins), a last line without newline; a second 'p'
+    line or an arc before 'p' follow the line-by-line rules (read by the sequential reader)."""
+    rng = np.random.default_rng(3)
+    n, m = 5000, 300000
+    u = rng.integers(1, n + 1, m)
+    v = rng.integers(1, n + 1, m)
+    w = rng.integers(1, 100, m)
+    arcs = [f"a {a} {b} {c}" for a, b, c in zip(u, v, w)]
+    head = ["c generated", f"p sp {n} {m}", "s 3"]
+    text = "\n".join(head + arcs[: m // 2] + ["c middle", "s 17"] + arcs[m // 2:])
+    path = _write(tmp_path, "rules.gr", text)  # no final newline
+    one = _with_threads(1, lambda: smdp.read_graph(path, smdp.FMT_DIMACS9))
+    many = _with_threads(8, lambda: smdp.read_graph(path, smdp.FMT_DIMACS9))
+    _same(one, many)
+    assert many.source == 16 and many.m == m
+    bad = _write(tmp_path, "early.gr", "\n".join(arcs[:10] + head + arcs[10:]) + "\n")
+    for k in (1, 8):
+        with pytest.raises(smdp.SmdpError):
+            _with_threads(k, lambda: smdp.read_graph(bad, smdp.FMT_DIMACS9))
+    short = _write(tmp_path, "short.gr", "\n".join(head + arcs[:-1]) + "\n")  # one arc missing
+    for k in (1, 8):
+        with pytest.raises(smdp.SmdpError):
+            _with_threads(k, lambda: smdp.read_graph(short, smdp.FMT_DIMACS9))
+    oob = _write(tmp_path, "oob.gr", "\n".join(head + arcs[:-1] + [f"a {n + 1} 1 5"]) + "\n")
+    for k in (1, 8):
+        with pytest.raises(smdp.SmdpError):
+            _with_threads(k, lambda: smdp.read_graph(oob, smdp.FMT_DIMACS9))
