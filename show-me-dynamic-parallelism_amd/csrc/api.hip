@@ -6,6 +6,8 @@
 // freed (and the device reset) on every call.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <chrono>
 #include <cstdio>
 #include <algorithm>
@@ -260,11 +262,20 @@ int smdp_graph_create(smdp_ctx* ctx, const smdp_csr* h, smdp_graph** out) {
   if (!ctx || !h || !out || h->n < 0 || h->m < 0 || !h->row || (h->m > 0 && !h->col)) return SMDP_ERR_ARG;
   if (h->row[h->n] != h->m || h->row[0] != 0) return SMDP_ERR_ARG;
   // Every kernel indexes row[u..u+1] and dist[col[e]] without bounds checks,
-  // so the CSR is validated here once.
-  for (int32_t v = 0; v < h->n; ++v)
-    if (h->row[v + 1] < h->row[v]) return SMDP_ERR_ARG;
-  for (int32_t e = 0; e < h->m; ++e)
-    if ((uint32_t)h->col[e] >= (uint32_t)h->n) return SMDP_ERR_ARG;
+  // so the CSR is validated here once (on all host threads).
+  {
+    std::atomic<bool> bad{false};
+    parallel_ranges(h->n, [&](int64_t b, int64_t e) {
+      for (int64_t v = b; v < e; ++v)
+        if (h->row[v + 1] < h->row[v]) bad = true;
+    });
+    parallel_ranges(h->m, [&](int64_t b, int64_t e) {
+      bool local = false;
+      for (int64_t k = b; k < e; ++k) local |= (uint32_t)h->col[k] >= (uint32_t)h->n;
+      if (local) bad = true;
+    });
+    if (bad) return SMDP_ERR_ARG;
+  }
   *out = nullptr;
   SMDP_HIP(hipSetDevice(ctx->device));
   smdp_graph* g = new smdp_graph();
