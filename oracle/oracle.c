@@ -448,6 +448,24 @@ int orc_pagerank_f64_par(int n, const int* in_row, const int* in_col, const int*
   return nt;
 }
 
+/* Appends to a shared array through a per-thread buffer: one atomic reserve
+ * per ORC_APPEND entries instead of one per entry. */
+#define ORC_APPEND 1024
+typedef struct {
+  int buf[ORC_APPEND];
+  int k;
+} orc_appender;
+static void orc_append_flush(orc_appender* a, int* out, int64_t* len) {
+  if (a->k == 0) return;
+  const int64_t at = __atomic_fetch_add(len, (int64_t)a->k, __ATOMIC_RELAXED);
+  memcpy(out + at, a->buf, sizeof(int) * (size_t)a->k);
+  a->k = 0;
+}
+static void orc_append(orc_appender* a, int v, int* out, int64_t* len) {
+  a->buf[a->k++] = v;
+  if (a->k == ORC_APPEND) orc_append_flush(a, out, len);
+}
+
 /* Level-synchronous BFS: the frontier of level k is expanded by all threads,
  * a vertex joins level k+1 when a compare-and-swap claims it.  Levels equal
  * orc_bfs_fifo's (shortest hop counts). */
@@ -473,16 +491,22 @@ int orc_bfs_par(int n, const int* row, const int* col, int src, int* level) {
   int64_t nc = 1;
   for (int lv = 0; nc > 0; ++lv) {
     int64_t nn = 0;
-#pragma omp parallel for schedule(dynamic, 64)
-    for (int64_t i = 0; i < nc; ++i) {
-      const int u = cur[i];
-      for (int e = row[u]; e < row[u + 1]; ++e) {
-        const int v = col[e];
-        int exp = ORC_UNDEF;
-        if (__atomic_load_n(&level[v], __ATOMIC_RELAXED) == ORC_UNDEF &&
-            __atomic_compare_exchange_n(&level[v], &exp, lv + 1, 0, __ATOMIC_RELAXED, __ATOMIC_RELAXED))
-          nxt[__atomic_fetch_add(&nn, 1, __ATOMIC_RELAXED)] = v;
+#pragma omp parallel
+    {
+      orc_appender ap;
+      ap.k = 0;
+#pragma omp for schedule(dynamic, 64)
+      for (int64_t i = 0; i < nc; ++i) {
+        const int u = cur[i];
+        for (int e = row[u]; e < row[u + 1]; ++e) {
+          const int v = col[e];
+          int exp = ORC_UNDEF;
+          if (__atomic_load_n(&level[v], __ATOMIC_RELAXED) == ORC_UNDEF &&
+              __atomic_compare_exchange_n(&level[v], &exp, lv + 1, 0, __ATOMIC_RELAXED, __ATOMIC_RELAXED))
+            orc_append(&ap, v, nxt, &nn);
+        }
       }
+      orc_append_flush(&ap, nxt, &nn);
     }
     int* t = cur;
     cur = nxt;
@@ -543,27 +567,39 @@ int orc_color_layers_par(int n, const int* row, const int* col, const unsigned* 
     for (int e = row[v]; e < row[v + 1]; ++e) tcol[__atomic_fetch_add(&fill[col[e]], 1, __ATOMIC_RELAXED)] = v;
   free(fill);
   int64_t nc = 0;
-#pragma omp parallel for schedule(dynamic, 4096)
-  for (int v = 0; v < n; ++v) {
-    int c = 0;
-    for (int e = row[v]; e < row[v + 1]; ++e) c += col[e] != v && precedes(prio, col[e], v);
-    pend[v] = c;
-    color[v] = 0;
-    if (c == 0) cur[__atomic_fetch_add(&nc, 1, __ATOMIC_RELAXED)] = v;
+#pragma omp parallel
+  {
+    orc_appender ap;
+    ap.k = 0;
+#pragma omp for schedule(dynamic, 4096)
+    for (int v = 0; v < n; ++v) {
+      int c = 0;
+      for (int e = row[v]; e < row[v + 1]; ++e) c += col[e] != v && precedes(prio, col[e], v);
+      pend[v] = c;
+      color[v] = 0;
+      if (c == 0) orc_append(&ap, v, cur, &nc);
+    }
+    orc_append_flush(&ap, cur, &nc);
   }
   int c = 0;
   while (nc > 0) {
     ++c;
     int64_t nn = 0;
-#pragma omp parallel for schedule(dynamic, 64)
-    for (int64_t i = 0; i < nc; ++i) {
-      const int u = cur[i];
-      color[u] = c;
-      for (int e = tdeg[u]; e < tdeg[u + 1]; ++e) {
-        const int v = tcol[e];
-        if (v != u && precedes(prio, u, v) && __atomic_sub_fetch(&pend[v], 1, __ATOMIC_RELAXED) == 0)
-          nxt[__atomic_fetch_add(&nn, 1, __ATOMIC_RELAXED)] = v;
+#pragma omp parallel
+    {
+      orc_appender ap;
+      ap.k = 0;
+#pragma omp for schedule(dynamic, 64)
+      for (int64_t i = 0; i < nc; ++i) {
+        const int u = cur[i];
+        color[u] = c;
+        for (int e = tdeg[u]; e < tdeg[u + 1]; ++e) {
+          const int v = tcol[e];
+          if (v != u && precedes(prio, u, v) && __atomic_sub_fetch(&pend[v], 1, __ATOMIC_RELAXED) == 0)
+            orc_append(&ap, v, nxt, &nn);
+        }
       }
+      orc_append_flush(&ap, nxt, &nn);
     }
     int* t = cur;
     cur = nxt;

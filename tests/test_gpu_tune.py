@@ -18,7 +18,8 @@ def test_autotune_sssp_cache_and_parity(ctx, tmp_path):
     try:
         with smdp.Graph(ctx, g) as dg:
             p1 = dg.autotune(smdp.APP_SSSP, g.source, cache)
-            assert p1.split_threshold in (512, 2048, 8192) and p1.wg_per_cu in (2, 4, 8)
+            # one of the SSSP candidates of csrc/tune.hip
+            assert p1.split_threshold in (512, 2048, 8192) and p1.wg_per_cu in (4, 8, 12)
             data = json.loads(cache.read_text())
             assert len(data) == 1 and list(data)[0].startswith("gfx")
             p2 = dg.autotune(smdp.APP_SSSP, g.source, cache)  # cache hit
