@@ -377,6 +377,7 @@ struct LArgs {
   int32_t heavy;  // rows of at least this many arcs are counted by a wave
   int32_t split;  // rows of at least this many arcs go to grid chunks
   int32_t chunk;  // arcs per grid chunk
+  int32_t grid;   // workgroups of k_layer_expand (the round's tile shape depends on it)
 };
 
 constexpr int kLayerIlp = 4;
@@ -649,7 +650,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_expand(LArgs a, int r) {
       for (int t = 0; t < kLayerIlp; ++t) pushes += out.push(won[t], v[t], qout, &cout->qlen);
     }
   } else {
-  const LayerShape sh = layer_shape(a, L, (int)gridDim.x);
+  const LayerShape sh = layer_shape(a, L, a.grid);  // launched with a.grid workgroups
   const int ntiles = (L + sh.tile - 1) / sh.tile;
   // workgroups past the tile count would only contend for the counter
   if ((int)blockIdx.x >= ntiles) return;
@@ -745,7 +746,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_drain(LArgs a, int r) {
   Ctl* cout = a.ctl + ((r + 1) & 3);
   const int H = cin->hlen;
   if ((int)blockIdx.x >= H) return;
-  const int chunk = layer_shape(a, cin->qlen, (int)gridDim.x).chunk;
+  const int chunk = layer_shape(a, cin->qlen, a.grid).chunk;
   int32_t* qout = (r & 1) ? a.q0 : a.q1;
   const int newc = r + 2;
   __shared__ int s_idx;
@@ -904,6 +905,8 @@ static int run_color_layers(smdp_graph* g, int use_prio, smdp_stats* st) {
   a.split = std::max(p.split_threshold, p.heavy_threshold);
   a.chunk = p.chunk;
   const int grid = grid_size(ctx, p);
+  const int drain_grid = std::max(1, std::min(grid, ctx->cus));
+  a.grid = grid;
   hipStream_t s = ctx->stream;
   prof_reset(ctx);
   int launches = 0;
@@ -921,7 +924,9 @@ static int run_color_layers(smdp_graph* g, int use_prio, smdp_stats* st) {
       hipLaunchKernelGGL(k_layer_expand, dim3(grid), dim3(SMDP_WG), 0, s, a, r);
       prof_mark(ctx, 0);
       prof_mark(ctx, 1);
-      hipLaunchKernelGGL(k_layer_drain, dim3(grid), dim3(SMDP_WG), 0, s, a, r);
+      // one workgroup per CU claims the round's chunks: most rounds have none,
+      // and an empty launch of the full grid cost 4.3 us per round
+      hipLaunchKernelGGL(k_layer_drain, dim3(drain_grid), dim3(SMDP_WG), 0, s, a, r);
       prof_mark(ctx, 1);
       launches += 2;
     }
