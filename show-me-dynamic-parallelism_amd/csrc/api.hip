@@ -306,7 +306,7 @@ int smdp_graph_create(smdp_ctx* ctx, const smdp_csr* h, smdp_graph** out) {
 void smdp_graph_destroy(smdp_graph* g) {
   if (!g) return;
   (void)hipSetDevice(g->ctx->device);
-  void* bufs[] = {g->row,    g->col,      g->w,       g->dist,     g->stamp,  g->q[0],  g->q[1],
+  void* bufs[] = {g->row,    g->col,      g->w,       g->dist,     g->stamp,  g->hasout, g->q[0],  g->q[1],
                   g->hq,     g->ctl,      g->cnt,     g->val,      g->x,      g->y,     g->outdeg,
                   g->rank[0], g->rank[1], g->contrib, g->partials, g->chunk_part, g->prio, g->resume,
                   g->rmax,   g->mdist};

@@ -71,6 +71,11 @@ struct FArgs {
   int32_t heavy;  // lane solutions: heavy degree; tuned: split degree
   int32_t chunk;  // grid-queue chunk in edges; 0 = whole vertex
   int32_t small_q;  // tuned: a queue shorter than this splits vertices of >= kSmallSplit arcs
+  int32_t forest;   // tuned BFS: no vertex has two in-arcs, so levels are stored without atomics
+  // Tuned solution: bit v set when row v has arcs.  A vertex without arcs is
+  // never queued (there is nothing to relax from it); on RMAT-style inputs
+  // and trees most reached vertices are such.  nullptr: queue every claim.
+  const uint32_t* __restrict__ hasout;
   unsigned long long* rlog;  // profiling: per round (queue length, arcs), or nullptr
 };
 
@@ -95,13 +100,25 @@ __device__ __forceinline__ int arc_weight(const FArgs& a, int e) {
 // BFS: every lane of round r offers level r+1, so only the lane that finds v
 // unvisited lowers it; that atomic alone makes the claim unique and the stamp
 // is not needed.
+// BFS on a forest (a.forest): v has a single in-arc, so exactly one lane ever
+// offers v a level and a plain store is enough.
+__device__ __forceinline__ bool has_arcs(const FArgs& a, int v) {
+  return !a.hasout || ((a.hasout[v >> 5] >> (v & 31)) & 1u);
+}
+
 template <bool BFS>
 __device__ __forceinline__ bool relax_seen(const FArgs& a, int v, int alt, int seen, int label, Sink& out) {
   if (alt >= seen) return false;
   ++out.tries;
+  if (BFS && a.forest) {
+    a.dist[v] = alt;
+    ++out.wins;
+    return has_arcs(a, v);
+  }
   int old = atomicMin(&a.dist[v], alt);
   if (alt >= old) return false;
   ++out.wins;
+  if (!has_arcs(a, v)) return false;
   if (BFS) return true;
   if (a.stamp[v] == label) return false;
   return atomicExch(&a.stamp[v], label) != label;
@@ -606,6 +623,7 @@ struct Relabel {
   int32_t* col = nullptr;
   int32_t* w = nullptr;
   int32_t* newid = nullptr;          // caller id -> copy id (device)
+  uint32_t* hasout = nullptr;        // bit v set when row v of the copy has arcs
   std::vector<int32_t> h_newid;      // the same on the host (source mapping)
 };
 }  // namespace
@@ -619,7 +637,7 @@ __global__ void k_unpermute(const int32_t* __restrict__ d, const int32_t* __rest
 void relabel_free(smdp_graph* g) {
   Relabel* rl = static_cast<Relabel*>(g->relabel);
   if (!rl) return;
-  void* ptrs[] = {rl->row, rl->col, rl->w, rl->newid};
+  void* ptrs[] = {rl->row, rl->col, rl->w, rl->newid, rl->hasout};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   delete rl;
@@ -697,6 +715,56 @@ static int ensure_relabel(smdp_graph* g, Relabel** out) {
   return SMDP_OK;
 }
 
+// Bit v of words[v / 32] set when row v has arcs.
+__global__ void k_hasout(const int32_t* __restrict__ row, int32_t n, uint32_t* words) {
+  const int32_t nw = (n + 31) / 32;
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += gridDim.x * blockDim.x) {
+    uint32_t b = 0;
+    for (int k = 0; k < 32; ++k) {
+      const int32_t v = i * 32 + k;
+      if (v < n && row[v + 1] > row[v]) b |= 1u << k;
+    }
+    words[i] = b;
+  }
+}
+
+static int build_hasout(smdp_graph* g, const int32_t* row, uint32_t** out) {
+  if (*out) return SMDP_OK;
+  const int64_t nw = std::max<int64_t>(1, ((int64_t)g->n + 31) / 32);
+  SMDP_CHECK(alloc_once(out, nw));
+  hipLaunchKernelGGL(k_hasout, dim3((unsigned)std::min<int64_t>((nw + 255) / 256, 4096)), dim3(256), 0,
+                     g->ctx->stream, row, g->n, *out);
+  SMDP_HIP(hipGetLastError());
+  return SMDP_OK;
+}
+
+// Whether every vertex has at most one in-arc, checked once per resident graph.
+__global__ void k_indeg_at_most_one(const int32_t* __restrict__ col, int64_t m, int32_t* cnt, int32_t* multi) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += (int64_t)gridDim.x * blockDim.x)
+    if (atomicAdd(&cnt[col[e]], 1) >= 1) *multi = 1;
+}
+
+static int ensure_forest(smdp_graph* g) {
+  if (g->forest >= 0) return SMDP_OK;
+  int32_t* cnt = nullptr;
+  SMDP_CHECK(alloc_once(&cnt, (int64_t)g->n + 1));
+  hipStream_t s = g->ctx->stream;
+  int rc = SMDP_OK;
+  if (hipMemsetAsync(cnt, 0, sizeof(int32_t) * ((size_t)g->n + 1), s) != hipSuccess) rc = SMDP_ERR_DEVICE;
+  if (rc == SMDP_OK && g->m > 0) {
+    hipLaunchKernelGGL(k_indeg_at_most_one, dim3(std::max(1, g->ctx->cus * 8)), dim3(256), 0, s, g->col,
+                       (int64_t)g->m, cnt, cnt + g->n);
+    if (hipGetLastError() != hipSuccess) rc = SMDP_ERR_DEVICE;
+  }
+  int32_t multi = 1;
+  if (rc == SMDP_OK && (hipMemcpyAsync(&multi, cnt + g->n, sizeof multi, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                        hipStreamSynchronize(s) != hipSuccess))
+    rc = SMDP_ERR_DEVICE;
+  (void)hipFree(cnt);
+  if (rc == SMDP_OK) g->forest = multi ? 0 : 1;
+  return rc;
+}
+
 static bool use_relabel(const smdp_params& p, const smdp_graph* g) {
   return p.layout == SMDP_LAYOUT_HOT || (p.layout == SMDP_LAYOUT_AUTO && !g->transient && g->m >= (1 << 20));
 }
@@ -744,6 +812,17 @@ int run_frontier(smdp_graph* g, int32_t src, int sol, bool bfs, smdp_stats* st) 
   if (a.rlog) SMDP_HIP(hipMemsetAsync(a.rlog, 0, sizeof(unsigned long long) * 2 * kRoundLogCap, ctx->stream));
   const int grid = grid_size(ctx, p);
   a.small_q = grid * SMDP_WG;
+  a.hasout = nullptr;
+  if (tuned) {
+    uint32_t** slot = relabel ? &static_cast<Relabel*>(g->relabel)->hasout : &g->hasout;
+    SMDP_CHECK(build_hasout(g, row, slot));
+    a.hasout = *slot;
+  }
+  a.forest = 0;
+  if (tuned && bfs && !g->transient) {
+    SMDP_CHECK(ensure_forest(g));
+    a.forest = g->forest;
+  }
   const int32_t inf = bfs ? SMDP_BFS_UNDEF : SMDP_SSSP_INF;
   const bool two_phase = sol == SMDP_SOL_GRID || tuned;
   hipStream_t s = ctx->stream;

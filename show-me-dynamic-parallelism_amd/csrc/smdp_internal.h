@@ -160,6 +160,8 @@ struct smdp_graph {
   void* relabel = nullptr;   // renumbered copy for the tuned frontier (frontier.hip)
   bool dist_perm = false;    // dist holds the renumbered copy's result
   bool transient = false;    // created by a drop-in call for one solve
+  int8_t forest = -1;        // every vertex has at most one in-arc: 1 yes, 0 no, -1 not checked yet
+  uint32_t* hasout = nullptr; // tuned frontier: bit v set when row v has arcs
 };
 
 namespace smdp {

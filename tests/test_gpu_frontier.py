@@ -186,3 +186,32 @@ def test_hot_layout_is_frontier_only(ctx):
         with pytest.raises(smdp.SmdpError):
             ctx.set_params(app, layout=smdp.LAYOUT_HOT)
         assert ctx.get_params(app).as_dict() == old.as_dict()
+
+
+def _with_arc(g, u, v):
+    """g plus the arc u -> v (appended to row u)."""
+    row = np.asarray(g.row, np.int64).copy()
+    col = np.asarray(g.col).tolist()
+    col.insert(int(row[u + 1]), v)
+    row[u + 1:] += 1
+    return smdp.HostGraph(row.astype(np.int32), np.asarray(col, np.int32))
+
+
+def test_bfs_forest_fast_path(ctx):
+    """Resident BFS on graphs whose vertices have at most one in-arc stores levels without atomics;
+    a graph with a vertex of two in-arcs keeps the atomic path.  Levels must equal the oracle's."""
+    t = smdp.gen_zipf_tree(200000, 1.5, 4096, 7)
+    cyc = _with_arc(t, int(np.argmax(np.diff(t.row) == 0)), 0)  # a leaf points back at the root
+    two = _with_arc(t, 5, 17)                                    # vertex 17 gets a second in-arc
+    half = smdp.gen_zipf_tree(1000, 1.5, 64, 3)
+    # two trees side by side, BFS from the second one's root: the first stays unreached
+    forest = smdp.HostGraph(np.concatenate([half.row, half.row[1:] + half.m]).astype(np.int32),
+                            np.concatenate([half.col, half.col + half.n]).astype(np.int32))
+    for name, g, src in (("tree", t, 0), ("cycle to root", cyc, 0), ("two in-arcs", two, 0),
+                         ("forest", forest, half.n)):
+        print(f"forest case {name}", flush=True)
+        ref = orc.bfs(g.row, g.col, src)
+        with smdp.Graph(ctx, g) as dg:
+            for _ in range(2):
+                dg.bfs(src, smdp.SOL_TUNED)
+                assert np.array_equal(dg.read(smdp.RES_LEVEL), ref), name
