@@ -85,6 +85,12 @@ def test_bfs_zipf_tree_4m(ctx):
         parent = np.repeat(np.arange(t.n), np.diff(t.row))
         assert lv[0] == 0 and np.array_equal(lv[t.col], lv[parent] + 1)
         assert np.array_equal(lv, orc.bfs(t.row, t.col, 0))
+    # resident: the renumbered copy, the forest path (one in-arc per vertex) and no queue entries for leaves
+    ref = orc.bfs(t.row, t.col, 0)
+    with smdp.Graph(ctx, t) as dg:
+        for _ in range(2):
+            dg.bfs(0, smdp.SOL_TUNED)
+            assert np.array_equal(dg.read(smdp.RES_LEVEL), ref)
 
 
 def test_batched_rmat21_and_small_graphs(ctx):
