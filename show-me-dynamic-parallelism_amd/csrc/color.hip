@@ -490,18 +490,23 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_count_chunks(LArgs a) {
 }
 
 // Colour 1 for the vertices without preceding neighbours: the queue of round 0.
+// A vertex that no row lists (no transposed arcs) has nobody to notify and is
+// coloured without being queued; RMAT-23's first layer is mostly such.
 __global__ __launch_bounds__(SMDP_WG) void k_layer_seed(LArgs a) {
   __shared__ int32_t s_out[SMDP_WAVES_PER_WG * kCOut];
   WaveOut<kCOut> out;
   out.init(s_out + (threadIdx.x / SMDP_WAVE) * kCOut);
+  // colour 1 exists whenever n > 0, even if none of its vertices is queued
+  if (blockIdx.x == 0 && threadIdx.x == 0 && a.n > 0) atomicMax(&cnt_of(a.cnt)->rounds, 1ull);
   for (int base = blockIdx.x * SMDP_WG; base < a.n; base += gridDim.x * SMDP_WG) {
     const int v = base + (int)threadIdx.x;
-    bool ready = false;
+    bool ready = false, queue = false;
     if (v < a.n) {
       ready = a.pend[v] == 0;
       a.color[v] = ready ? 1 : 0;
+      queue = ready && a.trow[v + 1] > a.trow[v];
     }
-    out.push(ready, v, a.q0, &a.ctl[0].qlen);
+    out.push(queue, v, a.q0, &a.ctl[0].qlen);
   }
   out.flush(a.q0, &a.ctl[0].qlen);
 }
