@@ -248,17 +248,18 @@ __global__ __launch_bounds__(kBinNT, 2) void k_bin_reduce(RArgs a, BArgs b, int 
     uint16_t perm[kRowsPerThread];
     float rold[kRowsPerThread];
     int od[kRowsPerThread];
+    // Loads go to clamped positions (a bin has at least one row): a load under
+    // a per-lane condition made the compiler wait for it before the next one,
+    // which held back the bin's value loads by three round trips.  Values
+    // loaded past nr are never used.
 #pragma unroll
-    for (int q = 0; q < kRowLoads; ++q) {
-      const int i = (int)threadIdx.x + q * kBinNT;
-      roff[q] = i <= nr ? a.row[r0 + i] : 0;
-    }
+    for (int q = 0; q < kRowLoads; ++q) roff[q] = a.row[r0 + min((int)threadIdx.x + q * kBinNT, nr)];
 #pragma unroll
     for (int q = 0; q < kRowsPerThread; ++q) {
-      const int r = (int)threadIdx.x + q * kBinNT;
-      perm[q] = r < nr ? b.rperm[r0 + r] : (uint16_t)0;
-      rold[q] = r < nr ? a.rold[r0 + r] : 0.f;
-      od[q] = r < nr ? a.outdeg[r0 + r] : 0;
+      const int r = r0 + min((int)threadIdx.x + q * kBinNT, nr - 1);
+      perm[q] = b.rperm[r];
+      rold[q] = a.rold[r];
+      od[q] = a.outdeg[r];
     }
     static_assert(kQuads * kBinNT * 4 >= kBinCap + 3, "one round of quads covers a bin");
     f32x4 v[kQuads];
