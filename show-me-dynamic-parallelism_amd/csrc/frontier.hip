@@ -116,17 +116,63 @@ __device__ __forceinline__ bool relax_seen(const FArgs& a, int v, int alt, int s
     return has_arcs(a, v);
   }
   int old = atomicMin(&a.dist[v], alt);
+  // read while the atomic is in flight: neither depends on its result, and
+  // the stamp is only a filter (the exchange decides)
+  const bool arcs = has_arcs(a, v);
+  const int stamp = BFS ? 0 : a.stamp[v];
   if (alt >= old) return false;
   ++out.wins;
-  if (!has_arcs(a, v)) return false;
+  if (!arcs) return false;
   if (BFS) return true;
-  if (a.stamp[v] == label) return false;
+  if (stamp == label) return false;
   return atomicExch(&a.stamp[v], label) != label;
 }
 
 template <bool BFS>
 __device__ __forceinline__ bool relax(const FArgs& a, int v, int alt, int label, Sink& out) {
   return relax_seen<BFS>(a, v, alt, a.dist[v], label, out);
+}
+
+// K candidate relaxations per lane; want[t] says the plain read found v[t]
+// improvable.  About 4% of SSSP's arcs get that far, yet with 64 lanes each
+// of the K slots has a candidate in most steps, and a slot's atomics (min,
+// stamp read, exchange) run in a branch that waits for each of them before
+// the next slot: up to 3K dependent round trips per step.  The candidates
+// are packed instead to the low lanes through the wave's LDS area (s_pair:
+// 2 * K * 64 ints), so the wave relaxes them one per lane, normally in a
+// single round, and the winners are queued from there.
+template <bool BFS, int K>
+__device__ __forceinline__ void relax_packed(const FArgs& a, const int* v, const int* alt, const bool* want,
+                                             int label, int32_t* s_pair, Sink& out) {
+  const int lane = lane_id();
+  int total = 0;
+#pragma unroll
+  for (int t = 0; t < K; ++t) {
+    const uint64_t m = ballot(want[t]);
+    if (want[t]) {
+      const int i = total + mbcnt(m);
+      s_pair[2 * i] = v[t];
+      s_pair[2 * i + 1] = alt[t];
+    }
+    total += __popcll(m);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (int r0 = 0; r0 < total; r0 += SMDP_WAVE) {
+    const int i = r0 + lane;
+    int vv = 0;
+    bool p = false;
+    if (i < total) {
+      vv = s_pair[2 * i];
+      p = relax_seen<BFS>(a, vv, s_pair[2 * i + 1], INT32_MAX, label, out);
+    }
+    out.push(p, vv);
+  }
+  // s_pair is written again by the next step
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __device__ __forceinline__ void clear_slot(Ctl* ctl, int r) { ctl_clear(ctl, (r + 2) & 3); }
@@ -377,6 +423,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_expand_tile(FArgs a, int r) {
   __shared__ int s_off[SMDP_WG];
   __shared__ int s_scan[SMDP_WAVES_PER_WG];
   __shared__ int32_t s_out[SMDP_WAVES_PER_WG * kOutCap];
+  __shared__ int32_t s_pair[SMDP_WAVES_PER_WG * 2 * kIlp * SMDP_WAVE];  // relax_packed
   Sink out;
   out.init(s_out + (threadIdx.x / SMDP_WAVE) * kOutCap, qout, &cout->qlen);
 
@@ -443,13 +490,15 @@ __global__ __launch_bounds__(SMDP_WG) void k_expand_tile(FArgs a, int r) {
       int seen[kIlp];
 #pragma unroll
       for (int t = 0; t < kIlp; ++t) seen[t] = a.dist[v[t]];
+      bool want[kIlp];
 #pragma unroll
       for (int t = 0; t < kIlp; ++t) {
         const bool ok = jb + (int)threadIdx.x + t * SMDP_WG < tot;
-        const bool p = ok && relax_seen<BFS>(a, v[t], alt[t], seen[t], label, out);
+        want[t] = ok && alt[t] < seen[t];
         edges += ok;
-        out.push(p, v[t]);
       }
+      relax_packed<BFS, kIlp>(a, v, alt, want, label, s_pair + (threadIdx.x / SMDP_WAVE) * 2 * kIlp * SMDP_WAVE,
+                              out);
     }
     __syncthreads();
     }
@@ -481,6 +530,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_drain(FArgs a, int r) {
   __shared__ int s_du[kDrainTile];
   __shared__ int s_off[kDrainTile];
   __shared__ int32_t s_out[SMDP_WAVES_PER_WG * kOutCap];
+  __shared__ int32_t s_pair[SMDP_WAVES_PER_WG * 2 * kIlp * SMDP_WAVE];  // relax_packed
   Sink out;
   out.init(s_out + (threadIdx.x / SMDP_WAVE) * kOutCap, qout, &cout->qlen);
   unsigned long long edges = 0;
@@ -528,13 +578,15 @@ __global__ __launch_bounds__(SMDP_WG) void k_drain(FArgs a, int r) {
       int seen[kIlp];
 #pragma unroll
       for (int t = 0; t < kIlp; ++t) seen[t] = a.dist[v[t]];
+      bool want[kIlp];
 #pragma unroll
       for (int t = 0; t < kIlp; ++t) {
         const bool ok = jb + (int)threadIdx.x + t * SMDP_WG < tot;
-        const bool p = ok && relax_seen<BFS>(a, v[t], alt[t], seen[t], label, out);
+        want[t] = ok && alt[t] < seen[t];
         edges += ok;
-        out.push(p, v[t]);
       }
+      relax_packed<BFS, kIlp>(a, v, alt, want, label, s_pair + (threadIdx.x / SMDP_WAVE) * 2 * kIlp * SMDP_WAVE,
+                              out);
     }
     __syncthreads();
   }
