@@ -511,16 +511,20 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_seed(LArgs a) {
   out.flush(a.q0, &a.ctl[0].qlen);
 }
 
-// Arcs u -> v[t] of the transposed CSR (ok[t]: the arc exists; v[t] is a
-// valid vertex for every t).  The priorities are gathered unconditionally, K
-// at a time (a load under a per-lane condition makes the compiler wait for it
-// before the next one).  The atomics stay conditional: unconditional ones
-// (subtracting 0) were measured 38x slower, since the padded slots of a
-// step's last pass all hit one vertex.  The lane that brings pend[v] to zero
-// colours v and queues it (won[t]).
+// Arcs u[t] -> v[t] of the transposed CSR, K per lane (ok[t]: the arc exists;
+// v[t] is a valid vertex for every t).  An arc whose source precedes its
+// target decrements pend[v]; the lane that brings it to zero colours v with
+// newc and queues it.  Priorities are gathered unconditionally, K at a time.
+// The decrements are packed to the low lanes through the wave's LDS area
+// (s_pack: K * 64 ints) and issued one per lane: K conditional atomics ran
+// one after another, each waiting for its result (about half of the arcs
+// decrement).  Unconditional atomics that subtract 0 were measured 38x
+// slower, since the padded slots of a step's last pass all hit one vertex.
+// Returns the vertices this wave queued.
 template <int K>
-__device__ __forceinline__ void notify(const LArgs& a, const int* u, const uint32_t* pu, const int* v,
-                                       const bool* ok, int newc, bool* won) {
+__device__ __forceinline__ int notify_packed(const LArgs& a, const int* u, const uint32_t* pu, const int* v,
+                                             const bool* ok, int newc, int32_t* s_pack, WaveOut<kCOut>& out,
+                                             int32_t* qout, int32_t* qlen) {
   bool want[K];
   if (a.prio) {
     uint32_t pv[K];
@@ -532,14 +536,33 @@ __device__ __forceinline__ void notify(const LArgs& a, const int* u, const uint3
 #pragma unroll
     for (int t = 0; t < K; ++t) want[t] = ok[t] && u[t] < v[t];
   }
-  int old[K];
-#pragma unroll
-  for (int t = 0; t < K; ++t) old[t] = want[t] ? atomicSub(&a.pend[v[t]], 1) : 0;
+  int total = 0;
 #pragma unroll
   for (int t = 0; t < K; ++t) {
-    won[t] = want[t] && old[t] == 1;
-    if (won[t]) a.color[v[t]] = newc;
+    const uint64_t m = ballot(want[t]);
+    if (want[t]) s_pack[total + mbcnt(m)] = v[t];
+    total += __popcll(m);
   }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  int pushes = 0;
+  for (int r0 = 0; r0 < total; r0 += SMDP_WAVE) {
+    const int i = r0 + lane_id();
+    int vv = 0;
+    bool won = false;
+    if (i < total) {
+      vv = s_pack[i];
+      won = atomicSub(&a.pend[vv], 1) == 1;
+      if (won) a.color[vv] = newc;
+    }
+    pushes += out.push(won, vv, qout, qlen);
+  }
+  // s_pack is written again by the next step
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  return pushes;
 }
 
 // Shape of round r's work, from its queue length L and the grid (the expand
@@ -590,6 +613,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_expand(LArgs a, int r) {
   __shared__ int s_tile;
   __shared__ int s_scan[SMDP_WAVES_PER_WG];
   __shared__ int32_t s_out[SMDP_WAVES_PER_WG * kCOut];
+  __shared__ int32_t s_pack[SMDP_WAVES_PER_WG * kLayerIlp * SMDP_WAVE];  // notify_packed
   WaveOut<kCOut> out;
   out.init(s_out + (threadIdx.x / SMDP_WAVE) * kCOut);
   unsigned long long edges = 0, verts = 0, pushes = 0, pg = 0;
@@ -632,7 +656,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_expand(LArgs a, int r) {
     for (int jb = a0; jb < a1; jb += kLayerIlp * SMDP_WG) {
       int j[kLayerIlp], own[kLayerIlp], v[kLayerIlp], uu[kLayerIlp];
       uint32_t pu[kLayerIlp];
-      bool ok[kLayerIlp], won[kLayerIlp];
+      bool ok[kLayerIlp];
 #pragma unroll
       for (int t = 0; t < kLayerIlp; ++t) j[t] = min(jb + (int)threadIdx.x + t * SMDP_WG, a1 - 1);
       tile_owners<kLayerIlp, kShortL>(s_of, j, own);
@@ -650,9 +674,9 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_expand(LArgs a, int r) {
 #pragma unroll
         for (int t = 0; t < kLayerIlp; ++t) pu[t] = 0u;
       }
-      notify<kLayerIlp>(a, uu, pu, v, ok, newc, won);
-#pragma unroll
-      for (int t = 0; t < kLayerIlp; ++t) pushes += out.push(won[t], v[t], qout, &cout->qlen);
+      pushes += notify_packed<kLayerIlp>(a, uu, pu, v, ok, newc,
+                                         s_pack + (threadIdx.x / SMDP_WAVE) * kLayerIlp * SMDP_WAVE, out, qout,
+                                         &cout->qlen);
     }
   } else {
   const LayerShape sh = layer_shape(a, L, a.grid);  // launched with a.grid workgroups
@@ -713,7 +737,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_expand(LArgs a, int r) {
     for (int jb = 0; jb < tot; jb += kLayerIlp * SMDP_WG) {
       int j[kLayerIlp], own[kLayerIlp], v[kLayerIlp], uu[kLayerIlp];
       uint32_t pu[kLayerIlp];
-      bool ok[kLayerIlp], won[kLayerIlp];
+      bool ok[kLayerIlp];
 #pragma unroll
       for (int t = 0; t < kLayerIlp; ++t) j[t] = min(jb + (int)threadIdx.x + t * SMDP_WG, tot - 1);
       tile_owners<kLayerIlp>(s_off, j, own);
@@ -725,9 +749,9 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_expand(LArgs a, int r) {
         ok[t] = jb + (int)threadIdx.x + t * SMDP_WG < tot && v[t] != uu[t];
         edges += jb + (int)threadIdx.x + t * SMDP_WG < tot;
       }
-      notify<kLayerIlp>(a, uu, pu, v, ok, newc, won);
-#pragma unroll
-      for (int t = 0; t < kLayerIlp; ++t) pushes += out.push(won[t], v[t], qout, &cout->qlen);
+      pushes += notify_packed<kLayerIlp>(a, uu, pu, v, ok, newc,
+                                         s_pack + (threadIdx.x / SMDP_WAVE) * kLayerIlp * SMDP_WAVE, out, qout,
+                                         &cout->qlen);
     }
     __syncthreads();
   }
@@ -756,6 +780,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_drain(LArgs a, int r) {
   const int newc = r + 2;
   __shared__ int s_idx;
   __shared__ int32_t s_out[SMDP_WAVES_PER_WG * kCOut];
+  __shared__ int32_t s_pack[SMDP_WAVES_PER_WG * kLayerIlp * SMDP_WAVE];  // notify_packed
   WaveOut<kCOut> out;
   out.init(s_out + (threadIdx.x / SMDP_WAVE) * kCOut);
   unsigned long long edges = 0, pushes = 0;
@@ -775,7 +800,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_drain(LArgs a, int r) {
     for (int b = ch.first; b < e; b += kLayerIlp * SMDP_WG) {
       int v[kLayerIlp], uu[kLayerIlp];
       uint32_t pu[kLayerIlp];
-      bool ok[kLayerIlp], won[kLayerIlp];
+      bool ok[kLayerIlp];
 #pragma unroll
       for (int t = 0; t < kLayerIlp; ++t) {
         const int k = b + (int)threadIdx.x + t * SMDP_WG;
@@ -785,9 +810,9 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_drain(LArgs a, int r) {
         ok[t] = k < e && v[t] != ch.v;
         edges += k < e;
       }
-      notify<kLayerIlp>(a, uu, pu, v, ok, newc, won);
-#pragma unroll
-      for (int t = 0; t < kLayerIlp; ++t) pushes += out.push(won[t], v[t], qout, &cout->qlen);
+      pushes += notify_packed<kLayerIlp>(a, uu, pu, v, ok, newc,
+                                         s_pack + (threadIdx.x / SMDP_WAVE) * kLayerIlp * SMDP_WAVE, out, qout,
+                                         &cout->qlen);
     }
   }
   out.flush(qout, &cout->qlen);
