@@ -259,7 +259,13 @@ __global__ __launch_bounds__(NT) void k_mg(GArgs a) {
   __shared__ int s_off[NT];
   __shared__ int s_scan[W];
   __shared__ int32_t s_out[W * kMgOut];
+  // per wave: the step's candidate (vertex, distance) pairs, packed to the
+  // low lanes so that their atomics run one per lane instead of one slot
+  // after another (each conditional atomic waits for its result)
+  __shared__ int32_t s_pair[W * 2 * kMgIlp * SMDP_WAVE];
   const int wave = threadIdx.x / SMDP_WAVE;
+  const int lane = lane_id();
+  int32_t* pair = s_pair + wave * 2 * kMgIlp * SMDP_WAVE;
   unsigned long long edges = 0, verts = 0;
   for (;;) {
     if (threadIdx.x == 0) {
@@ -331,17 +337,40 @@ __global__ __launch_bounds__(NT) void k_mg(GArgs a) {
           for (int t = 0; t < kMgIlp; ++t) alt[t] = s_du[own[t]] + stream_load(a.w + k[t]);
 #pragma unroll
           for (int t = 0; t < kMgIlp; ++t) seen[t] = dist[v[t]];
+          int total = 0;
 #pragma unroll
           for (int t = 0; t < kMgIlp; ++t) {
             const bool ok = jb + (int)threadIdx.x + t * NT < tot;
-            bool p = false;
-            if (ok && alt[t] < seen[t]) {
-              const int old = atomicMin(&dist[v[t]], alt[t]);
-              if (alt[t] < old && stamp[v[t]] != label) p = atomicExch(&stamp[v[t]], label) != label;
-            }
             edges += ok;
-            out.push(p, v[t], qout, &s_qlen[cur ^ 1]);
+            const bool want = ok && alt[t] < seen[t];
+            const uint64_t m = ballot(want);
+            if (want) {
+              const int i = total + mbcnt(m);
+              pair[2 * i] = v[t];
+              pair[2 * i + 1] = alt[t];
+            }
+            total += __popcll(m);
           }
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          for (int r0 = 0; r0 < total; r0 += SMDP_WAVE) {
+            const int i = r0 + lane;
+            int vv = 0;
+            bool p = false;
+            if (i < total) {
+              vv = pair[2 * i];
+              const int aa = pair[2 * i + 1];
+              const int old = atomicMin(&dist[vv], aa);
+              const int st = stamp[vv];  // a filter read while the atomic is in flight
+              if (aa < old && st != label) p = atomicExch(&stamp[vv], label) != label;
+            }
+            out.push(p, vv, qout, &s_qlen[cur ^ 1]);
+          }
+          // pair is written again by the next step
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
         __syncthreads();
       }
