@@ -145,6 +145,14 @@ int cli_apply_tuning(const CliConf& c, smdp_ctx* ctx, const smdp_csr* csr, int a
   return 0;
 }
 
+void cli_print_counters(FILE* f, const smdp_stats& st) {
+  fprintf(f, "Heavy work queued: wave %llu  workgroup %llu  grid %llu  (next-frontier pushes %llu)\n",
+          (unsigned long long)st.q_push[1], (unsigned long long)st.q_push[2], (unsigned long long)st.q_push[3],
+          (unsigned long long)st.q_push[0]);
+  fprintf(f, "Kernel launches: %llu  atomics tried: %llu  won: %llu\n", (unsigned long long)st.kernel_launches,
+          (unsigned long long)st.atomics_tried, (unsigned long long)st.atomics_won);
+}
+
 void output_cost(FILE* f, int32_t source, int32_t n, const int32_t* cost) {
   for (int32_t i = 0; i < n; ++i) fprintf(f, "From Node %d -> %d is %d\n", source + 1, i + 1, cost[i]);
 }

@@ -41,6 +41,11 @@ int cli_apply_tuning(const CliConf& c, smdp_ctx* ctx, const smdp_csr* csr, int a
 
 int cli_fail(const char* what, int rc);
 double cli_now();
+// Prints the work counters of a call (-v): the heavy vertices (or rows) handed
+// to each queue level, which stand where the reference's GPU_PROFILE build
+// counts nested kernel calls (apps/sssp/src/sssp_kernel.cu:14-25), plus the
+// kernels launched and, for SSSP/BFS, the atomics issued and won.
+void cli_print_counters(FILE* f, const smdp_stats& st);
 
 // Reference text outputs.
 void output_cost(FILE* f, int32_t source, int32_t n, const int32_t* cost);  // global.cpp:369-375

@@ -62,6 +62,7 @@ int main(int argc, char** argv) {
     fprintf(stdout, "HIP kernel execution:\t\t%lf\n", st.ms_kernel / 1e3);
     fprintf(stdout, "HIP total (malloc + H2D + kernels + D2H):\t\t%lf\n", st.ms_total / 1e3);
     fprintf(stdout, "L-inf error vs FP64: %.3e (max |y| %.3e)\n", linf, ymax);
+    cli_print_counters(stdout, st);
   }
   if (c.debug) {
     for (int32_t i = 0; i < g->n; ++i) {

@@ -61,6 +61,7 @@ int main(int argc, char** argv) {
     fprintf(stdout, "===MAIN=== :: HIP total (malloc + H2D + kernels + D2H):\t\t%.2lf ms.\n", st.ms_total);
     fprintf(stdout, "===MAIN=== :: levels: %llu  GTEPS: %.3f\n", (unsigned long long)st.rounds,
             st.ms_kernel > 0 ? st.m_reached / (st.ms_kernel * 1e6) : 0.0);
+    cli_print_counters(stdout, st);
   }
   if (c.debug) output_level(stdout, g->n, level);
   free(level);

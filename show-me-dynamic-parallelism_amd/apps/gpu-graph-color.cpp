@@ -53,6 +53,7 @@ int main(int argc, char** argv) {
   if (c.verbose) {
     fprintf(stdout, "HIP kernel execution:\t\t%lf\n", st.ms_kernel / 1e3);
     fprintf(stdout, "HIP total (malloc + H2D + kernels + D2H):\t\t%lf\n", st.ms_total / 1e3);
+    cli_print_counters(stdout, st);
   }
   if (c.debug) {
     fprintf(stderr, "Graph Coloring ends in %llu iterations.\n", (unsigned long long)st.rounds);

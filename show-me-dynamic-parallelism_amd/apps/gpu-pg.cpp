@@ -65,6 +65,7 @@ int main(int argc, char** argv) {
     fprintf(stdout, "HIP kernel execution:\t\t%lf\n", st.ms_kernel / 1e3);
     fprintf(stdout, "HIP total (malloc + H2D + kernels + D2H):\t\t%lf\n", st.ms_total / 1e3);
     fprintf(stdout, "L1 residual: %.3e  dangling mass: %.6e\n", st.residual, st.dangling);
+    cli_print_counters(stdout, st);
   }
   if (c.debug) {
     fprintf(stderr, "PageRank ends in %llu iterations.\n", (unsigned long long)st.rounds);

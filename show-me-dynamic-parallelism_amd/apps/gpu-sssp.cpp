@@ -54,6 +54,7 @@ int main(int argc, char** argv) {
     fprintf(stdout, "Rounds: %llu  arcs scanned: %llu  frontier entries: %llu\n", (unsigned long long)st.rounds,
             (unsigned long long)st.edges_relaxed, (unsigned long long)st.vertices_expanded);
     fprintf(stdout, "GTEPS: %.3f\n", st.ms_kernel > 0 ? st.m_reached / (st.ms_kernel * 1e6) : 0.0);
+    cli_print_counters(stdout, st);
   }
   if (c.debug) {
     fprintf(stderr, "Iteration: %llu\n", (unsigned long long)st.rounds);

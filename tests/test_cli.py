@@ -74,3 +74,20 @@ def test_gpu_spmv_and_pagerank_cli(data_test_path):
     r = _gpu_run("gpu-pg", data_test_path, "-s", "6", "-d")
     ranks = np.array([float(x) for x in re.findall(r"Node \d+: (\S+)", r.stdout)])
     assert len(ranks) == 20 and abs(ranks.sum() - 1000.0) < 1e-2
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sol", ["2", "3", "4"])
+def test_gpu_sssp_cli_counters(golden, data_test_path, sol):
+    """-v prints the heavy-work queue counters (the reference's GPU_PROFILE
+    nested-call count, apps/sssp/src/sssp_kernel.cu:14-25); a heavy degree of
+    1 sends every expanded vertex with arcs through the queues."""
+    r = _gpu_run("gpu-sssp", data_test_path, "-s", sol, "-v", "-d", "--threshold", "1")
+    got = re.findall(r"From Node (\d+) -> (\d+) is (\d+)", r.stdout)
+    assert [int(c) for _, _, c in got] == golden["dist"]
+    m = re.search(r"Heavy work queued: wave (\d+)  workgroup (\d+)  grid (\d+)  \(next-frontier pushes (\d+)\)",
+                  r.stdout)
+    assert m, r.stdout
+    assert sum(int(x) for x in m.groups()[:3]) > 0
+    k = re.search(r"Kernel launches: (\d+)  atomics tried: (\d+)  won: (\d+)", r.stdout)
+    assert k and int(k.group(1)) > 0 and int(k.group(3)) <= int(k.group(2))
