@@ -378,9 +378,16 @@ struct LArgs {
   int32_t split;  // rows of at least this many arcs go to grid chunks
   int32_t chunk;  // arcs per grid chunk
   int32_t grid;   // workgroups of k_layer_expand (the round's tile shape depends on it)
+  // Layer indices kept on the device, so that a launch can run several layers:
+  // rd[0] the layer k_layer_expand / k_layer_drain run next, rd[1] the layer
+  // the last drain ran, rd[2] set once a layer is empty (done), rd[3] set
+  // when a grid barrier gave up waiting.
+  int32_t* rd;
+  uint32_t* bar;  // grid barrier of k_layer_short: kBarLines lines of 32 words
 };
 
 constexpr int kLayerIlp = 4;
+constexpr int kBarLines = 10;    // grid barrier of k_layer_short: 8 arrival lines, a total, a release
 constexpr int kLayerCountSlot = 3;  // control slot of the counting chunks (round 1 clears it)
 
 // (prio[u], u) < (prio[v], v) with pu = prio[u] already loaded.
@@ -390,7 +397,8 @@ __device__ __forceinline__ bool lprec(const LArgs& a, int u, uint32_t pu, int v)
   return pu < pv || (pu == pv && u < v);
 }
 
-__global__ void k_layer_init(Ctl* ctl, Counters* cnt) {
+__global__ void k_layer_init(Ctl* ctl, Counters* cnt, int32_t* rd) {
+  if (threadIdx.x < 4) rd[threadIdx.x] = 0;
   int32_t* c = reinterpret_cast<int32_t*>(ctl);
   for (int i = threadIdx.x; i < 4 * kCtlInts; i += blockDim.x) c[i] = 0;
   unsigned long long* k = reinterpret_cast<unsigned long long*>(cnt);
@@ -595,15 +603,90 @@ constexpr int kShortL = 2048;
 constexpr int kShortPerWG = 8;
 constexpr int kShortMinWG = 128;
 
+// Expands short layer r (L <= kShortL queued vertices) on workgroups
+// 0 .. G-1 of the launch, G = min(workgroups, max(L / kShortPerWG,
+// kShortMinWG)); the others return at once.  s_raw: 3 * kShortL ints.
+__device__ __forceinline__ void short_layer(const LArgs& a, int r, int L, int32_t* s_raw, int* s_scan,
+                                            int32_t* s_pack, WaveOut<kCOut>& out, unsigned long long& edges,
+                                            unsigned long long& verts, unsigned long long& pushes) {
+  const int G = min((int)gridDim.x, max((L + kShortPerWG - 1) / kShortPerWG, kShortMinWG));
+  if ((int)blockIdx.x >= G) return;
+  Ctl* cout = a.ctl + ((r + 1) & 3);
+  const int32_t* qin = (r & 1) ? a.q1 : a.q0;
+  int32_t* qout = (r & 1) ? a.q0 : a.q1;
+  const int newc = r + 2;
+  int* s_v = s_raw;
+  int* s_st = s_raw + kShortL;
+  int* s_of = s_raw + 2 * kShortL;
+  constexpr int E = kShortL / SMDP_WG;  // consecutive entries staged per thread
+  int vv[E], st[E], en[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) vv[k] = qin[min((int)threadIdx.x * E + k, L - 1)];
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    st[k] = a.trow[vv[k]];
+    en[k] = a.trow[vv[k] + 1];
+  }
+  int sum = 0;
+#pragma unroll
+  for (int k = 0; k < E; ++k) sum += (int)threadIdx.x * E + k < L ? en[k] - st[k] : 0;
+  int A;
+  int run = wg_exclusive_scan(sum, s_scan, &A);
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int i = (int)threadIdx.x * E + k;
+    if (i < L) {
+      s_v[i] = vv[k];
+      s_st[i] = st[k];
+      s_of[i] = run;
+      run += en[k] - st[k];
+    } else {
+      s_of[i] = A;  // never the owner of an arc j < A
+    }
+  }
+  __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x == 0) verts += L;
+  const int a0 = (int)((int64_t)A * blockIdx.x / G), a1 = (int)((int64_t)A * (blockIdx.x + 1) / G);
+  for (int jb = a0; jb < a1; jb += kLayerIlp * SMDP_WG) {
+    int j[kLayerIlp], own[kLayerIlp], v[kLayerIlp], uu[kLayerIlp];
+    uint32_t pu[kLayerIlp];
+    bool ok[kLayerIlp];
+#pragma unroll
+    for (int t = 0; t < kLayerIlp; ++t) j[t] = min(jb + (int)threadIdx.x + t * SMDP_WG, a1 - 1);
+    tile_owners<kLayerIlp, kShortL>(s_of, j, own);
+#pragma unroll
+    for (int t = 0; t < kLayerIlp; ++t) {
+      uu[t] = s_v[own[t]];
+      v[t] = stream_load(a.tcol + s_st[own[t]] + (j[t] - s_of[own[t]]));
+      ok[t] = jb + (int)threadIdx.x + t * SMDP_WG < a1 && v[t] != uu[t];
+      edges += jb + (int)threadIdx.x + t * SMDP_WG < a1;
+    }
+    if (a.prio) {
+#pragma unroll
+      for (int t = 0; t < kLayerIlp; ++t) pu[t] = a.prio[uu[t]];
+    } else {
+#pragma unroll
+      for (int t = 0; t < kLayerIlp; ++t) pu[t] = 0u;
+    }
+    pushes += notify_packed<kLayerIlp>(a, uu, pu, v, ok, newc,
+                                       s_pack + (threadIdx.x / SMDP_WAVE) * kLayerIlp * SMDP_WAVE, out, qout,
+                                       &cout->qlen);
+  }
+}
+
 // Round r: the queue holds the vertices of colour r+1.  Short layers as
 // above; longer ones in edge-balanced tiles of queue entries (layer_shape),
 // with transposed rows of >= split arcs cut into grid chunks.
-__global__ __launch_bounds__(SMDP_WG) void k_layer_expand(LArgs a, int r) {
+__global__ __launch_bounds__(SMDP_WG) void k_layer_expand(LArgs a) {
+  const int r = a.rd[0];  // written by the kernels before this one
   ctl_clear(a.ctl, (r + 2) & 3);
   Ctl* cin = a.ctl + (r & 3);
   Ctl* cout = a.ctl + ((r + 1) & 3);
   const int L = cin->qlen;
-  if (L == 0) return;
+  if (L == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.rd[2] = 1;
+    return;
+  }
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&cnt_of(a.cnt)->rounds, (unsigned long long)(r + 1));
   const int32_t* qin = (r & 1) ? a.q1 : a.q0;
   int32_t* qout = (r & 1) ? a.q0 : a.q1;
@@ -619,65 +702,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_expand(LArgs a, int r) {
   unsigned long long edges = 0, verts = 0, pushes = 0, pg = 0;
 
   if (L <= kShortL) {
-    const int G = min((int)gridDim.x, max((L + kShortPerWG - 1) / kShortPerWG, kShortMinWG));
-    if ((int)blockIdx.x >= G) return;
-    int* s_v = s_raw;
-    int* s_st = s_raw + kShortL;
-    int* s_of = s_raw + 2 * kShortL;
-    constexpr int E = kShortL / SMDP_WG;  // consecutive entries staged per thread
-    int vv[E], st[E], en[E];
-#pragma unroll
-    for (int k = 0; k < E; ++k) vv[k] = qin[min((int)threadIdx.x * E + k, L - 1)];
-#pragma unroll
-    for (int k = 0; k < E; ++k) {
-      st[k] = a.trow[vv[k]];
-      en[k] = a.trow[vv[k] + 1];
-    }
-    int sum = 0;
-#pragma unroll
-    for (int k = 0; k < E; ++k) sum += (int)threadIdx.x * E + k < L ? en[k] - st[k] : 0;
-    int A;
-    int run = wg_exclusive_scan(sum, s_scan, &A);
-#pragma unroll
-    for (int k = 0; k < E; ++k) {
-      const int i = (int)threadIdx.x * E + k;
-      if (i < L) {
-        s_v[i] = vv[k];
-        s_st[i] = st[k];
-        s_of[i] = run;
-        run += en[k] - st[k];
-      } else {
-        s_of[i] = A;  // never the owner of an arc j < A
-      }
-    }
-    __syncthreads();
-    if (blockIdx.x == 0 && threadIdx.x == 0) verts = L;
-    const int a0 = (int)((int64_t)A * blockIdx.x / G), a1 = (int)((int64_t)A * (blockIdx.x + 1) / G);
-    for (int jb = a0; jb < a1; jb += kLayerIlp * SMDP_WG) {
-      int j[kLayerIlp], own[kLayerIlp], v[kLayerIlp], uu[kLayerIlp];
-      uint32_t pu[kLayerIlp];
-      bool ok[kLayerIlp];
-#pragma unroll
-      for (int t = 0; t < kLayerIlp; ++t) j[t] = min(jb + (int)threadIdx.x + t * SMDP_WG, a1 - 1);
-      tile_owners<kLayerIlp, kShortL>(s_of, j, own);
-#pragma unroll
-      for (int t = 0; t < kLayerIlp; ++t) {
-        uu[t] = s_v[own[t]];
-        v[t] = stream_load(a.tcol + s_st[own[t]] + (j[t] - s_of[own[t]]));
-        ok[t] = jb + (int)threadIdx.x + t * SMDP_WG < a1 && v[t] != uu[t];
-        edges += jb + (int)threadIdx.x + t * SMDP_WG < a1;
-      }
-      if (a.prio) {
-#pragma unroll
-        for (int t = 0; t < kLayerIlp; ++t) pu[t] = a.prio[uu[t]];
-      } else {
-#pragma unroll
-        for (int t = 0; t < kLayerIlp; ++t) pu[t] = 0u;
-      }
-      pushes += notify_packed<kLayerIlp>(a, uu, pu, v, ok, newc,
-                                         s_pack + (threadIdx.x / SMDP_WAVE) * kLayerIlp * SMDP_WAVE, out, qout,
-                                         &cout->qlen);
-    }
+    short_layer(a, r, L, s_raw, s_scan, s_pack, out, edges, verts, pushes);
   } else {
   const LayerShape sh = layer_shape(a, L, a.grid);  // launched with a.grid workgroups
   const int ntiles = (L + sh.tile - 1) / sh.tile;
@@ -769,10 +794,97 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_expand(LArgs a, int r) {
   }
 }
 
+// Grid barrier of k_layer_short, whose workgroups are all resident
+// (cooperative launch).  Workgroup b arrives on line b % 8 (about one eighth
+// of the workgroups per line: arrivals on one address run one at a time);
+// the last arrival of a line arrives on line 8, and the last of those
+// publishes the epoch on line 9, which everyone polls.  Counts only grow
+// (epoch e is complete at e arrivals per member), and the lines are zeroed
+// before each launch.  Gives up after about a second of polling: returns
+// false in every workgroup that gave up.
+constexpr int kBarSpinLimit = 1 << 21;
+__device__ __forceinline__ bool grid_barrier(uint32_t* bar, uint32_t epoch, int* s_ok) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t nwg = gridDim.x;
+    const uint32_t grp = blockIdx.x & 7;
+    const uint32_t members = (nwg - grp + 7) / 8;  // workgroups g < nwg with g % 8 == grp
+    const uint32_t ngroups = nwg < 8 ? nwg : 8;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this workgroup's writes (made before the __syncthreads)
+    if (atomicAdd(&bar[32 * grp], 1u) + 1 == epoch * members &&
+        atomicAdd(&bar[32 * 8], 1u) + 1 == epoch * ngroups)
+      __hip_atomic_store(&bar[32 * 9], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    // relaxed polls (an acquire load invalidates the caches at every poll,
+    // which slowed every workgroup of the XCD), one acquire fence after
+    int ok = 1;
+    for (int spins = 0; __hip_atomic_load(&bar[32 * 9], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch;) {
+      __builtin_amdgcn_s_sleep(2);
+      if (++spins > kBarSpinLimit) {
+        ok = 0;
+        break;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // everyone's writes, for the whole workgroup
+    *s_ok = ok;
+  }
+  __syncthreads();
+  return *s_ok != 0;
+}
+
+// Runs short layers one after another in one launch: layers rd[1] + 1,
+// rd[1] + 2, ... while they hold at most kShortL vertices, with a grid
+// barrier after each (a layer's queue is complete only when every workgroup
+// has pushed).  Then rd[0] = the first layer not run, for k_layer_expand.
+// RMAT-23 has ~2300 such layers; one expand and one drain launch per layer
+// cost about 18 us, mostly launch and ramp.
+__global__ __launch_bounds__(SMDP_WG) void k_layer_short(LArgs a) {
+  __shared__ int32_t s_raw[3 * kShortL];
+  __shared__ int s_scan[SMDP_WAVES_PER_WG];
+  __shared__ int32_t s_out[SMDP_WAVES_PER_WG * kCOut];
+  __shared__ int32_t s_pack[SMDP_WAVES_PER_WG * kLayerIlp * SMDP_WAVE];  // notify_packed
+  __shared__ int s_ok;
+  const int r0 = a.rd[1];
+  if (a.ctl[r0 & 3].qlen == 0) return;  // the colouring is complete (every workgroup sees it)
+  WaveOut<kCOut> out;
+  out.init(s_out + (threadIdx.x / SMDP_WAVE) * kCOut);
+  unsigned long long edges = 0, verts = 0, pushes = 0;
+  int r = r0 + 1;
+  for (uint32_t epoch = 1;; ++epoch) {
+    // read after the previous barrier: the same in every workgroup
+    const int L = a.ctl[r & 3].qlen;
+    if (L == 0 || L > kShortL) break;
+    ctl_clear(a.ctl, (r + 2) & 3);  // workgroup 0; no one uses that slot until layer r + 1
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&cnt_of(a.cnt)->rounds, (unsigned long long)(r + 1));
+    short_layer(a, r, L, s_raw, s_scan, s_pack, out, edges, verts, pushes);
+    out.flush((r & 1) ? a.q0 : a.q1, &a.ctl[(r + 1) & 3].qlen);
+    if (!grid_barrier(a.bar, epoch, &s_ok)) {
+      if (threadIdx.x == 0) a.rd[3] = 1;
+      break;
+    }
+    ++r;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.rd[0] = r;
+  edges = wave_sum(edges);
+  verts = wave_sum(verts);
+  pushes = wave_sum(pushes);
+  if (lane_id() == 0) {
+    if (edges) atomicAdd(&cnt_of(a.cnt)->edges, edges);
+    if (verts) atomicAdd(&cnt_of(a.cnt)->verts, verts);
+    if (pushes) atomicAdd(&cnt_of(a.cnt)->push[0], pushes);
+  }
+}
+
 // Grid chunks of round r: a workgroup takes one chunk of a transposed row.
-__global__ __launch_bounds__(SMDP_WG) void k_layer_drain(LArgs a, int r) {
+__global__ __launch_bounds__(SMDP_WG) void k_layer_drain(LArgs a) {
+  const int r = a.rd[0];
   Ctl* cin = a.ctl + (r & 3);
   Ctl* cout = a.ctl + ((r + 1) & 3);
+  if (blockIdx.x == 0) {
+    // for k_layer_short, which runs next: the layer done here, and fresh
+    // barrier lines (no other kernel is using them now)
+    if (threadIdx.x == 0) a.rd[1] = r;
+    for (int i = threadIdx.x; i < kBarLines * 32; i += blockDim.x) a.bar[i] = 0;
+  }
   const int H = cin->hlen;
   if ((int)blockIdx.x >= H) return;
   const int chunk = layer_shape(a, cin->qlen, a.grid).chunk;
@@ -857,6 +969,8 @@ struct ColorBufs {
   int32_t* pend = nullptr;   // tuned: preceding arcs not yet coloured
   int32_t* trow = nullptr;   // tuned: transposed CSR, built once per graph
   int32_t* tcol = nullptr;
+  int32_t* rd = nullptr;     // tuned: device layer indices (LArgs::rd)
+  uint32_t* bar = nullptr;   // tuned: grid barrier lines of k_layer_short
 };
 }  // namespace
 
@@ -870,6 +984,8 @@ void colorbufs_free(smdp_graph* g) {
   if (b->pend) (void)hipFree(b->pend);
   if (b->trow) (void)hipFree(b->trow);
   if (b->tcol) (void)hipFree(b->tcol);
+  if (b->rd) (void)hipFree(b->rd);
+  if (b->bar) (void)hipFree(b->bar);
   delete b;
   g->colorbufs = nullptr;
 }
@@ -917,6 +1033,8 @@ static int run_color_layers(smdp_graph* g, int use_prio, smdp_stats* st) {
   SMDP_CHECK(ensure(&g->hq, &g->hq_cap,
                     n1 + (int64_t)g->m / std::max(1, std::min(p.chunk, kLayerShortChunk)) + 64));
   SMDP_CHECK(ensure_transpose(g, b));
+  SMDP_CHECK(alloc_once(&b->rd, 4));
+  SMDP_CHECK(alloc_once(&b->bar, kBarLines * 32));
   LArgs a;
   a.row = g->row;
   a.col = g->col;
@@ -937,34 +1055,54 @@ static int run_color_layers(smdp_graph* g, int use_prio, smdp_stats* st) {
   const int grid = grid_size(ctx, p);
   const int drain_grid = std::max(1, std::min(grid, ctx->cus));
   a.grid = grid;
+  a.rd = b->rd;
+  a.bar = reinterpret_cast<uint32_t*>(b->bar);
+  // k_layer_short: all workgroups resident (cooperative launch).  Its grid
+  // barrier costs more per workgroup than a short layer gains from more of
+  // them: on RMAT-23, 256/128/64/32/16/8 workgroups took 48.6/43.0/39.1/
+  // 42.6/56.0/83.8 ms per colouring, so one per four CUs.
+  int per_cu = 0;
+  SMDP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_layer_short),
+                                                          SMDP_WG, 0));
+  if (per_cu < 1) return SMDP_ERR_DEVICE;
+  const int short_grid = std::max(8, ctx->cus / 4);
+  void* short_args[] = {&a};
   hipStream_t s = ctx->stream;
   prof_reset(ctx);
   int launches = 0;
   SMDP_HIP(hipEventRecord(ctx->ev_begin, s));
-  hipLaunchKernelGGL(k_layer_init, dim3(1), dim3(256), 0, s, g->ctl, g->cnt);
+  hipLaunchKernelGGL(k_layer_init, dim3(1), dim3(256), 0, s, g->ctl, g->cnt, a.rd);
   hipLaunchKernelGGL(k_layer_count, dim3(grid), dim3(SMDP_WG), 0, s, a);
   hipLaunchKernelGGL(k_layer_count_chunks, dim3(grid), dim3(SMDP_WG), 0, s, a);
   hipLaunchKernelGGL(k_layer_seed, dim3(std::min<int64_t>((n1 + SMDP_WG - 1) / SMDP_WG, grid)), dim3(SMDP_WG), 0, s, a);
   launches += 4;
-  int r = 0;
+  // A step is one layer on the whole grid (expand, then the drain of its
+  // chunks) followed by the run of short layers after it in one launch.  The
+  // layer indices live on the device (rd), so steps are enqueued in batches
+  // and the host only checks rd between batches.
   int batch = std::max(1, p.batch_rounds);
   for (;;) {
-    for (int j = 0; j < batch; ++j, ++r) {
-      prof_mark(ctx, 0);
-      hipLaunchKernelGGL(k_layer_expand, dim3(grid), dim3(SMDP_WG), 0, s, a, r);
-      prof_mark(ctx, 0);
-      prof_mark(ctx, 1);
+    for (int j = 0; j < batch; ++j) {
+      SMDP_CHECK(prof_mark(ctx, 0));
+      hipLaunchKernelGGL(k_layer_expand, dim3(grid), dim3(SMDP_WG), 0, s, a);
+      SMDP_CHECK(prof_mark(ctx, 0));
+      SMDP_CHECK(prof_mark(ctx, 1));
       // one workgroup per CU claims the round's chunks: most rounds have none,
       // and an empty launch of the full grid cost 4.3 us per round
-      hipLaunchKernelGGL(k_layer_drain, dim3(drain_grid), dim3(SMDP_WG), 0, s, a, r);
-      prof_mark(ctx, 1);
-      launches += 2;
+      hipLaunchKernelGGL(k_layer_drain, dim3(drain_grid), dim3(SMDP_WG), 0, s, a);
+      SMDP_CHECK(prof_mark(ctx, 1));
+      SMDP_CHECK(prof_mark(ctx, 0));
+      SMDP_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_layer_short), dim3(short_grid),
+                                          dim3(SMDP_WG), short_args, 0, s));
+      SMDP_CHECK(prof_mark(ctx, 0));
+      launches += 3;
     }
     SMDP_HIP(hipGetLastError());
-    SMDP_HIP(hipMemcpyAsync(ctx->pinned, &g->ctl[r & 3].qlen, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    SMDP_HIP(hipMemcpyAsync(ctx->pinned, a.rd, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     SMDP_HIP(hipStreamSynchronize(s));
-    if (ctx->pinned[0] == 0) break;
-    if (r >= p.max_rounds || r >= g->n + 1) return SMDP_ERR_OVERFLOW;
+    if (ctx->pinned[3]) return SMDP_ERR_DEVICE;  // a grid barrier gave up waiting
+    if (ctx->pinned[2]) break;
+    if (ctx->pinned[0] >= p.max_rounds || ctx->pinned[0] >= g->n + 1) return SMDP_ERR_OVERFLOW;
     batch = std::min(batch * 2, 64);
   }
   SMDP_HIP(hipEventRecord(ctx->ev_end, s));
