@@ -347,6 +347,28 @@ def bfs_section(ctx, smdp, orc, np):
                               "orc_bfs_fifo) / level-synchronous (orc_bfs_par)")}
 
 
+def tree_section(ctx, smdp, orc, np):
+    """Tree Height / Tree Descendant (named in the reference's README.md:12-13)
+    on the same 4M-node Zipf(1.5) tree: a down pass per level, then an up pass."""
+    t = smdp.gen_zipf_tree(4194304, 1.5, 4096, 5)
+    with smdp.Graph(ctx, t) as dg:
+        ms, st = _median_runs(lambda: dg.tree(0))
+        h = dg.read(smdp.RES_HEIGHT)
+        d = dg.read(smdp.RES_DESC)
+    t0 = time.perf_counter()
+    rh, rd, rth = orc.tree(t.row, t.col, 0)
+    t_cpu = time.perf_counter() - t0
+    # down: 8 B/arc (child id read, slot write) + 12 B/vertex (slot read, two row offsets);
+    # up: 12 B/arc (child id, child height and count) + 20 B/vertex (slot, two offsets, two results)
+    alg = 20 * t.m + 32 * t.n
+    return {"workload": "tree_zipf_4m", "vertices": t.n, "arcs": t.m, "ms_device_median": round(ms, 4),
+            "levels": int(st.rounds), "tree_height": int(rth), "descendants_of_root": int(rd[0]),
+            "exact_vs_oracle": bool(np.array_equal(h, rh) and np.array_equal(d, rd)),
+            "gteps": round(t.m / (ms * 1e6), 3), "roofline": _roof(alg, ms),
+            "cpu": {"ms": round(t_cpu * 1e3, 1), "cores": 1, "kind": "port", "gteps": round(t.m / t_cpu / 1e9, 4),
+                    "sample": "one pass: BFS order from the root, then that order backwards (orc_tree)"}}
+
+
 def stream_copy_bandwidth(nbytes=1 << 29, reps=5):
     """Measured device-to-device copy rate (read + write bytes per second)."""
     import ctypes as C
@@ -456,6 +478,12 @@ def only_section(args, smdp, orc, np):
             for _ in range(args.warmup + args.steps):
                 st = dg.bfs(0, smdp.SOL_TUNED)
             log(f"bfs zipf tree 4M: {st.ms_kernel:.3f} ms, {st.rounds} rounds")
+    elif args.only == "tree":
+        t = smdp.gen_zipf_tree(4194304, 1.5, 4096, 5)
+        with smdp.Graph(ctx, t) as dg:
+            for _ in range(args.warmup + args.steps):
+                st = dg.tree(0)
+            log(f"tree zipf 4M: {st.ms_kernel:.3f} ms, {st.rounds} levels")
     else:
         log(json.dumps(batched_section(ctx, args, smdp, orc, np)))
     ctx.close()
@@ -478,7 +506,7 @@ def main():
                     help="PageRank arc layout: 0 auto, 1 plain CSR, 2 binned, 8 XCD blocks")
     ap.add_argument("--params", default="", help="profiling runs: JSON object of smdp_params fields to set")
     ap.add_argument("--round-log", action="store_true", help="profiling runs: print SSSP per-round log")
-    ap.add_argument("--only", choices=["sssp", "pagerank", "batched", "copy", "spmv", "color", "bfs"],
+    ap.add_argument("--only", choices=["sssp", "pagerank", "batched", "copy", "spmv", "color", "bfs", "tree"],
                     help="profiling runs: one section only, no JSON contract line")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -532,7 +560,8 @@ def main():
         except smdp.SmdpError as e:
             line["batched"] = {"error": str(e)}
     if not args.skip_extra:
-        for name, fn in (("spmv", spmv_section), ("color", color_section), ("bfs", bfs_section)):
+        for name, fn in (("spmv", spmv_section), ("color", color_section), ("bfs", bfs_section),
+                         ("tree", tree_section)):
             try:
                 line[name] = fn(ctx, smdp, orc, np)
                 log(f"{name}: {line[name]}")

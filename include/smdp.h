@@ -189,6 +189,17 @@ int smdp_pagerank_f32(smdp_ctx* ctx, const smdp_csr* in_csr, const int32_t* outd
 int smdp_color(smdp_ctx* ctx, const smdp_csr* g, const uint32_t* prio, int solution,
                int32_t* color, smdp_stats* st);
 
+/* Tree Height and Tree Descendant, the recursive benchmarks that the
+ * reference's README.md:12-13 lists but does not ship.  g holds the child
+ * lists of a tree (w is ignored); a vertex with two or more in-arcs anywhere
+ * in g is SMDP_ERR_ARG, and arcs into the root are ignored.  For every
+ * vertex reached from root: height[v] = 0 for a leaf, else 1 + the largest
+ * child height; desc[v] = the number of its descendants.  Both are -1 for
+ * vertices not reached.  height or desc may be NULL; *tree_height (may be
+ * NULL) = height[root]. */
+int smdp_tree(smdp_ctx* ctx, const smdp_csr* g, int32_t root, int32_t* height, int32_t* desc,
+              int32_t* tree_height, smdp_stats* st);
+
 /* ---- batched SSSP (no counterpart in the reference) ------------------------*/
 /* k sources on one graph in one launch sequence; dist is [k][n] row-major. */
 int smdp_sssp_multi_source(smdp_ctx* ctx, const smdp_csr* g, const int32_t* srcs, int32_t k,
@@ -213,6 +224,8 @@ int  smdp_spmv_run(smdp_graph* gr, int solution, smdp_stats* st);
 int  smdp_pagerank_run(smdp_graph* gr, const smdp_pr_opts* opt, int solution, smdp_stats* st);
 int  smdp_color_run(smdp_graph* gr, int use_prio, int solution, smdp_stats* st);
 int  smdp_sssp_multi_source_run(smdp_graph* gr, const int32_t* srcs, int32_t k, smdp_stats* st);
+/* smdp_tree on a resident graph; results: SMDP_RES_HEIGHT and SMDP_RES_DESC. */
+int  smdp_tree_run(smdp_graph* gr, int32_t root, smdp_stats* st);
 
 /* Copy the last result of the named kind to the host. */
 #define SMDP_RES_DIST   0   /* int32[n]   SSSP */
@@ -221,6 +234,8 @@ int  smdp_sssp_multi_source_run(smdp_graph* gr, const int32_t* srcs, int32_t k, 
 #define SMDP_RES_RANK   3   /* float[n]   PageRank */
 #define SMDP_RES_COLOR  4   /* int32[n]   coloring */
 #define SMDP_RES_MDIST  5   /* int32[k*n] multi-source SSSP, source-major */
+#define SMDP_RES_HEIGHT 6   /* int32[n]   tree: subtree heights */
+#define SMDP_RES_DESC   7   /* int32[n]   tree: descendant counts (same run as SMDP_RES_HEIGHT) */
 int  smdp_graph_read(smdp_graph* gr, int what, void* host_out, int64_t count);
 
 /* Batched small-graph SSSP kept resident: upload once, replay the captured

@@ -234,6 +234,64 @@ int orc_bfs_fifo(int n, const int* row, const int* col, int src, int* level) {
   return 0;
 }
 
+/* ---------------------------------------------------------------- tree ---*/
+
+/* Tree Height and Tree Descendant, the two recursive benchmarks that the
+ * reference's README.md:12-13 names but does not ship.  The input is a tree
+ * given as child lists (row/col) and its root; it is rejected (-3) when some
+ * vertex has two or more in-arcs (counting every arc of the CSR).  Arcs into
+ * the root are ignored (a cycle through the root).  For every vertex reached
+ * from the root: height = 0 for a leaf, else 1 + the largest child height;
+ * desc = the number of its descendants.  Unreached vertices get -1 in both.
+ * Sequential: the vertices in BFS order from the root, then that order
+ * backwards, each summing over its children.  Returns height[root]. */
+int orc_tree(int n, const int* row, const int* col, int root, int* height, int* desc) {
+  if (n <= 0 || root < 0 || root >= n) return -1;
+  int* indeg = (int*)calloc((size_t)n, sizeof(int));
+  int* order = (int*)malloc(sizeof(int) * (size_t)n);
+  if (!indeg || !order) {
+    free(indeg);
+    free(order);
+    return -2;
+  }
+  int bad = 0;
+  for (int e = 0; e < row[n] && !bad; ++e)
+    if (++indeg[col[e]] > 1) bad = 1;
+  if (bad) {
+    free(indeg);
+    free(order);
+    return -3;
+  }
+  for (int i = 0; i < n; ++i) height[i] = desc[i] = -1;
+  int64_t head = 0, tail = 0;
+  order[tail++] = root;
+  height[root] = 0;  /* marks "reached" until the backward pass */
+  while (head < tail) {
+    const int u = order[head++];
+    for (int e = row[u]; e < row[u + 1]; ++e)
+      if (col[e] != root) {  /* in-degree <= 1: every other child is new */
+        height[col[e]] = 0;
+        order[tail++] = col[e];
+      }
+  }
+  for (int64_t i = tail - 1; i >= 0; --i) {
+    const int u = order[i];
+    int h = 0, d = 0;
+    for (int e = row[u]; e < row[u + 1]; ++e) {
+      const int c = col[e];
+      if (c == root) continue;
+      if (1 + height[c] > h) h = 1 + height[c];
+      d += 1 + desc[c];
+    }
+    height[u] = h;
+    desc[u] = d;
+  }
+  const int hr = height[root];
+  free(indeg);
+  free(order);
+  return hr;
+}
+
 /* ------------------------------------------------------------ coloring ---*/
 
 /* u precedes v: (prio[u], u) < (prio[v], v); prio == NULL is the reference's

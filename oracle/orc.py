@@ -43,6 +43,7 @@ def lib():
         L.orc_spmv_f64_par.argtypes = [i, vp, vp, vp, vp, vp]
         L.orc_pagerank_f64_par.argtypes = [i, vp, vp, vp, C.c_double, C.c_double, i, i, vp, C.POINTER(C.c_double)]
         L.orc_bfs_par.argtypes = [i, vp, vp, i, vp]
+        L.orc_tree.argtypes = [i, vp, vp, i, vp, vp]
         L.orc_color_layers_par.argtypes = [i, vp, vp, vp, vp, C.POINTER(C.c_int)]
         L.orc_pagerank_f64.argtypes = [i, vp, vp, vp, C.c_double, C.c_double, i, C.c_double, i, i, vp,
                                        C.POINTER(C.c_int), C.POINTER(C.c_double)]
@@ -99,6 +100,21 @@ def bfs(row, col, src):
     lv = np.zeros(len(row) - 1, dtype=np.int32)
     assert lib().orc_bfs_fifo(len(row) - 1, _p(row), _p(col), src, _p(lv)) == 0
     return lv
+
+
+def tree(row, col, root):
+    """Subtree heights and descendant counts of a rooted tree (child lists);
+    returns (height, desc, tree_height), or raises ValueError when some vertex
+    has two in-arcs."""
+    row, col = _i32(row), _i32(col)
+    n = len(row) - 1
+    h = np.zeros(n, dtype=np.int32)
+    d = np.zeros(n, dtype=np.int32)
+    r = lib().orc_tree(n, _p(row), _p(col), root, _p(h), _p(d))
+    if r == -3:
+        raise ValueError("not a tree: a vertex has two in-arcs")
+    assert r >= 0
+    return h, d, int(r)
 
 
 def color_rounds(row, col, prio=None):

@@ -309,7 +309,7 @@ void smdp_graph_destroy(smdp_graph* g) {
   void* bufs[] = {g->row,    g->col,      g->w,       g->dist,     g->stamp,  g->hasout, g->q[0],  g->q[1],
                   g->hq,     g->ctl,      g->cnt,     g->val,      g->x,      g->y,     g->outdeg,
                   g->rank[0], g->rank[1], g->contrib, g->partials, g->chunk_part, g->prio, g->resume,
-                  g->rmax,   g->mdist};
+                  g->rmax,   g->mdist,    g->tree_lo, g->tree_inner};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   rowbufs_free(g);
@@ -374,6 +374,13 @@ int smdp_graph_read(smdp_graph* g, int what, void* out, int64_t count) {
     case SMDP_RES_RANK:
       src = g->rank[g->rank_cur];
       avail = g->n;
+      break;
+    case SMDP_RES_HEIGHT:
+    case SMDP_RES_DESC:
+      if (g->last_res != SMDP_RES_HEIGHT) return SMDP_ERR_ARG;  // both come from the last tree run
+      src = what == SMDP_RES_HEIGHT ? g->dist : g->stamp;
+      avail = g->n;
+      what = SMDP_RES_HEIGHT;
       break;
     case SMDP_RES_MDIST:
       return SMDP_ERR_ARG;  // multi-source results are read by the multi-source call

@@ -780,7 +780,7 @@ __global__ void k_hasout(const int32_t* __restrict__ row, int32_t n, uint32_t* w
   }
 }
 
-static int build_hasout(smdp_graph* g, const int32_t* row, uint32_t** out) {
+int build_hasout(smdp_graph* g, const int32_t* row, uint32_t** out) {
   if (*out) return SMDP_OK;
   const int64_t nw = std::max<int64_t>(1, ((int64_t)g->n + 31) / 32);
   SMDP_CHECK(alloc_once(out, nw));
@@ -796,7 +796,7 @@ __global__ void k_indeg_at_most_one(const int32_t* __restrict__ col, int64_t m, 
     if (atomicAdd(&cnt[col[e]], 1) >= 1) *multi = 1;
 }
 
-static int ensure_forest(smdp_graph* g) {
+int ensure_forest(smdp_graph* g) {
   if (g->forest >= 0) return SMDP_OK;
   int32_t* cnt = nullptr;
   SMDP_CHECK(alloc_once(&cnt, (int64_t)g->n + 1));

@@ -162,6 +162,8 @@ struct smdp_graph {
   bool transient = false;    // created by a drop-in call for one solve
   int8_t forest = -1;        // every vertex has at most one in-arc: 1 yes, 0 no, -1 not checked yet
   uint32_t* hasout = nullptr; // tuned frontier: bit v set when row v has arcs
+  int32_t* tree_lo = nullptr; // tree: level bookkeeping and slots (tree.hip)
+  int32_t* tree_inner = nullptr; // tree: per vertex, its arcs to vertices that have arcs (tree.hip)
 };
 
 namespace smdp {
@@ -232,6 +234,10 @@ void parallel_ranges(int64_t count, F body) {
 // un-permuting gather writes the first `count` results in caller numbering.
 void relabel_free(smdp_graph* g);
 int relabel_unpermute(smdp_graph* g, int32_t* out_dev, int64_t count);
+// Sets g->forest (1 when no vertex has two in-arcs), checked once per graph.
+int ensure_forest(smdp_graph* g);
+// Builds *out (once): bit v of word v / 32 set when row v of `row` has arcs.
+int build_hasout(smdp_graph* g, const int32_t* row, uint32_t** out);
 int prof_collect(smdp_ctx* ctx, smdp_stats* st);
 
 void rowbufs_free(smdp_graph* g);

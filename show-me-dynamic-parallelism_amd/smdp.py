@@ -20,7 +20,7 @@ SOL_FLAT, SOL_NAIVE, SOL_WAVE, SOL_BLOCK, SOL_GRID, SOL_HIER, SOL_TUNED = range(
 SOLUTIONS = {"flat": 0, "naive": 1, "wave": 2, "block": 3, "grid": 4, "hier": 5, "tuned": 6}
 APP_SSSP, APP_BFS, APP_SPMV, APP_PAGERANK, APP_COLOR, APP_MSSSP = range(6)
 LAYOUT_AUTO, LAYOUT_CSR, LAYOUT_BINNED, LAYOUT_HOT, LAYOUT_XCD8 = 0, 1, 2, 3, 8
-RES_DIST, RES_LEVEL, RES_Y, RES_RANK, RES_COLOR, RES_MDIST = range(6)
+RES_DIST, RES_LEVEL, RES_Y, RES_RANK, RES_COLOR, RES_MDIST, RES_HEIGHT, RES_DESC = range(8)
 FMT_DIMACS9, FMT_DIMACS10, FMT_SLNDC = range(3)
 
 SSSP_INF = 1073741824
@@ -101,6 +101,7 @@ def _declare(L):
         "smdp_round_log": (C.c_int, [vp, vp, vp, vp, i32]),
         "smdp_sssp": (C.c_int, [vp, P(Csr), i32, C.c_int, vp, P(Stats)]),
         "smdp_bfs": (C.c_int, [vp, P(Csr), i32, C.c_int, vp, P(Stats)]),
+        "smdp_tree": (C.c_int, [vp, P(Csr), i32, vp, vp, vp, P(Stats)]),
         "smdp_spmv_f32": (C.c_int, [vp, P(Csr), vp, vp, C.c_int, vp, P(Stats)]),
         "smdp_pagerank_f32": (C.c_int, [vp, P(Csr), vp, P(PrOpts), C.c_int, vp, P(Stats)]),
         "smdp_color": (C.c_int, [vp, P(Csr), vp, C.c_int, vp, P(Stats)]),
@@ -114,6 +115,7 @@ def _declare(L):
         "smdp_graph_set_prio": (C.c_int, [vp, vp]),
         "smdp_sssp_run": (C.c_int, [vp, i32, C.c_int, P(Stats)]),
         "smdp_bfs_run": (C.c_int, [vp, i32, C.c_int, P(Stats)]),
+        "smdp_tree_run": (C.c_int, [vp, i32, P(Stats)]),
         "smdp_spmv_run": (C.c_int, [vp, C.c_int, P(Stats)]),
         "smdp_pagerank_run": (C.c_int, [vp, P(PrOpts), C.c_int, P(Stats)]),
         "smdp_color_run": (C.c_int, [vp, C.c_int, C.c_int, P(Stats)]),
@@ -332,6 +334,16 @@ class Context:
         _check(lib().smdp_bfs(self.h, C.byref(g.csr()), src, sol, _ptr(out), C.byref(st)), "smdp_bfs")
         return out, st
 
+    def tree(self, g, root):
+        """Tree Height / Tree Descendant: (height, desc, tree_height, stats)."""
+        h = np.zeros(g.n, dtype=np.int32)
+        d = np.zeros(g.n, dtype=np.int32)
+        th = C.c_int32()
+        st = Stats()
+        _check(lib().smdp_tree(self.h, C.byref(g.csr()), root, _ptr(h), _ptr(d), C.byref(th), C.byref(st)),
+               "smdp_tree")
+        return h, d, int(th.value), st
+
     def spmv(self, a, val, x, sol):
         val = np.ascontiguousarray(val, dtype=np.float32)
         x = np.ascontiguousarray(x, dtype=np.float32)
@@ -420,6 +432,11 @@ class Graph:
     def bfs(self, src, sol):
         st = Stats()
         _check(lib().smdp_bfs_run(self.h, src, sol, C.byref(st)), "smdp_bfs_run")
+        return st
+
+    def tree(self, root):
+        st = Stats()
+        _check(lib().smdp_tree_run(self.h, root, C.byref(st)), "smdp_tree_run")
         return st
 
     def spmv(self, sol):

@@ -270,3 +270,99 @@ def _invert(row, col):
     r = np.zeros(n + 1, np.int64)
     np.add.at(r, col[o] + 1, 1)
     return np.cumsum(r).astype(np.int32), src[o].astype(np.int32)
+
+
+# ---- Tree Height / Tree Descendant (README.md:12-13 names them; no reference code)
+
+def _tree_csr(parent, order_seed=None):
+    """Child lists of the tree given by parent[] (parent[root] = -1), children
+    of each vertex in a seeded random order."""
+    n = len(parent)
+    kids = [[] for _ in range(n)]
+    for v, p in enumerate(parent):
+        if p >= 0:
+            kids[p].append(v)
+    rng = np.random.default_rng(order_seed)
+    for k in kids:
+        rng.shuffle(k)
+    row = np.zeros(n + 1, np.int32)
+    row[1:] = np.cumsum([len(k) for k in kids])
+    col = np.array([c for k in kids for c in k], np.int32)
+    return row, col
+
+
+def _random_parent(n, seed):
+    """A random tree on a shuffled id space: vertex i > 0 of the hidden order
+    hangs below a random earlier one."""
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(n)
+    parent = np.full(n, -1)
+    for i in range(1, n):
+        parent[perm[i]] = perm[rng.integers(0, i)]
+    return parent, int(perm[0])
+
+
+@pytest.mark.parametrize("k,h", [(2, 5), (3, 4), (5, 2), (1, 7)])
+def test_tree_complete_kary_closed_form(k, h):
+    """Complete k-ary tree of height h, heap-numbered: a vertex at depth t has
+    height h - t and (k^(h-t+1) - 1)/(k - 1) - 1 descendants (h - t for a chain)."""
+    n = sum(k ** t for t in range(h + 1))
+    parent = np.array([-1] + [(v - 1) // k for v in range(1, n)])
+    row, col = _tree_csr(parent, 1)
+    depth = np.zeros(n, int)
+    for v in range(1, n):
+        depth[v] = depth[parent[v]] + 1
+    hh, dd, th = orc.tree(row, col, 0)
+    assert th == h
+    assert np.array_equal(hh, h - depth)
+    size = (h - depth + 1) if k == 1 else (k ** (h - depth + 1) - 1) // (k - 1)
+    assert np.array_equal(dd, size - 1)
+
+
+def test_tree_star_and_single_vertex():
+    n = 50
+    row, col = _tree_csr(np.array([-1] + [0] * (n - 1)), 2)
+    hh, dd, th = orc.tree(row, col, 0)
+    assert th == 1 and hh[0] == 1 and dd[0] == n - 1
+    assert np.all(hh[1:] == 0) and np.all(dd[1:] == 0)
+    hh, dd, th = orc.tree(np.array([0, 0], np.int32), np.zeros(0, np.int32), 0)
+    assert th == 0 and hh[0] == 0 and dd[0] == 0
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_tree_vs_brute_force(seed):
+    """Against a separate BFS from every vertex: desc = reached - 1 and
+    height = the deepest BFS level; vertices outside the root's tree get -1."""
+    n = 60 + 17 * seed
+    parent, root = _random_parent(n, seed)
+    # a second tree on other ids stays unreached
+    extra = 9
+    parent = np.concatenate([parent, [-1] + [n + int(i) for i in np.random.default_rng(seed).integers(0, np.arange(1, extra))]])
+    row, col = _tree_csr(parent, seed)
+    hh, dd, th = orc.tree(row, col, root)
+    for v in range(n + extra):
+        if v >= n:
+            assert hh[v] == -1 and dd[v] == -1
+            continue
+        lv = orc.bfs(row, col, v)
+        reached = lv != orc.UNDEF
+        assert dd[v] == reached.sum() - 1
+        assert hh[v] == lv[reached].max()
+    assert th == hh[root]
+
+
+def test_tree_rejects_two_in_arcs_and_ignores_arcs_into_root():
+    row, col = _tree_csr(np.array([-1, 0, 0, 1]), 3)
+    # vertex 3 gets a second in-arc (from 2)
+    row2 = row.copy()
+    row2[3:] += 1
+    col2 = np.insert(col, row[2], 3)
+    with pytest.raises(ValueError):
+        orc.tree(row2, col2, 0)
+    # an arc 3 -> 0 (into the root) closes a cycle and is ignored
+    row3 = row.copy()
+    row3[4:] += 1
+    col3 = np.append(col, 0).astype(np.int32)
+    a = orc.tree(row, col, 0)
+    b = orc.tree(row3, col3, 0)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2] == 2
