@@ -388,7 +388,7 @@ struct LArgs {
 
 constexpr int kLayerIlp = 4;
 constexpr int kBarLines = 10;    // grid barrier of k_layer_short: 8 arrival lines, a total, a release
-constexpr int kLayerCountSlot = 3;  // control slot of the counting chunks (round 1 clears it)
+constexpr int kLayerCountSlot = 3;  // control slot of the counting chunks (cleared when layer r = 1 starts)
 
 // (prio[u], u) < (prio[v], v) with pu = prio[u] already loaded.
 __device__ __forceinline__ bool lprec(const LArgs& a, int u, uint32_t pu, int v) {
