@@ -72,3 +72,40 @@ def test_multi_graph_bad_args(ctx):
     with pytest.raises(smdp.SmdpError) as e:
         ctx.sssp_multi_graph([g], np.array([5], dtype=np.int32))
     assert e.value.code == smdp.ERR_ARG
+
+
+def test_batch_big_graph_solved_by_a_group(ctx):
+    """A graph far above a workgroup's share of the batch is solved by a group
+    of workgroups that meet at a barrier after each round; the small ones are
+    claimed one per workgroup.  Exact on every replay."""
+    big = smdp.gen_small_graph(0, vlo=60000, vhi=60000, dlo=16.0, dhi=16.0, seed=11)
+    graphs = [big] + [smdp.gen_small_graph(i, vlo=64, vhi=2048, seed=12) for i in range(1, 31)]
+    srcs = np.zeros(len(graphs), dtype=np.int32)
+    srcs[0] = 17
+    refs = [orc.dijkstra(g.row, g.col, g.w, int(s)) for g, s in zip(graphs, srcs)]
+    print(f"group batch: big graph {big.n} vertices {big.m} arcs", flush=True)
+    with smdp.Batch(ctx, graphs, srcs) as b:
+        for rep in range(3):
+            b.run()
+            for i in range(len(graphs)):
+                assert np.array_equal(b.read(i), refs[i]), (rep, i)
+
+
+def test_batch_group_negative_cycle(ctx):
+    """A negative cycle in a group-solved graph ends after n rounds with the
+    overflow error, like one solved by a single workgroup."""
+    big = smdp.gen_small_graph(0, vlo=4000, vhi=4000, dlo=16.0, dhi=16.0, seed=13)
+    row = np.asarray(big.row, np.int64).copy()
+    col = np.asarray(big.col).tolist()
+    w = np.asarray(big.w).tolist()
+    # arcs 0 -> 1 and 1 -> 0 of weight -5: a cycle reached from source 0
+    for u, v in ((0, 1), (1, 0)):
+        col.insert(int(row[u + 1]), v)
+        w.insert(int(row[u + 1]), -5)
+        row[u + 1:] += 1
+    neg = smdp.HostGraph(row.astype(np.int32), np.asarray(col, np.int32), np.asarray(w, np.int32), 0)
+    graphs = [neg] + [smdp.gen_small_graph(i, vlo=64, vhi=512, seed=14) for i in range(1, 9)]
+    print("group batch: negative cycle", flush=True)
+    with pytest.raises(smdp.SmdpError) as e:
+        ctx.sssp_multi_graph(graphs, np.zeros(len(graphs), dtype=np.int32))
+    assert e.value.code == smdp.ERR_OVERFLOW
