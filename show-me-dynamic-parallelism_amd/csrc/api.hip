@@ -50,6 +50,12 @@ static smdp_params default_params(int app) {
       p.split_threshold = 512;
       p.chunk = 256;
       break;
+    case SMDP_APP_BFS:
+      // rounds enqueued before the host first checks the queue: a BFS of up
+      // to 8 levels then needs one check (4M-node Zipf tree, 6 levels: 0.232
+      // -> 0.204 ms); SSSP was the same with 4, 8 or 16
+      p.batch_rounds = 8;
+      break;
     default: break;
   }
   return p;
