@@ -215,3 +215,36 @@ def test_bfs_forest_fast_path(ctx):
             for _ in range(2):
                 dg.bfs(src, smdp.SOL_TUNED)
                 assert np.array_equal(dg.read(smdp.RES_LEVEL), ref), name
+
+
+@pytest.mark.parametrize("sol", SSSP_SOLS)
+def test_sssp_round_cap_and_negative_cycle(ctx, sol):
+    """Overflow handling: a round cap below what the graph needs, and a
+    negative cycle reached from the source, both end with ERR_OVERFLOW
+    instead of looping (the reference has no bound on its round loop)."""
+    g = GRAPHS["rmat12"]
+    old = ctx.get_params(smdp.APP_SSSP)
+    try:
+        ctx.set_params(smdp.APP_SSSP, max_rounds=2)
+        with pytest.raises(smdp.SmdpError) as e:
+            ctx.sssp(g, g.source, sol)
+        assert e.value.code == smdp.ERR_OVERFLOW
+        ctx.set_params(smdp.APP_SSSP, max_rounds=200)
+        # arcs 0 -> 1 and 1 -> 0 of weight -3: a cycle reached from vertex 0
+        row = np.asarray(g.row, np.int64).copy()
+        col = np.asarray(g.col).tolist()
+        w = np.asarray(g.w).tolist()
+        for u, v in ((0, 1), (1, 0)):
+            col.insert(int(row[u + 1]), v)
+            w.insert(int(row[u + 1]), -3)
+            row[u + 1:] += 1
+        neg = smdp.HostGraph(row.astype(np.int32), np.asarray(col, np.int32), np.asarray(w, np.int32), 0)
+        with pytest.raises(smdp.SmdpError) as e:
+            ctx.sssp(neg, 0, sol)
+        assert e.value.code == smdp.ERR_OVERFLOW
+        # the context still works afterwards
+        ctx.set_params(smdp.APP_SSSP, **old.as_dict())
+        d, _ = ctx.sssp(g, g.source, sol)
+        assert np.array_equal(d, orc.dijkstra(g.row, g.col, g.w, g.source))
+    finally:
+        ctx.set_params(smdp.APP_SSSP, **old.as_dict())
