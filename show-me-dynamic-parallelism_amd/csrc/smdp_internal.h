@@ -109,6 +109,8 @@ struct smdp_ctx {
   std::vector<hipEvent_t> prof_ev;  // pairs of events around profiled launches
   std::vector<int> prof_kind;       // kernel kind of each pair
   int32_t* pinned = nullptr;        // small host readback buffer (hipHostMalloc)
+  int32_t* stage[2] = {nullptr, nullptr};  // pinned staging of large uploads (hipHostMalloc, lazily)
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
   unsigned long long* rlog = nullptr;  // per-round (queue length, arcs) when profiling
   std::vector<int64_t> log_qlen, log_arcs;
   std::vector<double> log_ms;

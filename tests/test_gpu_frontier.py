@@ -248,3 +248,21 @@ def test_sssp_round_cap_and_negative_cycle(ctx, sol):
         assert np.array_equal(d, orc.dijkstra(g.row, g.col, g.w, g.source))
     finally:
         ctx.set_params(smdp.APP_SSSP, **old.as_dict())
+
+
+def test_large_csr_checked_while_staged(ctx):
+    """Graphs of >= 4M arcs go up through pinned staging, with the column
+    check made in the same pass: a column past n, or a negative one, is
+    still rejected, and a valid graph still solves exactly."""
+    g = smdp.gen_rmat(18, 17, wlo=1, whi=255, seed=8)  # 4.46M arcs
+    assert g.m >= 1 << 22
+    print(f"staged upload {g.m} arcs", flush=True)
+    d, _ = ctx.sssp(g, g.source, smdp.SOL_TUNED)
+    assert np.array_equal(d, orc.dijkstra(g.row, g.col, g.w, g.source))
+    for bad in (g.n, -7):
+        col = np.asarray(g.col).copy()
+        col[len(col) - 3] = bad
+        h = smdp.HostGraph(g.row, col, g.w, g.source)
+        with pytest.raises(smdp.SmdpError) as e:
+            ctx.sssp(h, g.source, smdp.SOL_TUNED)
+        assert e.value.code == smdp.ERR_ARG
