@@ -150,6 +150,7 @@ int fill_stats(smdp_graph* g, smdp_stats* st, double ms_kernel, uint64_t launche
 // [0, check_n) (the CSR column check, made in the same pass so host memory
 // is read once); *bad is set otherwise.  Returns after the last copy.
 constexpr int64_t kStageInts = int64_t(8) << 20;  // 32 MB per staging buffer
+constexpr int64_t kStagedMin = int64_t(1) << 22;  // 4-byte entries: smaller arrays go up in one plain copy
 static int staged_upload(smdp_ctx* ctx, int32_t* dst, const int32_t* src, int64_t count, int64_t check_n,
                          std::atomic<bool>* bad) {
   for (int i = 0; i < 2; ++i) {
@@ -212,6 +213,20 @@ static int staged_upload(smdp_ctx* ctx, int32_t* dst, const int32_t* src, int64_
   for (auto& x : team) x.join();
   SMDP_HIP(hipStreamSynchronize(ctx->stream));
   return err.load();
+}
+
+// One array of 4-byte entries (values, x, out-degrees, priorities).
+template <typename T>
+static int upload4(smdp_ctx* ctx, T** d, const T* s, int64_t cnt) {
+  static_assert(sizeof(T) == sizeof(int32_t), "4-byte entries");
+  if (!s) return SMDP_ERR_ARG;
+  SMDP_CHECK(alloc_once(d, cnt));
+  if (cnt >= kStagedMin) {
+    std::atomic<bool> unused{false};
+    return staged_upload(ctx, reinterpret_cast<int32_t*>(*d), reinterpret_cast<const int32_t*>(s), cnt, -1, &unused);
+  }
+  if (cnt > 0) SMDP_HIP(hipMemcpy(*d, s, sizeof(T) * cnt, hipMemcpyHostToDevice));
+  return SMDP_OK;
 }
 
 }  // namespace smdp
@@ -347,7 +362,6 @@ int smdp_graph_create(smdp_ctx* ctx, const smdp_csr* h, smdp_graph** out) {
   // Every kernel indexes row[u..u+1] and dist[col[e]] without bounds checks,
   // so the CSR is validated here once (on all host threads).
   // (columns of large graphs are checked while they are staged for upload, below)
-  constexpr int64_t kStagedMin = int64_t(1) << 22;  // ints: smaller arrays go up in one plain copy
   {
     std::atomic<bool> bad{false};
     parallel_ranges(h->n, [&](int64_t b, int64_t e) {
@@ -407,40 +421,29 @@ void smdp_graph_destroy(smdp_graph* g) {
   delete g;
 }
 
-static int upload_f32(float** d, const float* s, int64_t cnt) {
-  if (!s) return SMDP_ERR_ARG;
-  SMDP_CHECK(alloc_once(d, cnt));
-  if (cnt > 0) SMDP_HIP(hipMemcpy(*d, s, sizeof(float) * cnt, hipMemcpyHostToDevice));
-  return SMDP_OK;
-}
-
 int smdp_graph_set_values(smdp_graph* g, const float* val) {
   if (!g) return SMDP_ERR_ARG;
   SMDP_HIP(hipSetDevice(g->ctx->device));
   ++g->val_version;  // layouts derived from the values must be rebuilt
-  return upload_f32(&g->val, val, g->m);
+  return upload4(g->ctx, &g->val, val, g->m);
 }
 
 int smdp_graph_set_x(smdp_graph* g, const float* x) {
   if (!g) return SMDP_ERR_ARG;
   SMDP_HIP(hipSetDevice(g->ctx->device));
-  return upload_f32(&g->x, x, g->n);
+  return upload4(g->ctx, &g->x, x, g->n);
 }
 
 int smdp_graph_set_outdeg(smdp_graph* g, const int32_t* outdeg) {
   if (!g || !outdeg) return SMDP_ERR_ARG;
   SMDP_HIP(hipSetDevice(g->ctx->device));
-  SMDP_CHECK(alloc_once(&g->outdeg, g->n));
-  if (g->n > 0) SMDP_HIP(hipMemcpy(g->outdeg, outdeg, sizeof(int32_t) * g->n, hipMemcpyHostToDevice));
-  return SMDP_OK;
+  return upload4(g->ctx, &g->outdeg, outdeg, g->n);
 }
 
 int smdp_graph_set_prio(smdp_graph* g, const uint32_t* prio) {
   if (!g || !prio) return SMDP_ERR_ARG;
   SMDP_HIP(hipSetDevice(g->ctx->device));
-  SMDP_CHECK(alloc_once(&g->prio, g->n));
-  if (g->n > 0) SMDP_HIP(hipMemcpy(g->prio, prio, sizeof(uint32_t) * g->n, hipMemcpyHostToDevice));
-  return SMDP_OK;
+  return upload4(g->ctx, &g->prio, prio, g->n);
 }
 
 int smdp_graph_read(smdp_graph* g, int what, void* out, int64_t count) {
