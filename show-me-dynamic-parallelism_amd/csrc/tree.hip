@@ -76,13 +76,36 @@ __global__ void k_tree_init(TArgs a) {
   }
 }
 
-// inner[u] = arcs of u to vertices that have arcs (once per graph).
+// inner[u] = arcs of u to vertices that have arcs (once per graph, inner[]
+// zeroed before).  A thread takes kInnerSeg consecutive arcs, finds the
+// first one's row by binary search and adds one count per row it touches, so
+// a 4096-arc hub is counted by 256 threads instead of one (one thread per
+// vertex took 674 us on the 4M-node Zipf tree).
+constexpr int kInnerSeg = 16;
 __global__ void k_tree_inner(const int32_t* __restrict__ row, const int32_t* __restrict__ col,
                              const uint32_t* __restrict__ hasout, int32_t n, int32_t* inner) {
-  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n; u += (int64_t)gridDim.x * blockDim.x) {
-    int k = 0;
-    for (int e = row[u]; e < row[u + 1]; ++e) k += (hasout[col[e] >> 5] >> (col[e] & 31)) & 1u;
-    inner[u] = k;
+  const int64_t m = row[n];
+  const int64_t nseg = (m + kInnerSeg - 1) / kInnerSeg;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nseg; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t e0 = t * kInnerSeg, e1 = min(e0 + kInnerSeg, m);
+    int lo = 0, hi = n - 1;  // the last u with row[u] <= e0 (a row with arcs)
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (row[mid] <= e0) lo = mid;
+      else hi = mid - 1;
+    }
+    int u = lo, c = 0;
+    int end = row[u + 1];
+    for (int64_t e = e0; e < e1; ++e) {
+      while (e >= end) {  // the next row with arcs
+        if (c) atomicAdd(&inner[u], c);
+        c = 0;
+        ++u;
+        end = row[u + 1];
+      }
+      c += (hasout[col[e] >> 5] >> (col[e] & 31)) & 1u;
+    }
+    if (c) atomicAdd(&inner[u], c);
   }
 }
 
@@ -263,8 +286,11 @@ static int run_tree(smdp_graph* g, int32_t root, smdp_stats* st) {
   SMDP_CHECK(build_hasout(g, g->row, &g->hasout));
   if (!g->tree_inner) {
     SMDP_CHECK(alloc_once(&g->tree_inner, n));
-    hipLaunchKernelGGL(k_tree_inner, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 4096)), dim3(256), 0,
-                       ctx->stream, g->row, g->col, g->hasout, g->n, g->tree_inner);
+    SMDP_HIP(hipMemsetAsync(g->tree_inner, 0, sizeof(int32_t) * n, ctx->stream));
+    const int64_t nseg = ((int64_t)g->m + kInnerSeg - 1) / kInnerSeg;
+    if (nseg > 0)
+      hipLaunchKernelGGL(k_tree_inner, dim3((unsigned)std::min<int64_t>((nseg + 255) / 256, 8192)), dim3(256), 0,
+                         ctx->stream, g->row, g->col, g->hasout, g->n, g->tree_inner);
     SMDP_HIP(hipGetLastError());
   }
   // one buffer: level starts and per-level reservations (indices up to
