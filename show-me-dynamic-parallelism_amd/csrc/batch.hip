@@ -53,23 +53,25 @@ __device__ __forceinline__ void ms_arcs(const MArgs& a, int s, int e, int du, in
                                         unsigned long long& wins) {
   const int lane = lane_id();
   for (int b = s; b < e; b += SMDP_WAVE) {
-    // one coalesced load of up to 64 arcs, then broadcast one at a time
+    // one coalesced load of up to 64 arcs (at clamped positions), then 8 arcs
+    // at a time: their 8 rows of 64 distances are loaded before any is used
+    // (4 at a time was 3% slower on RMAT-21)
     const int k = b + lane;
-    const int cv = k < e ? a.col[k] : 0;
-    const int cw = k < e ? a.w[k] : 0;
+    const int cv = a.col[min(k, e - 1)];
+    const int cw = a.w[min(k, e - 1)];
     const int cnt = min(SMDP_WAVE, e - b);
-    for (int j0 = 0; j0 < cnt; j0 += 4) {
-      int v[4], dv[4], alt[4];
+    for (int j0 = 0; j0 < cnt; j0 += 8) {
+      int v[8], dv[8], alt[8];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
+      for (int t = 0; t < 8; ++t) {
         const int j = min(j0 + t, cnt - 1);
         v[t] = __builtin_amdgcn_readlane(cv, j);
         alt[t] = du + __builtin_amdgcn_readlane(cw, j);
       }
 #pragma unroll
-      for (int t = 0; t < 4; ++t) dv[t] = a.mdist[(int64_t)v[t] * kLanes + lane];
+      for (int t = 0; t < 8; ++t) dv[t] = a.mdist[(int64_t)v[t] * kLanes + lane];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
+      for (int t = 0; t < 8; ++t) {
         bool imp = false;
         if (j0 + t < cnt && du < SMDP_SSSP_INF && alt[t] < dv[t]) {
           const int old = atomicMin(&a.mdist[(int64_t)v[t] * kLanes + lane], alt[t]);
