@@ -31,6 +31,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
+#include <cstring>
 #include <vector>
 
 #include "device_common.h"
@@ -969,9 +971,16 @@ int run_pagerank(smdp_graph* g, const smdp_pr_opts* o, int sol, smdp_stats* st) 
   hipLaunchKernelGGL(k_pr_init, dim3(std::min<int64_t>((g->n + 255) / 256, 4096)), dim3(256), 0, s, g->rank[0],
                      cbuf[0], (const int32_t*)g->outdeg, g->n, init, b->st, dang0, g->cnt);
   ++launches;
+  // Iterations are enqueued in batches; between batches the host reads the
+  // stop flag and the last L1 residual.  From the second check on, the next
+  // batch is sized by the residual's measured decay per iteration, so that
+  // few launches are left to return at once after the stop rule fires.
+  static_assert(offsetof(RState, done) == offsetof(RState, residual) + sizeof(double), "RState readback layout");
   int k = 0;
   int batch = std::max(4, p.batch_rounds);
   int32_t done = 0;
+  double r_prev = 0.0;
+  int k_prev = 0;
   while (k < o->max_iter) {
     for (int j = 0; j < batch && k < o->max_iter; ++j, ++k) {
       a.x = cbuf[k & 1];
@@ -982,11 +991,24 @@ int run_pagerank(smdp_graph* g, const smdp_pr_opts* o, int sol, smdp_stats* st) 
       else launch_pass<true>(sol, grid, s, a, k, &launches, ctx);
     }
     SMDP_HIP(hipGetLastError());
-    SMDP_HIP(hipMemcpyAsync(ctx->pinned, &b->st->done, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    SMDP_HIP(hipMemcpyAsync(ctx->pinned, &b->st->residual, sizeof(double) + sizeof(int32_t), hipMemcpyDeviceToHost,
+                            s));
     SMDP_HIP(hipStreamSynchronize(s));
-    done = ctx->pinned[0];
+    double r = 0.0;
+    std::memcpy(&r, ctx->pinned, sizeof(double));
+    done = ctx->pinned[2];
     if (done) break;
-    batch = std::min(batch * 2, 32);
+    int next = std::min(batch * 2, 32);
+    if (k_prev > 0 && r > a.tol_abs && r < r_prev) {
+      // residual ~ r * rho^j after j more iterations; one extra iteration
+      // is cheaper than another host round trip when the estimate is short
+      const double rho = std::pow(r / r_prev, 1.0 / (k - k_prev));
+      const double need = std::log(a.tol_abs / r) / std::log(rho);
+      if (std::isfinite(need) && need >= 0.0) next = std::max(1, std::min(32, (int)std::ceil(need) + 1));
+    }
+    r_prev = r;
+    k_prev = k;
+    batch = next;
   }
   SMDP_HIP(hipEventRecord(ctx->ev_end, s));
   SMDP_HIP(hipEventSynchronize(ctx->ev_end));
