@@ -342,3 +342,21 @@ def test_binned_layout_is_pagerank_only(ctx):
         ctx.set_params(smdp.APP_SPMV, layout=smdp.LAYOUT_BINNED)
     with pytest.raises(smdp.SmdpError):
         ctx.set_params(smdp.APP_PAGERANK, layout=3)
+
+
+def test_pagerank_batches_follow_residual_decay(binned_ctx):
+    """Iterations are enqueued in batches sized from the residual's decay, so
+    a converging solve enqueues few iterations past the one where its stop
+    rule fires (doubling batches of 4, 8, 16 would enqueue 28 for an
+    18-iteration solve).  Launches: one initialisation, then a scatter and a
+    reduce per enqueued iteration."""
+    g = smdp.gen_rmat(16, 16, seed=3)
+    ig, od = smdp.invert(g), g.outdegree()
+    r, st = binned_ctx.pagerank(ig, od, smdp.pr_opts(0.85, 1.0, 1e-6, 100), smdp.SOL_TUNED)
+    it = int(st.rounds)
+    enqueued = (int(st.kernel_launches) - 1) // 2
+    print(f"pagerank rmat16: {it} iterations, {enqueued} enqueued", flush=True)
+    assert 12 <= it < 100
+    assert it <= enqueued <= it + 4, (it, enqueued)
+    ref, _, _ = orc.pagerank(ig.row, ig.col, od, d=0.85, T=1.0, fixed_iters=it)
+    assert np.abs(r.astype(np.float64) - ref).sum() <= 1e-6
