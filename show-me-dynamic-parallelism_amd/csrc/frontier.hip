@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 
 #include "device_common.h"
 #include "smdp_internal.h"
@@ -885,8 +886,14 @@ int run_frontier(smdp_graph* g, int32_t src, int sol, bool bfs, smdp_stats* st) 
   hipLaunchKernelGGL(k_init_frontier, dim3(std::min<int64_t>((g->n + 255) / 256, 4096)), dim3(256), 0, s,
                      g->dist, g->stamp, g->n, inf, src, g->q[0], g->ctl, g->cnt);
   ++launches;
+  // Rounds are enqueued in batches; between batches the host reads the queue
+  // length of the next round.  Batches double while the queue grows; once it
+  // shrinks, the next batch is the number of rounds its measured decay per
+  // round needs to empty it, plus one, so few empty rounds are enqueued.
   int r = 0;
   int batch = std::max(1, p.batch_rounds);
+  int32_t q_prev = 0;
+  int r_prev = 0;
   for (;;) {
     for (int j = 0; j < batch; ++j, ++r) {
       SMDP_CHECK(prof_mark(ctx, 0));
@@ -903,9 +910,19 @@ int run_frontier(smdp_graph* g, int32_t src, int sol, bool bfs, smdp_stats* st) 
     SMDP_HIP(hipGetLastError());
     SMDP_HIP(hipMemcpyAsync(ctx->pinned, &g->ctl[r & 3].qlen, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     SMDP_HIP(hipStreamSynchronize(s));
-    if (ctx->pinned[0] == 0) break;
+    const int32_t q = ctx->pinned[0];
+    if (q == 0) break;
     if (r >= p.max_rounds) return SMDP_ERR_OVERFLOW;
-    batch = std::min(batch * 2, 16);
+    int next = std::min(batch * 2, 16);
+    if (r_prev > 0 && q < q_prev) {
+      // q * rho^j < 1 after j more rounds
+      const double rho = std::pow((double)q / q_prev, 1.0 / (r - r_prev));
+      const double need = std::log((double)q) / -std::log(rho);
+      if (std::isfinite(need) && need >= 0.0) next = std::max(1, std::min(16, (int)std::ceil(need) + 1));
+    }
+    q_prev = q;
+    r_prev = r;
+    batch = next;
   }
   SMDP_HIP(hipEventRecord(ctx->ev_end, s));
   SMDP_HIP(hipEventSynchronize(ctx->ev_end));
