@@ -534,9 +534,10 @@ int run_multi_source(smdp_graph* g, const int32_t* srcs, int32_t k, int32_t* hos
                        g->ctl, g->cnt);
     hipLaunchKernelGGL(k_ms_seed, dim3(1), dim3(64), 0, s, g->mdist, g->stamp, d_srcs + s0, kk, g->q[0], g->ctl);
     launches += 2;
-    int r = 0, batch = std::max(1, p.batch_rounds);
+    int r = 0;
+    RoundBatcher rb(p.batch_rounds, 16);
     for (;;) {
-      for (int j = 0; j < batch; ++j, ++r) {
+      for (int j = 0; j < rb.batch; ++j, ++r) {
         prof_mark(ctx, 0);
         hipLaunchKernelGGL(k_ms_expand, dim3(grid), dim3(SMDP_WG), 0, s, a, r);
         prof_mark(ctx, 0);
@@ -556,7 +557,7 @@ int run_multi_source(smdp_graph* g, const int32_t* srcs, int32_t k, int32_t* hos
         rc = SMDP_ERR_OVERFLOW;
         break;
       }
-      batch = std::min(batch * 2, 16);
+      rb.advance(ctx->pinned[0], r);
     }
     if (rc != SMDP_OK) break;
     hipLaunchKernelGGL(k_ms_transpose, dim3((unsigned)((n + 63) / 64)), dim3(64, 16), 0, s, g->mdist, g->n, kk,

@@ -25,7 +25,6 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <cmath>
 
 #include "device_common.h"
 #include "smdp_internal.h"
@@ -886,16 +885,10 @@ int run_frontier(smdp_graph* g, int32_t src, int sol, bool bfs, smdp_stats* st) 
   hipLaunchKernelGGL(k_init_frontier, dim3(std::min<int64_t>((g->n + 255) / 256, 4096)), dim3(256), 0, s,
                      g->dist, g->stamp, g->n, inf, src, g->q[0], g->ctl, g->cnt);
   ++launches;
-  // Rounds are enqueued in batches; between batches the host reads the queue
-  // length of the next round.  Batches double while the queue grows; once it
-  // shrinks, the next batch is the number of rounds its measured decay per
-  // round needs to empty it, plus one, so few empty rounds are enqueued.
   int r = 0;
-  int batch = std::max(1, p.batch_rounds);
-  int32_t q_prev = 0;
-  int r_prev = 0;
+  RoundBatcher rb(p.batch_rounds, 16);
   for (;;) {
-    for (int j = 0; j < batch; ++j, ++r) {
+    for (int j = 0; j < rb.batch; ++j, ++r) {
       SMDP_CHECK(prof_mark(ctx, 0));
       if (bfs) launch_round<true>(sol, grid, s, a, r, &launches);
       else launch_round<false>(sol, grid, s, a, r, &launches);
@@ -913,16 +906,7 @@ int run_frontier(smdp_graph* g, int32_t src, int sol, bool bfs, smdp_stats* st) 
     const int32_t q = ctx->pinned[0];
     if (q == 0) break;
     if (r >= p.max_rounds) return SMDP_ERR_OVERFLOW;
-    int next = std::min(batch * 2, 16);
-    if (r_prev > 0 && q < q_prev) {
-      // q * rho^j < 1 after j more rounds
-      const double rho = std::pow((double)q / q_prev, 1.0 / (r - r_prev));
-      const double need = std::log((double)q) / -std::log(rho);
-      if (std::isfinite(need) && need >= 0.0) next = std::max(1, std::min(16, (int)std::ceil(need) + 1));
-    }
-    q_prev = q;
-    r_prev = r;
-    batch = next;
+    rb.advance(q, r);
   }
   SMDP_HIP(hipEventRecord(ctx->ev_end, s));
   SMDP_HIP(hipEventSynchronize(ctx->ev_end));

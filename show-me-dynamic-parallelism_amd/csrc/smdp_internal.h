@@ -5,6 +5,7 @@
 
 #include <string>
 #include <algorithm>
+#include <cmath>
 #include <thread>
 #include <vector>
 
@@ -221,6 +222,30 @@ void prof_reset(smdp_ctx* ctx);
 // Records one event; call before and after each launch of kernel `kind`
 // (0: phase-1 / dominant kernel, 1: phase-2 kernel).
 int prof_mark(smdp_ctx* ctx, int kind);
+// Rounds to enqueue in the next batch of a host loop that reads the next
+// round's queue length q after round r: batches double (up to cap) while the
+// queue grows; once it shrinks, the next batch is the number of rounds that
+// its measured decay per round needs to empty it (q * rho^j < 1), plus one,
+// so few empty rounds are enqueued after the last one with work.
+struct RoundBatcher {
+  int batch;
+  int cap;
+  int32_t q_prev = 0;
+  int r_prev = 0;
+  RoundBatcher(int first, int cap_) : batch(std::max(1, first)), cap(cap_) {}
+  void advance(int32_t q, int r) {
+    int next = std::min(batch * 2, cap);
+    if (r_prev > 0 && q > 0 && q < q_prev) {
+      const double rho = std::pow((double)q / q_prev, 1.0 / (r - r_prev));
+      const double need = std::log((double)q) / -std::log(rho);
+      if (std::isfinite(need) && need >= 0.0) next = std::max(1, std::min(cap, (int)std::ceil(need) + 1));
+    }
+    q_prev = q;
+    r_prev = r;
+    batch = next;
+  }
+};
+
 // Runs body(lo, hi) over [0, count) split into contiguous ranges, one per
 // host thread (at most 16).
 template <typename F>
