@@ -1079,7 +1079,12 @@ static int run_color_layers(smdp_graph* g, int use_prio, smdp_stats* st) {
   // A step is one layer on the whole grid (expand, then the drain of its
   // chunks) followed by the run of short layers after it in one launch.  The
   // layer indices live on the device (rd), so steps are enqueued in batches
-  // and the host only checks rd between batches.
+  // and the host only checks rd between batches.  A step enqueued after the
+  // last one with work still costs ~20 us (three launches, one cooperative),
+  // so batches stop growing at 8: on RMAT-23 (at most 148 steps with work)
+  // caps of 64, 16 and 8 enqueued 188, 156 and 148 steps and took 38.9, 38.1
+  // and 38.0 ms; sizing batches from the next layer's decay per step
+  // overshot (203 steps).
   int batch = std::max(1, p.batch_rounds);
   for (;;) {
     for (int j = 0; j < batch; ++j) {
@@ -1103,7 +1108,7 @@ static int run_color_layers(smdp_graph* g, int use_prio, smdp_stats* st) {
     if (ctx->pinned[3]) return SMDP_ERR_DEVICE;  // a grid barrier gave up waiting
     if (ctx->pinned[2]) break;
     if (ctx->pinned[0] >= p.max_rounds || ctx->pinned[0] >= g->n + 1) return SMDP_ERR_OVERFLOW;
-    batch = std::min(batch * 2, 64);
+    batch = std::min(batch * 2, 8);
   }
   SMDP_HIP(hipEventRecord(ctx->ev_end, s));
   SMDP_HIP(hipEventSynchronize(ctx->ev_end));
