@@ -266,3 +266,18 @@ def test_large_csr_checked_while_staged(ctx):
         with pytest.raises(smdp.SmdpError) as e:
             ctx.sssp(h, g.source, smdp.SOL_TUNED)
         assert e.value.code == smdp.ERR_ARG
+
+
+def test_round_batches_follow_queue_decay(ctx):
+    """Rounds are enqueued in batches; once the queue shrinks, the next batch
+    is sized from its decay per round, so a solve enqueues few rounds past
+    the last one with work (doubling batches of 4, 8, 16 would enqueue 28
+    rounds for a 13- to 27-round solve; RMAT-18 takes about 13).  Launches: one initialisation, then
+    the tile kernel and the grid-queue drain per enqueued round."""
+    g = smdp.gen_rmat(18, 16, wlo=1, whi=255, seed=3)
+    d, st = ctx.sssp(g, g.source, smdp.SOL_TUNED)
+    assert np.array_equal(d, orc.dijkstra(g.row, g.col, g.w, g.source))
+    rounds = int(st.rounds)
+    enqueued = (int(st.kernel_launches) - 1) // 2
+    print(f"sssp rmat18: {rounds} rounds, {enqueued} enqueued", flush=True)
+    assert rounds <= enqueued <= max(rounds + 6, 12), (rounds, enqueued)
