@@ -92,6 +92,9 @@ typedef struct {
   int32_t batch_rounds;     /* rounds enqueued between two host checks of the queue length */
   int32_t max_rounds;       /* safety cap on rounds (SMDP_ERR_OVERFLOW when hit) */
   int32_t layout;           /* SpMV/PageRank tuned: arc layout, one of SMDP_LAYOUT_* (0 = automatic) */
+  int32_t delta;            /* SSSP tuned, bucketed layout: bucket width, a power of two (0 = 64) */
+  int32_t persistent;       /* SSSP tuned, bucketed layout: 0 = one launch per pass; 1 = every phase of the solve in
+                               one launch whose workgroups are all resident, with a grid barrier between passes */
 } smdp_params;
 
 /* Arc layouts of the tuned SpMV/PageRank solution (smdp_params.layout). */
@@ -102,6 +105,12 @@ typedef struct {
 #define SMDP_LAYOUT_HOT     3  /* SSSP/BFS tuned: the solve runs on a copy of the graph with vertices renumbered by
                                   falling in-degree (built once per resident graph; results are reported in the
                                   caller's numbering).  Automatic picks it for resident graphs of >= 2^20 arcs. */
+#define SMDP_LAYOUT_BUCKETS 4  /* SSSP tuned: bucketed rounds (delta-stepping) on the renumbered copy.  Vertices are
+                                  taken in order of distance buckets of width delta; within a bucket, arcs that stay in
+                                  the bucket are relaxed until no distance in it changes, then every other arc of the
+                                  bucket's vertices is relaxed once.  Automatic picks it for resident graphs of
+                                  >= 2^20 arcs without negative weights; a negative weight falls back to
+                                  SMDP_LAYOUT_HOT. */
 #define SMDP_LAYOUT_XCD8    8  /* sources split into 8 blocks, one per XCD, so each XCD gathers from a slice
                                   that fits its L2 */
 
@@ -125,6 +134,8 @@ typedef struct {
   uint64_t edges_phase2;       /* arcs of edges_relaxed scanned by the phase-2 kernel */
   uint64_t atomics_tried;      /* SSSP/BFS: atomicMin issued after a plain read said "improves" */
   uint64_t atomics_won;        /* SSSP/BFS: atomicMin that lowered the distance */
+  uint64_t arcs_skipped;       /* bucketed SSSP: arcs read but left to the bucket's other pass (not relaxed,
+                                  not in edges_relaxed) */
 } smdp_stats;
 
 /* PageRank options (the reference hard-codes them: DAMPING_FACTOR 0.5 and

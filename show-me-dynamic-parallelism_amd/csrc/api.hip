@@ -42,6 +42,7 @@ static smdp_params default_params(int app) {
   p.wg_per_cu = 8;
   p.batch_rounds = 4;
   p.max_rounds = 1 << 22;
+  p.delta = 64;
   switch (app) {
     case SMDP_APP_SPMV:
     case SMDP_APP_PAGERANK:
@@ -121,6 +122,7 @@ int read_counters(const Counters* dev, hipStream_t s, Counters* out) {
     c.edges2 += p.edges2;
     c.atom_try += p.atom_try;
     c.atom_win += p.atom_win;
+    c.skipped += p.skipped;
   }
   *out = c;
   return SMDP_OK;
@@ -139,6 +141,7 @@ int fill_stats(smdp_graph* g, smdp_stats* st, double ms_kernel, uint64_t launche
   st->edges_phase2 = c.edges2;
   st->atomics_tried = c.atom_try;
   st->atomics_won = c.atom_win;
+  st->arcs_skipped = c.skipped;
   st->kernel_launches = launches;
   return SMDP_OK;
 }
@@ -312,12 +315,16 @@ int smdp_set_params(smdp_ctx* ctx, int app, const smdp_params* p) {
   if (q.wg_per_cu <= 0) q.wg_per_cu = d.wg_per_cu;
   if (q.batch_rounds <= 0) q.batch_rounds = d.batch_rounds;
   if (q.max_rounds <= 0) q.max_rounds = d.max_rounds;
+  if (q.delta <= 0) q.delta = d.delta;
+  if (q.persistent != 0 && q.persistent != 1) return SMDP_ERR_ARG;
   if (q.wg_per_cu > 64 || q.chunk < 64) return SMDP_ERR_ARG;
+  if ((q.delta & (q.delta - 1)) != 0 || q.delta > (1 << 20)) return SMDP_ERR_ARG;  // a power of two
   if (q.layout != SMDP_LAYOUT_AUTO && q.layout != SMDP_LAYOUT_CSR && q.layout != SMDP_LAYOUT_BINNED &&
-      q.layout != SMDP_LAYOUT_HOT && q.layout != SMDP_LAYOUT_XCD8)
+      q.layout != SMDP_LAYOUT_HOT && q.layout != SMDP_LAYOUT_BUCKETS && q.layout != SMDP_LAYOUT_XCD8)
     return SMDP_ERR_ARG;
   if (q.layout == SMDP_LAYOUT_BINNED && app != SMDP_APP_PAGERANK) return SMDP_ERR_ARG;
   if (q.layout == SMDP_LAYOUT_HOT && app != SMDP_APP_SSSP && app != SMDP_APP_BFS) return SMDP_ERR_ARG;
+  if (q.layout == SMDP_LAYOUT_BUCKETS && app != SMDP_APP_SSSP) return SMDP_ERR_ARG;
   ctx->params[app] = q;
   return SMDP_OK;
 }
@@ -437,6 +444,7 @@ int smdp_graph_set_x(smdp_graph* g, const float* x) {
 int smdp_graph_set_outdeg(smdp_graph* g, const int32_t* outdeg) {
   if (!g || !outdeg) return SMDP_ERR_ARG;
   SMDP_HIP(hipSetDevice(g->ctx->device));
+  ++g->outdeg_version;  // the dangling count derived from them must be redone
   return upload4(g->ctx, &g->outdeg, outdeg, g->n);
 }
 

@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "device_common.h"
+#include "grid_barrier.h"
 #include "smdp_internal.h"
 
 namespace smdp {
@@ -387,7 +388,6 @@ struct LArgs {
 };
 
 constexpr int kLayerIlp = 4;
-constexpr int kBarLines = 10;    // grid barrier of k_layer_short: 8 arrival lines, a total, a release
 constexpr int kLayerCountSlot = 3;  // control slot of the counting chunks (cleared when layer r = 1 starts)
 
 // (prio[u], u) < (prio[v], v) with pu = prio[u] already loaded.
@@ -794,43 +794,6 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_expand(LArgs a) {
   }
 }
 
-// Grid barrier of k_layer_short, whose workgroups are all resident
-// (cooperative launch).  Workgroup b arrives on line b % 8 (about one eighth
-// of the workgroups per line: arrivals on one address run one at a time);
-// the last arrival of a line arrives on line 8, and the last of those
-// publishes the epoch on line 9, which everyone polls.  Counts only grow
-// (epoch e is complete at e arrivals per member), and the lines are zeroed
-// before each launch.  Gives up after about a second of polling: returns
-// false in every workgroup that gave up.
-constexpr int kBarSpinLimit = 1 << 21;
-__device__ __forceinline__ bool grid_barrier(uint32_t* bar, uint32_t epoch, int* s_ok) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const uint32_t nwg = gridDim.x;
-    const uint32_t grp = blockIdx.x & 7;
-    const uint32_t members = (nwg - grp + 7) / 8;  // workgroups g < nwg with g % 8 == grp
-    const uint32_t ngroups = nwg < 8 ? nwg : 8;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this workgroup's writes (made before the __syncthreads)
-    if (atomicAdd(&bar[32 * grp], 1u) + 1 == epoch * members &&
-        atomicAdd(&bar[32 * 8], 1u) + 1 == epoch * ngroups)
-      __hip_atomic_store(&bar[32 * 9], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    // relaxed polls (an acquire load invalidates the caches at every poll,
-    // which slowed every workgroup of the XCD), one acquire fence after
-    int ok = 1;
-    for (int spins = 0; __hip_atomic_load(&bar[32 * 9], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch;) {
-      __builtin_amdgcn_s_sleep(2);
-      if (++spins > kBarSpinLimit) {
-        ok = 0;
-        break;
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // everyone's writes, for the whole workgroup
-    *s_ok = ok;
-  }
-  __syncthreads();
-  return *s_ok != 0;
-}
-
 // Runs short layers one after another in one launch: layers rd[1] + 1,
 // rd[1] + 2, ... while they hold at most kShortL vertices, with a grid
 // barrier after each (a layer's queue is complete only when every workgroup
@@ -1132,6 +1095,7 @@ int run_color(smdp_graph* g, int use_prio, int sol, smdp_stats* st) {
   if (use_prio && !g->prio) return SMDP_ERR_ARG;
   smdp_ctx* ctx = g->ctx;
   SMDP_HIP(hipSetDevice(ctx->device));
+  begin_run(g);
   if (!g->colorbufs) g->colorbufs = new ColorBufs();
   ColorBufs* b = static_cast<ColorBufs*>(g->colorbufs);
   if (sol == SMDP_SOL_TUNED && g->n > 0) return run_color_layers(g, use_prio, st);

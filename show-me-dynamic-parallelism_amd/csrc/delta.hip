@@ -1,0 +1,809 @@
+// Bucketed SSSP (delta-stepping) for the tuned solution on gfx950.
+//
+// The reference relaxes in unordered rounds: every queued vertex relaxes
+// every arc, and a vertex is queued again each time its cost drops
+// (apps/sssp/src/sssp_kernel.cu:40-61, :496-506; apps/sssp/src/
+// sssp_wrapper.cu:147-302).  On RMAT-22 with weights in [1,255] that relaxes
+// about 3.3 arcs per reachable arc, most of them from hubs reached first by
+// long paths and expanded again when a shorter one arrives.  Distances are
+// the same for every relaxation order, so here vertices are taken in order of
+// distance buckets of width delta = 2^shift (Meyer and Sanders' delta-stepping)
+// and each arc is relaxed about once.
+//
+// A bucket b (distances in [b*delta, (b+1)*delta)) is processed in phases:
+//   light sweep  every vertex is read; those in bucket b relax the arcs of
+//                the light part of their row (w < delta, stored first) whose
+//                candidate stays in bucket b;
+//   light        vertices lowered within bucket b by the previous phase are
+//                expanded the same way from a queue (one entry per vertex per
+//                phase, deduplicated by a phase stamp), until a phase lowers
+//                none;
+//   settle       every vertex is read again; those in bucket b now have their
+//                final distance and relax, once, every arc of their row whose
+//                candidate lies beyond bucket b.  The pass also records the
+//                smallest distance beyond bucket b (distances it reads and
+//                candidates that won their atomicMin), which names the next
+//                bucket, so empty buckets cost nothing.
+// A vertex is expanded only while its distance is in the current bucket, and
+// it relaxes each arc in at most one of the light and settle passes with its
+// final distance (plus light re-expansions inside the bucket), so on RMAT-22
+// the arcs relaxed fall to about 1.06 per reachable arc.
+//
+// Phases are launched back to back; each kernel computes its own phase from
+// the previous phase's descriptor and counters (values no kernel of the
+// current phase writes), so the host enqueues phases blindly and reads the
+// state back once per batch.  Vertices with many arcs are cut into chunks on
+// the grid queue and drained by a second kernel per phase, as in the
+// unordered tuned solution.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "device_common.h"
+#include "frontier.h"
+#include "grid_barrier.h"
+#include "smdp_internal.h"
+
+namespace smdp {
+
+constexpr int kDIlp = 4;          // arcs per thread per step
+constexpr int kDSmallSplit = 32;  // split degree when the light queue is short
+constexpr int kDDrainTile = 8;    // grid-queue entries per drain tile (a power of two, at most 64)
+constexpr int kRunPhases = 1024;  // phases per persistent launch (the host checks the solve between launches)
+
+enum : int32_t { kSweepLight = 0, kLight = 1, kSettle = 2, kDone = 3 };
+
+// Descriptor of one phase; phase p's lives in slot p & 3 (one 128-byte line).
+struct DPhase {
+  int32_t mode;
+  int32_t bucket;
+  int32_t nin;  // entries: queue length (light) or vertices (sweeps)
+  int32_t pad[29];
+};
+static_assert(sizeof(DPhase) == 128, "one line per descriptor");
+constexpr int kPhaseInts = (int)(sizeof(DPhase) / sizeof(int32_t));
+
+// Phase after `prev`: qlen = light-queue entries pushed by prev, nb = the
+// largest next-bucket code stored by prev (INT32_MAX - bucket, 0 = none).
+__host__ __device__ inline DPhase next_phase(const DPhase& prev, int32_t qlen, int32_t nb, int32_t n) {
+  DPhase d = prev;
+  switch (prev.mode) {
+    case kSweepLight:
+    case kLight:
+      if (qlen > 0) {
+        d.mode = kLight;
+        d.nin = qlen;
+      } else {
+        d.mode = kSettle;
+        d.nin = n;
+      }
+      break;
+    case kSettle:
+      if (nb > 0) {
+        d.mode = kSweepLight;
+        d.bucket = INT32_MAX - nb;
+        d.nin = n;
+      } else {
+        d.mode = kDone;
+        d.nin = 0;
+      }
+      break;
+    default:
+      d.mode = kDone;
+      d.nin = 0;
+      break;
+  }
+  return d;
+}
+
+struct DArgs {
+  const int32_t* __restrict__ row;
+  const int32_t* __restrict__ col;
+  const int32_t* __restrict__ w;
+  const int32_t* __restrict__ lsplit;  // first heavy arc of each row (light arcs, w < delta, come first)
+  const uint32_t* __restrict__ hasout; // bit v set when row v has arcs
+  int32_t* dist;
+  int32_t* stamp;  // phase label of the last light-queue push
+  int32_t* q0;
+  int32_t* q1;
+  Chunk* hq;
+  Ctl* ctl;
+  DPhase* ph;
+  Counters* cnt;
+  unsigned long long* rlog;  // profiling: per phase (entries, arcs relaxed), or nullptr
+  int32_t n;
+  int32_t shift;    // delta = 1 << shift
+  int32_t split;    // an entry with at least this many arcs is cut into grid chunks
+  int32_t chunk;    // arcs per grid chunk
+  int32_t small_q;  // a light queue shorter than this splits entries of >= kDSmallSplit arcs
+};
+
+__device__ __forceinline__ bool d_has_arcs(const DArgs& a, int v) { return (a.hasout[v >> 5] >> (v & 31)) & 1u; }
+
+// Phase p, computed by every workgroup of phase p from phase p-1's descriptor
+// (written by workgroup 0 of phase p-1) and from slot p & 3, which phase p-1
+// appended to and no kernel of phase p writes.
+__device__ __forceinline__ DPhase phase_of(const DArgs& a, int p) {
+  if (p == 0) {
+    DPhase d;
+    d.mode = kSweepLight;
+    d.bucket = 0;  // the source, at distance 0
+    d.nin = a.n;
+    return d;
+  }
+  const DPhase prev = a.ph[(p - 1) & 3];
+  const Ctl* c = a.ctl + (p & 3);
+  int nb = 0;
+  if (prev.mode == kSettle)
+    for (int k = 0; k < kClaimParts; ++k) nb = max(nb, c->nbk[k * kClaimStride]);
+  return next_phase(prev, c->qlen, nb, a.n);
+}
+
+// Candidate relaxations of one step, packed to the low lanes through the
+// wave's LDS area (s_pair: 2 * K * 64 ints) so that each lane issues one
+// atomic.  Light phases queue the winners for the next light phase; the
+// settle phase keeps the smallest winning distance of a vertex with arcs.
+template <int K>
+__device__ __forceinline__ void relax_packed_d(const DArgs& a, const int* v, const int* alt, const bool* want,
+                                               bool settle, int label, int32_t* s_pair, Sink& out, int& nbmin) {
+  const int lane = lane_id();
+  int total = 0;
+#pragma unroll
+  for (int t = 0; t < K; ++t) {
+    const uint64_t m = ballot(want[t]);
+    if (want[t]) {
+      const int i = total + mbcnt(m);
+      s_pair[2 * i] = v[t];
+      s_pair[2 * i + 1] = alt[t];
+    }
+    total += __popcll(m);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (int r0 = 0; r0 < total; r0 += SMDP_WAVE) {
+    const int i = r0 + lane;
+    int vv = 0;
+    bool push = false;
+    if (i < total) {
+      vv = s_pair[2 * i];
+      const int aa = s_pair[2 * i + 1];
+      ++out.tries;
+      const int old = atomicMin(&a.dist[vv], aa);
+      // read while the atomic is in flight (the stamp is only a filter)
+      const bool arcs = d_has_arcs(a, vv);
+      const int st = settle ? 0 : a.stamp[vv];
+      if (aa < old) {
+        ++out.wins;
+        if (arcs) {
+          if (settle) nbmin = min(nbmin, aa);
+          else if (st != label) push = atomicExch(&a.stamp[vv], label) != label;
+        }
+      }
+    }
+    out.push(push, vv);
+  }
+  // s_pair is written again by the next step
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The settle pass's smallest distance beyond the bucket, over the workgroup,
+// into one of the kClaimParts next-bucket copies of slot `c`.  All threads.
+__device__ __forceinline__ void store_next_bucket(Ctl* c, int nbmin, int shift, int* s_red) {
+  int m = nbmin;
+#pragma unroll
+  for (int off = SMDP_WAVE / 2; off > 0; off >>= 1) m = min(m, __shfl_xor(m, off));
+  if (lane_id() == 0) s_red[threadIdx.x / SMDP_WAVE] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < SMDP_WAVES_PER_WG; ++k) m = min(m, s_red[k]);
+    if (m != INT32_MAX) atomicMax(&c->nbk[(blockIdx.x % kClaimParts) * kClaimStride], INT32_MAX - (m >> shift));
+  }
+}
+
+__device__ __forceinline__ void log_phase(const DArgs& a, int p, int L, unsigned long long wave_arcs) {
+  if (!a.rlog || p >= kRoundLogCap) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && L >= 0) a.rlog[2 * p] = (unsigned long long)L;
+  if (lane_id() == 0 && wave_arcs) atomicAdd(&a.rlog[2 * p + 1], wave_arcs);
+}
+
+// flags[0]: set when a grid barrier gave up waiting; flags[1]: the first
+// phase a persistent launch did not run.
+__global__ void k_delta_init(int32_t* dist, int32_t* stamp, int32_t n, int32_t src, Ctl* ctl, Counters* cnt,
+                             uint32_t* bar, int32_t* flags) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    dist[i] = (i == src) ? 0 : SMDP_SSSP_INF;
+    stamp[i] = -1;
+  }
+  if (blockIdx.x == 0) {
+    int32_t* c = reinterpret_cast<int32_t*>(ctl);
+    for (int i = threadIdx.x; i < 4 * kCtlInts; i += blockDim.x) c[i] = 0;
+    unsigned long long* k = reinterpret_cast<unsigned long long*>(cnt);
+    for (int i = threadIdx.x; i < (int)(kCntParts * sizeof(Counters) / 8); i += blockDim.x) k[i] = 0;
+    for (int i = threadIdx.x; i < kBarLines * 32; i += blockDim.x) bar[i] = 0;
+    if (threadIdx.x < 4) flags[threadIdx.x] = 0;
+  }
+}
+
+// Fresh barrier lines before a further persistent launch of the same solve.
+__global__ void k_delta_bar_reset(uint32_t* bar) {
+  for (int i = threadIdx.x; i < kBarLines * 32; i += blockDim.x) bar[i] = 0;
+}
+
+constexpr int kSweepPer = 16;                    // vertices per thread per sweep batch
+constexpr int kSweepGroup = 2;                   // of which one compaction takes this many
+constexpr int kSweepList = kSweepGroup * SMDP_WG;
+
+// LDS of the phase kernels.
+struct DShared {
+  int tile, tend, tot, ok;
+  int32_t start[SMDP_WG];  // first arc of each entry of the tile
+  int32_t du[SMDP_WG];     // its distance
+  int32_t off[SMDP_WG];    // exclusive scan of the arcs expanded in the tile
+  int32_t scan[SMDP_WAVES_PER_WG];
+  int32_t list[kSweepList];  // sweep: the selected vertices of a group
+  int32_t ldu[kSweepList];   // and their distances
+  int32_t out[SMDP_WAVES_PER_WG * kOutCap];
+  int32_t pair[SMDP_WAVES_PER_WG * 2 * kDIlp * SMDP_WAVE];
+  int32_t flush[SMDP_WAVES_PER_WG + 1];
+  unsigned long long red[SMDP_WAVES_PER_WG * 8];
+};
+
+// Per-workgroup state of one phase's expansion.
+struct DTile {
+  int32_t* s_pair;   // this wave's 2 * kDIlp * SMDP_WAVE ints
+  Ctl* cin;          // the phase's input slot (grid queue, claims)
+  Ctl* cout;         // the phase's output slot (light queue, next bucket)
+  int split;
+  int lim;
+  int label;
+  bool settle;
+  unsigned long long edges, skipped, pg, verts;
+  int nbmin;
+};
+
+// One phase's state in this workgroup; out must be initialised on the
+// phase's output queue.
+__device__ __forceinline__ void tile_init(const DArgs& a, const DPhase& d, int p, DShared& sh, DTile& t) {
+  t.s_pair = sh.pair + (threadIdx.x / SMDP_WAVE) * 2 * kDIlp * SMDP_WAVE;
+  t.cin = a.ctl + (p & 3);
+  t.cout = a.ctl + ((p + 1) & 3);
+  t.split = (d.mode == kLight && d.nin < a.small_q) ? min(a.split, kDSmallSplit) : a.split;
+  t.lim = (d.bucket + 1) << a.shift;  // first distance beyond the bucket
+  t.label = p + 1;
+  t.settle = d.mode == kSettle;
+  t.nbmin = INT32_MAX;
+}
+
+// Relaxes the arcs [s_start + j - s_off] of the NE staged entries,
+// edge-balanced: kDIlp arcs per thread per step, all loads of a step issued
+// before its atomics.  Every thread of the workgroup calls it.
+template <int NE>
+__device__ __forceinline__ void relax_staged(const DArgs& a, DTile& t, const int32_t* s_start, const int32_t* s_du,
+                                             const int32_t* s_off, int tot, Sink& out) {
+  for (int jb = 0; jb < tot; jb += kDIlp * SMDP_WG) {
+    int j[kDIlp], own[kDIlp], k[kDIlp], v[kDIlp], alt[kDIlp], seen[kDIlp];
+    bool rel[kDIlp], want[kDIlp];
+#pragma unroll
+    for (int q = 0; q < kDIlp; ++q) j[q] = min(jb + (int)threadIdx.x + q * SMDP_WG, tot - 1);
+    tile_owners<kDIlp, NE>(s_off, j, own);
+#pragma unroll
+    for (int q = 0; q < kDIlp; ++q) k[q] = s_start[own[q]] + (j[q] - s_off[own[q]]);
+#pragma unroll
+    for (int q = 0; q < kDIlp; ++q) v[q] = stream_load(a.col + k[q]);
+#pragma unroll
+    for (int q = 0; q < kDIlp; ++q) alt[q] = s_du[own[q]] + stream_load(a.w + k[q]);
+#pragma unroll
+    for (int q = 0; q < kDIlp; ++q) {
+      const bool ok = jb + (int)threadIdx.x + q * SMDP_WG < tot;
+      rel[q] = ok && (t.settle ? alt[q] >= t.lim : alt[q] < t.lim);
+      t.skipped += ok && !rel[q];
+      t.edges += rel[q];
+    }
+    // only arcs this pass relaxes gather their distance
+#pragma unroll
+    for (int q = 0; q < kDIlp; ++q) seen[q] = rel[q] ? a.dist[v[q]] : 0;
+#pragma unroll
+    for (int q = 0; q < kDIlp; ++q) want[q] = rel[q] && alt[q] < seen[q];
+    relax_packed_d<kDIlp>(a, v, alt, want, t.settle, t.label, t.s_pair, out, t.nbmin);
+  }
+}
+
+// Expands one tile: this thread's entry is (u, s, deg, du) with deg = 0 for
+// none.  Entries of >= split arcs go to the grid queue as chunks; the others
+// are relaxed edge-balanced.  Every thread of the workgroup calls it.
+__device__ __forceinline__ void expand_tile(const DArgs& a, DShared& sh, DTile& t, int u, int s, int deg, int du,
+                                            Sink& out) {
+  const int lane = lane_id();
+  const int nch = deg >= t.split ? (deg + a.chunk - 1) / a.chunk : 0;
+  const int incl = wave_inclusive_scan(nch);
+  const int wtot = __shfl(incl, SMDP_WAVE - 1);
+  if (wtot > 0) {
+    int b0 = 0;
+    if (lane == 0) b0 = atomicAdd(&t.cin->hlen, wtot);
+    b0 = __shfl(b0, 0);
+    const int off = b0 + incl - nch;
+    for (int c = 0; c < nch; ++c) a.hq[off + c] = Chunk{u, s + c * a.chunk};
+    t.pg += nch;
+  }
+  sh.start[threadIdx.x] = s;
+  sh.du[threadIdx.x] = du;
+  int tot;
+  const int ex = wg_exclusive_scan(nch ? 0 : deg, sh.scan, &tot);
+  sh.off[threadIdx.x] = ex;
+  __syncthreads();
+  relax_staged<SMDP_WG>(a, t, sh.start, sh.du, sh.off, tot, out);
+  __syncthreads();
+}
+
+// Pass 1 of phase d (all threads of the workgroup).  Sweeps: tiles of
+// SMDP_WG consecutive vertices dealt grid-stride (the renumbering puts the
+// rows with most arcs first, so contiguous ranges per workgroup would give
+// the first workgroups most of the work), kSweepPer tiles read at once (most
+// hold no vertex of the bucket and cost one round of loads); the selected
+// vertices are compacted in LDS and expanded in tiles.  Light phases: tiles
+// of SMDP_WG queue entries claimed from the phase's counters.
+__device__ __forceinline__ void phase_tiles(const DArgs& a, const DPhase& d, int p, DShared& sh, DTile& t,
+                                            Sink& out) {
+  const int L = d.nin;
+  const int bucket = d.bucket;
+  const bool settle = d.mode == kSettle;
+  if (d.mode != kLight) {
+    const int ntiles = (L + SMDP_WG - 1) / SMDP_WG;
+    for (int t0 = blockIdx.x; t0 < ntiles; t0 += kSweepPer * gridDim.x) {
+      int dv[kSweepPer];
+      int cnt = 0;
+#pragma unroll
+      for (int k = 0; k < kSweepPer; ++k) {
+        const int v = (t0 + k * (int)gridDim.x) * SMDP_WG + (int)threadIdx.x;
+        dv[k] = SMDP_SSSP_INF;
+        if (t0 + k * (int)gridDim.x < ntiles && v < L && d_has_arcs(a, v)) dv[k] = a.dist[v];
+      }
+#pragma unroll
+      for (int k = 0; k < kSweepPer; ++k) {
+        const int b = dv[k] >> a.shift;
+        if (dv[k] != SMDP_SSSP_INF) {
+          if (b == bucket) ++cnt;
+          else if (settle && b > bucket) t.nbmin = min(t.nbmin, dv[k]);
+        }
+      }
+      int total;
+      (void)wg_exclusive_scan(cnt, sh.scan, &total);
+      if (total == 0) continue;
+      for (int g = 0; g < kSweepPer; g += kSweepGroup) {
+        int c = 0;
+#pragma unroll
+        for (int k = 0; k < kSweepGroup; ++k) c += dv[g + k] != SMDP_SSSP_INF && (dv[g + k] >> a.shift) == bucket;
+        int gtot;
+        int pos = wg_exclusive_scan(c, sh.scan, &gtot);
+        if (gtot == 0) continue;
+#pragma unroll
+        for (int k = 0; k < kSweepGroup; ++k) {
+          if (dv[g + k] != SMDP_SSSP_INF && (dv[g + k] >> a.shift) == bucket) {
+            sh.list[pos] = (t0 + (g + k) * (int)gridDim.x) * SMDP_WG + (int)threadIdx.x;
+            sh.ldu[pos] = dv[g + k];
+            ++pos;
+          }
+        }
+        __syncthreads();
+        for (int e0 = 0; e0 < gtot; e0 += SMDP_WG) {
+          const int i = e0 + (int)threadIdx.x;
+          int u = 0, s = 0, deg = 0, du = 0;
+          if (i < gtot) {
+            u = sh.list[i];
+            du = sh.ldu[i];
+            s = a.row[u];
+            deg = (settle ? a.row[u + 1] : a.lsplit[u]) - s;
+            ++t.verts;
+          }
+          expand_tile(a, sh, t, u, s, deg, du, out);
+        }
+      }
+    }
+  } else {
+    const int32_t* qin = (p & 1) ? a.q1 : a.q0;
+    const int ntiles = (L + SMDP_WG - 1) / SMDP_WG;
+    int part = 0;  // claim cursor of thread 0
+    for (;;) {
+      if (threadIdx.x == 0) {
+        int f = 0, e = 0;
+        sh.tile = claim_items(t.cin->tiles, ntiles, 1, part, &f, &e) ? f : -1;
+        sh.tend = e;
+      }
+      __syncthreads();
+      const int first = sh.tile, tend = sh.tend;
+      __syncthreads();
+      if (first < 0) break;
+      for (int tile = first; tile < tend; ++tile) {
+        const int i = tile * SMDP_WG + (int)threadIdx.x;
+        int u = 0, s = 0, deg = 0, du = 0;
+        if (i < L) {
+          u = qin[i];
+          s = a.row[u];
+          deg = a.lsplit[u] - s;
+          du = a.dist[u];
+          ++t.verts;
+        }
+        expand_tile(a, sh, t, u, s, deg, du, out);
+      }
+    }
+  }
+}
+
+// Pass 2 of phase d (all threads): the H chunks of the phase's grid queue,
+// kDDrainTile entries per workgroup pass (fewer while the tiles would not
+// reach every workgroup), expanded edge-balanced.  Returns the arcs relaxed.
+__device__ __forceinline__ unsigned long long phase_chunks(const DArgs& a, const DPhase& d, int H, DShared& sh,
+                                                           DTile& t, Sink& out) {
+  int ept = kDDrainTile;
+  while (ept > 1 && (H + ept / 2 - 1) / (ept / 2) <= (int)gridDim.x) ept >>= 1;
+  const int ntiles = (H + ept - 1) / ept;
+  const unsigned long long e0 = t.edges;
+  int part = 0;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      int f = 0, e = 0;
+      sh.tile = claim_items(t.cin->chunks, ntiles, 1, part, &f, &e) ? f : -1;
+    }
+    __syncthreads();
+    const int tile = sh.tile;
+    if (tile < 0) break;
+    if (threadIdx.x < SMDP_WAVE) {
+      const int idx = tile * ept + (int)threadIdx.x;
+      int len = 0, st = 0, du = 0;
+      if ((int)threadIdx.x < ept && idx < H) {
+        const Chunk c = a.hq[idx];
+        const int vend = t.settle ? a.row[c.v + 1] : a.lsplit[c.v];
+        const int end = vend - c.first > a.chunk ? c.first + a.chunk : vend;
+        len = end - c.first;
+        st = c.first;
+        du = a.dist[c.v];
+      }
+      const int incl = wave_inclusive_scan(len);
+      if ((int)threadIdx.x < kDDrainTile) {
+        sh.off[threadIdx.x] = incl - len;
+        sh.start[threadIdx.x] = st;
+        sh.du[threadIdx.x] = du;
+      }
+      if (threadIdx.x == kDDrainTile - 1) sh.tot = incl;
+    }
+    __syncthreads();
+    relax_staged<kDDrainTile>(a, t, sh.start, sh.du, sh.off, sh.tot, out);
+    __syncthreads();
+  }
+  return t.edges - e0;
+}
+
+// End of a pass: the workgroup's staged queue entries go out, and a settle
+// pass records its next-bucket minimum.  All threads.
+__device__ __forceinline__ void end_pass(const DArgs& a, DShared& sh, DTile& t, Sink& out) {
+  wg_flush(out, sh.flush);
+  if (t.settle) store_next_bucket(t.cout, t.nbmin, a.shift, sh.scan);
+}
+
+// The workgroup's work counters, one atomic per counter.  All threads.
+__device__ __forceinline__ void phase_counters(const DArgs& a, DShared& sh, DTile& t, Sink& out,
+                                               unsigned long long e2) {
+  const unsigned long long v[8] = {t.edges, t.verts, out.pushes, t.pg, out.tries, out.wins, t.skipped, e2};
+  const int f[8] = {kCntEdges, kCntVerts, kCntPush0, kCntPush3, kCntTries, kCntWins, kCntSkipped, kCntEdges2};
+  wg_add_counters<8>(a.cnt, v, f, sh.red);
+}
+
+__device__ __forceinline__ void phase_begin(const DArgs& a, const DPhase& d, int p) {
+  if (blockIdx.x == 0) {
+    ctl_clear(a.ctl, (p + 2) & 3);  // phase p + 1's output slot; no kernel of phase p uses it
+    if (threadIdx.x == 0) {
+      a.ph[p & 3] = d;
+      if (d.mode != kDone) atomicMax(&cnt_of(a.cnt)->rounds, (unsigned long long)(p + 1));
+    }
+  }
+}
+
+// One phase's pass 1 (launched per phase).
+__global__ __launch_bounds__(SMDP_WG) void k_delta_step(DArgs a, int p) {
+  const DPhase d = phase_of(a, p);
+  phase_begin(a, d, p);
+  if (d.mode == kDone) return;
+  if (d.mode == kLight && (int)blockIdx.x * SMDP_WG >= d.nin) return;  // more workgroups than light tiles
+  __shared__ DShared sh;
+  DTile t;
+  tile_init(a, d, p, sh, t);
+  t.edges = t.skipped = t.pg = t.verts = 0;
+  Sink out;
+  out.init(sh.out + (threadIdx.x / SMDP_WAVE) * kOutCap, (p & 1) ? a.q0 : a.q1, &t.cout->qlen);
+  phase_tiles(a, d, p, sh, t, out);
+  log_phase(a, p, d.mode == kLight ? d.nin : -1, wave_sum(t.edges));
+  end_pass(a, sh, t, out);
+  phase_counters(a, sh, t, out, 0);
+}
+
+// The same phase's pass 2: its grid chunks.
+__global__ __launch_bounds__(SMDP_WG) void k_delta_drain(DArgs a, int p) {
+  const DPhase d = a.ph[p & 3];
+  if (d.mode == kDone) return;
+  const int H = a.ctl[p & 3].hlen;
+  int ept = kDDrainTile;
+  while (ept > 1 && (H + ept / 2 - 1) / (ept / 2) <= (int)gridDim.x) ept >>= 1;
+  if ((int)blockIdx.x >= (H + ept - 1) / ept) return;
+  __shared__ DShared sh;
+  DTile t;
+  tile_init(a, d, p, sh, t);
+  t.edges = t.skipped = t.pg = t.verts = 0;
+  Sink out;
+  out.init(sh.out + (threadIdx.x / SMDP_WAVE) * kOutCap, (p & 1) ? a.q0 : a.q1, &t.cout->qlen);
+  const unsigned long long e2 = phase_chunks(a, d, H, sh, t, out);
+  log_phase(a, p, -1, wave_sum(e2));
+  end_pass(a, sh, t, out);
+  phase_counters(a, sh, t, out, e2);
+}
+
+// Phases [p0, pend) in one launch whose workgroups are all resident, with a
+// grid barrier after each pass (the chunk pass only when the phase pushed
+// chunks).  Every workgroup computes the same phase descriptor after each
+// barrier, so all take the same barriers and stop at the same phase.
+// flags[1] = the first phase not run (the finished solve's kDone phase, or
+// pend); flags[0] = 1 when a barrier gave up waiting.
+__global__ __launch_bounds__(SMDP_WG) void k_delta_run(DArgs a, int p0, int pend, uint32_t* bar, int32_t* flags) {
+  __shared__ DShared sh;
+  DTile t;
+  t.edges = t.skipped = t.pg = t.verts = 0;
+  unsigned long long e2 = 0;
+  Sink out;
+  out.init(sh.out + (threadIdx.x / SMDP_WAVE) * kOutCap, a.q0, &a.ctl[0].qlen);
+  uint32_t epoch = 0;
+  int p = p0;
+  for (; p < pend; ++p) {
+    if (__hip_atomic_load(&flags[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+    const DPhase d = phase_of(a, p);
+    phase_begin(a, d, p);
+    if (d.mode == kDone) break;
+    tile_init(a, d, p, sh, t);
+    out.q = (p & 1) ? a.q0 : a.q1;
+    out.qlen = &t.cout->qlen;
+    const unsigned long long eb = t.edges;
+    phase_tiles(a, d, p, sh, t, out);
+    end_pass(a, sh, t, out);
+    if (!grid_barrier(bar, ++epoch, &sh.ok)) {
+      if (threadIdx.x == 0) atomicExch(&flags[0], 1);
+      break;
+    }
+    const int H = __hip_atomic_load(&t.cin->hlen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (H > 0) {
+      e2 += phase_chunks(a, d, H, sh, t, out);
+      end_pass(a, sh, t, out);
+      if (!grid_barrier(bar, ++epoch, &sh.ok)) {
+        if (threadIdx.x == 0) atomicExch(&flags[0], 1);
+        break;
+      }
+    }
+    log_phase(a, p, d.mode == kLight ? d.nin : -1, wave_sum(t.edges - eb));
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) flags[1] = p;
+  phase_counters(a, sh, t, out, e2);
+}
+
+// Rows of the renumbered copy partitioned in place of order: light arcs (w <
+// delta) first, heavy arcs after, each in their original order; lsplit[u] =
+// first heavy arc of row u.  One wave per row.
+__global__ __launch_bounds__(SMDP_WG) void k_partition_rows(const int32_t* __restrict__ row,
+                                                            const int32_t* __restrict__ col_in,
+                                                            const int32_t* __restrict__ w_in, int32_t* col_out,
+                                                            int32_t* w_out, int32_t* lsplit, int32_t n,
+                                                            int32_t delta) {
+  const int lane = lane_id();
+  const int64_t nw = (int64_t)gridDim.x * SMDP_WAVES_PER_WG;
+  for (int64_t u = (int64_t)blockIdx.x * SMDP_WAVES_PER_WG + threadIdx.x / SMDP_WAVE; u < n; u += nw) {
+    const int s = row[u], e = row[u + 1];
+    int nl = 0;
+    for (int b = s; b < e; b += SMDP_WAVE) {
+      const int k = b + lane;
+      nl += __popcll(ballot(k < e && w_in[k] < delta));
+    }
+    int lo = s, hi = s + nl;
+    for (int b = s; b < e; b += SMDP_WAVE) {
+      const int k = b + lane;
+      const bool in = k < e;
+      const int wk = in ? w_in[k] : 0;
+      const bool light = in && wk < delta;
+      const uint64_t ml = ballot(light), mh = ballot(in && !light);
+      if (in) {
+        const int pos = light ? lo + mbcnt(ml) : hi + mbcnt(mh);
+        col_out[pos] = col_in[k];
+        w_out[pos] = wk;
+      }
+      lo += __popcll(ml);
+      hi += __popcll(mh);
+    }
+    if (lane == 0) lsplit[u] = s + nl;
+  }
+}
+
+// Partitions the renumbered copy's rows for bucket width delta (once per
+// delta) and allocates the phase descriptors.
+static int ensure_buckets(smdp_graph* g, Relabel* rl, int delta) {
+  hipStream_t s = g->ctx->stream;
+  // 4 phase descriptors, the persistent launch's barrier lines and flags
+  if (!rl->phase) SMDP_CHECK(alloc_once(&rl->phase, (int64_t)4 * kPhaseInts + kBarLines * 32 + 32));
+  if (!rl->hctl && hipHostMalloc(&rl->hctl, sizeof(Ctl) + sizeof(DPhase), hipHostMallocDefault) != hipSuccess) {
+    rl->hctl = nullptr;
+    return SMDP_ERR_NOMEM;
+  }
+  if (rl->lsplit && rl->lsplit_delta == delta) return SMDP_OK;
+  SMDP_CHECK(alloc_once(&rl->lsplit, std::max<int64_t>(1, g->n)));
+  const int64_t m = std::max<int64_t>(1, g->m);
+  int32_t *col2 = nullptr, *w2 = nullptr;
+  if (hipMalloc((void**)&col2, sizeof(int32_t) * m) != hipSuccess) return SMDP_ERR_NOMEM;
+  if (hipMalloc((void**)&w2, sizeof(int32_t) * m) != hipSuccess) {
+    (void)hipFree(col2);
+    return SMDP_ERR_NOMEM;
+  }
+  const int64_t blocks = std::min<int64_t>(((int64_t)g->n + SMDP_WAVES_PER_WG - 1) / SMDP_WAVES_PER_WG, 16384);
+  hipLaunchKernelGGL(k_partition_rows, dim3((unsigned)std::max<int64_t>(1, blocks)), dim3(SMDP_WG), 0, s, rl->row,
+                     rl->col, rl->w, col2, w2, rl->lsplit, g->n, delta);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+    (void)hipFree(col2);
+    (void)hipFree(w2);
+    return SMDP_ERR_DEVICE;
+  }
+  (void)hipFree(rl->col);
+  (void)hipFree(rl->w);
+  rl->col = col2;
+  rl->w = w2;
+  rl->lsplit_delta = delta;
+  return SMDP_OK;
+}
+
+// Bucketed SSSP from src (an id of the renumbered copy); results stay in
+// g->dist in the copy's numbering.  *ms is the device time from the first to
+// the last launch, *launches the kernels launched.  With profiling on, fills
+// the context's per-phase log.  The copy must have no negative weight.
+int run_delta(smdp_graph* g, Relabel* rl, int32_t src, const smdp_params& p, int grid, float* ms_out,
+              int* launches_out) {
+  smdp_ctx* ctx = g->ctx;
+  if (rl->minw < 0 || (g->m > 0 && !g->w)) return SMDP_ERR_ARG;
+  const int delta = p.delta > 0 ? p.delta : 64;
+  SMDP_CHECK(ensure_buckets(g, rl, delta));
+  SMDP_CHECK(build_hasout(g, rl->row, &rl->hasout));
+  int shift = 0;
+  while ((1 << shift) < delta) ++shift;
+
+  DArgs a;
+  a.row = rl->row;
+  a.col = rl->col;
+  a.w = rl->w;
+  a.lsplit = rl->lsplit;
+  a.hasout = rl->hasout;
+  a.dist = g->dist;
+  a.stamp = g->stamp;
+  a.q0 = g->q[0];
+  a.q1 = g->q[1];
+  a.hq = g->hq;
+  a.ctl = g->ctl;
+  a.ph = reinterpret_cast<DPhase*>(rl->phase);
+  a.cnt = g->cnt;
+  a.rlog = ctx->profiling ? ctx->rlog : nullptr;
+  a.n = g->n;
+  a.shift = shift;
+  a.split = p.split_threshold;
+  a.chunk = p.chunk;
+  a.small_q = grid * SMDP_WG;
+  hipStream_t s = ctx->stream;
+  if (a.rlog) SMDP_HIP(hipMemsetAsync(a.rlog, 0, sizeof(unsigned long long) * 2 * kRoundLogCap, s));
+  prof_reset(ctx);
+
+  int32_t* phase_buf = rl->phase;
+  uint32_t* bar = reinterpret_cast<uint32_t*>(phase_buf + 4 * kPhaseInts);
+  int32_t* flags = phase_buf + 4 * kPhaseInts + kBarLines * 32;
+  // The persistent launch needs every workgroup resident at once: its grid
+  // is capped at what the device holds (occupancy x CUs).
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_delta_run), SMDP_WG,
+                                                   0) != hipSuccess)
+    per_cu = 0;
+  const int pgrid = std::min(grid, per_cu * ctx->cus);
+  bool persistent = p.persistent && per_cu >= 1;
+
+  int launches = 0;
+  int ph = 0;
+  for (int attempt = 0;; ++attempt) {
+    SMDP_HIP(hipEventRecord(ctx->ev_begin, s));
+    hipLaunchKernelGGL(k_delta_init, dim3((unsigned)std::min<int64_t>(((int64_t)g->n + 255) / 256, 4096)), dim3(256),
+                       0, s, g->dist, g->stamp, g->n, src, g->ctl, g->cnt, bar, flags);
+    ++launches;
+    ph = 0;
+    if (!persistent) break;
+    // all phases in launches of up to kRunPhases phases each
+    a.small_q = pgrid * SMDP_WG;
+    int32_t* hf = ctx->pinned;
+    bool gave_up = false;
+    for (;;) {
+      SMDP_CHECK(prof_mark(ctx, 0));
+      hipLaunchKernelGGL(k_delta_run, dim3(pgrid), dim3(SMDP_WG), 0, s, a, ph, ph + kRunPhases, bar, flags);
+      SMDP_CHECK(prof_mark(ctx, 0));
+      ++launches;
+      SMDP_HIP(hipGetLastError());
+      SMDP_HIP(hipMemcpyAsync(hf, flags, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      SMDP_HIP(hipStreamSynchronize(s));
+      if (hf[0]) {
+        gave_up = true;
+        break;
+      }
+      const int next = hf[1];
+      if (next < ph + kRunPhases) {
+        ph = next;
+        break;
+      }
+      ph = next;
+      if (ph >= p.max_rounds) return SMDP_ERR_OVERFLOW;
+      hipLaunchKernelGGL(k_delta_bar_reset, dim3(1), dim3(256), 0, s, bar);
+      ++launches;
+    }
+    if (!gave_up) break;
+    // A barrier gave up: the workgroups were not all resident (another
+    // process on the device).  Solve again with one launch per pass.
+    if (attempt > 0) return SMDP_ERR_DEVICE;
+    persistent = false;
+    prof_reset(ctx);
+  }
+  if (!persistent) {
+    a.small_q = grid * SMDP_WG;
+    DPhase* hph = reinterpret_cast<DPhase*>(rl->hctl);
+    Ctl* hc = reinterpret_cast<Ctl*>(hph + 1);
+    int batch = std::max(8, p.batch_rounds);
+    for (;;) {
+      for (int j = 0; j < batch; ++j, ++ph) {
+        SMDP_CHECK(prof_mark(ctx, 0));
+        hipLaunchKernelGGL(k_delta_step, dim3(grid), dim3(SMDP_WG), 0, s, a, ph);
+        SMDP_CHECK(prof_mark(ctx, 0));
+        SMDP_CHECK(prof_mark(ctx, 1));
+        hipLaunchKernelGGL(k_delta_drain, dim3(grid), dim3(SMDP_WG), 0, s, a, ph);
+        SMDP_CHECK(prof_mark(ctx, 1));
+        launches += 2;
+      }
+      SMDP_HIP(hipGetLastError());
+      // the last phase's descriptor and what it appended decide whether another follows
+      SMDP_HIP(hipMemcpyAsync(hph, a.ph + ((ph - 1) & 3), sizeof(DPhase), hipMemcpyDeviceToHost, s));
+      SMDP_HIP(hipMemcpyAsync(hc, g->ctl + (ph & 3), sizeof(Ctl), hipMemcpyDeviceToHost, s));
+      SMDP_HIP(hipStreamSynchronize(s));
+      int nb = 0;
+      for (int k = 0; k < kClaimParts; ++k) nb = std::max(nb, hc->nbk[k * kClaimStride]);
+      if (hph->mode == kDone || next_phase(*hph, hc->qlen, nb, g->n).mode == kDone) break;
+      if (ph >= p.max_rounds) return SMDP_ERR_OVERFLOW;
+      batch = std::min(batch * 2, 32);
+    }
+  }
+  SMDP_HIP(hipEventRecord(ctx->ev_end, s));
+  SMDP_HIP(hipEventSynchronize(ctx->ev_end));
+  float ms = 0;
+  SMDP_HIP(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end));
+  if (a.rlog) {
+    // one log line per phase, timed over its expand and drain launches
+    const int k = std::min(ph, kRoundLogCap);
+    std::vector<unsigned long long> h(2 * (size_t)k);
+    if (k) SMDP_HIP(hipMemcpy(h.data(), ctx->rlog, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost));
+    ctx->log_qlen.assign(k, 0);
+    ctx->log_arcs.assign(k, 0);
+    ctx->log_ms.assign(k, 0.0);
+    for (int i = 0; i < k; ++i) {
+      ctx->log_qlen[i] = (int64_t)h[2 * i];
+      ctx->log_arcs[i] = (int64_t)h[2 * i + 1];
+      if (persistent) continue;  // phases share one launch: no time per phase
+      for (size_t e = 0; e < 4; e += 2) {
+        const size_t at = (size_t)i * 4 + e;
+        if (at + 1 >= ctx->prof_ev.size()) break;
+        float t = 0;
+        SMDP_HIP(hipEventElapsedTime(&t, ctx->prof_ev[at], ctx->prof_ev[at + 1]));
+        ctx->log_ms[i] += t;
+      }
+    }
+  }
+  *ms_out = ms;
+  *launches_out = launches;
+  return SMDP_OK;
+}
+
+}  // namespace smdp

@@ -27,35 +27,17 @@
 #include <algorithm>
 
 #include "device_common.h"
+#include "frontier.h"
 #include "smdp_internal.h"
 
 namespace smdp {
 
 constexpr int kWaveQ = 256;   // per-wave LDS queue capacity (SMDP_SOL_WAVE)
 constexpr int kBlockQ = 1024; // per-workgroup LDS queue capacity (SMDP_SOL_BLOCK)
-constexpr int kOutCap = 512;  // per-wave LDS staging of next-queue entries
 constexpr int kIlp = 4;       // arcs per thread per step in the tuned kernels
 constexpr int kTileGrab = 1;  // frontier tiles claimed per atomic
 constexpr int kSmallSplit = 32;  // split degree of the tuned solution when the queue is short
 constexpr int kDrainTile = 8;    // grid-queue entries per drain tile (a power of two, at most 64)
-
-// Next-queue output of one wave: LDS staging + the global queue it flushes to.
-struct Sink {
-  WaveOut<kOutCap> wo;
-  int32_t* q;
-  int32_t* qlen;
-  unsigned long long pushes;
-  unsigned long long tries;  // atomicMin attempts (plain read said "improves")
-  unsigned long long wins;   // atomicMin that lowered the value
-  __device__ __forceinline__ void init(int32_t* lds_region, int32_t* queue, int32_t* counter) {
-    wo.init(lds_region);
-    q = queue;
-    qlen = counter;
-    pushes = tries = wins = 0;
-  }
-  __device__ __forceinline__ void push(bool pred, int32_t v) { pushes += wo.push(pred, v, q, qlen); }
-  __device__ __forceinline__ void flush() { wo.flush(q, qlen); }
-};
 
 struct FArgs {
   const int32_t* __restrict__ row;
@@ -176,32 +158,6 @@ __device__ __forceinline__ void relax_packed(const FArgs& a, const int* v, const
 }
 
 __device__ __forceinline__ void clear_slot(Ctl* ctl, int r) { ctl_clear(ctl, (r + 2) & 3); }
-
-__device__ __forceinline__ void flush_counters(Counters* cnt, unsigned long long edges, unsigned long long verts,
-                                               unsigned long long p0, unsigned long long p1,
-                                               unsigned long long p2, unsigned long long p3,
-                                               Sink& out) {
-  out.flush();
-  unsigned long long tries = wave_sum(out.tries), wins = wave_sum(out.wins);
-  if (lane_id() == 0) {
-    if (tries) atomicAdd(&cnt_of(cnt)->atom_try, tries);
-    if (wins) atomicAdd(&cnt_of(cnt)->atom_win, wins);
-  }
-  edges = wave_sum(edges);
-  verts = wave_sum(verts);
-  p0 = wave_sum(p0);
-  p1 = wave_sum(p1);
-  p2 = wave_sum(p2);
-  p3 = wave_sum(p3);
-  if (lane_id() == 0) {
-    if (edges) atomicAdd(&cnt_of(cnt)->edges, edges);
-    if (verts) atomicAdd(&cnt_of(cnt)->verts, verts);
-    if (p0) atomicAdd(&cnt_of(cnt)->push[0], p0);
-    if (p1) atomicAdd(&cnt_of(cnt)->push[1], p1);
-    if (p2) atomicAdd(&cnt_of(cnt)->push[2], p2);
-    if (p3) atomicAdd(&cnt_of(cnt)->push[3], p3);
-  }
-}
 
 // Wave-cooperative expansion of arcs [s, e) of one vertex with cost du.
 template <bool BFS>
@@ -669,16 +625,6 @@ static int collect_round_log(smdp_ctx* ctx, int rounds, bool two_phase) {
 // 4.09 ms (median of 11).  The solve runs on the copy and results are
 // gathered back to the caller's numbering when read.
 
-namespace {
-struct Relabel {
-  int32_t* row = nullptr;
-  int32_t* col = nullptr;
-  int32_t* w = nullptr;
-  int32_t* newid = nullptr;          // caller id -> copy id (device)
-  uint32_t* hasout = nullptr;        // bit v set when row v of the copy has arcs
-  std::vector<int32_t> h_newid;      // the same on the host (source mapping)
-};
-}  // namespace
 
 __global__ void k_unpermute(const int32_t* __restrict__ d, const int32_t* __restrict__ newid, int64_t count,
                             int32_t* out) {
@@ -689,9 +635,10 @@ __global__ void k_unpermute(const int32_t* __restrict__ d, const int32_t* __rest
 void relabel_free(smdp_graph* g) {
   Relabel* rl = static_cast<Relabel*>(g->relabel);
   if (!rl) return;
-  void* ptrs[] = {rl->row, rl->col, rl->w, rl->newid, rl->hasout};
+  void* ptrs[] = {rl->row, rl->col, rl->w, rl->newid, rl->hasout, rl->lsplit, rl->phase};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
+  if (rl->hctl) (void)hipHostFree(rl->hctl);
   delete rl;
   g->relabel = nullptr;
 }
@@ -718,6 +665,8 @@ static int ensure_relabel(smdp_graph* g, Relabel** out) {
   if (m && g->w) SMDP_HIP(hipMemcpy(w.data(), g->w, sizeof(int32_t) * m, hipMemcpyDeviceToHost));
   std::vector<int32_t> indeg((size_t)n, 0);
   for (int64_t e = 0; e < m; ++e) ++indeg[(size_t)col[(size_t)e]];
+  int32_t minw = 0;
+  if (g->w && m) minw = *std::min_element(w.begin(), w.end());
   int32_t maxd = 0;
   for (int32_t d : indeg) maxd = std::max(maxd, d);
   // counting sort by falling in-degree, stable in the vertex id
@@ -749,6 +698,7 @@ static int ensure_relabel(smdp_graph* g, Relabel** out) {
   g->relabel = new Relabel();
   Relabel* rl = static_cast<Relabel*>(g->relabel);
   rl->h_newid = std::move(newid);
+  rl->minw = minw;
   int rc = alloc_once(&rl->row, n + 1);
   if (rc == SMDP_OK) rc = alloc_once(&rl->col, std::max<int64_t>(1, m));
   if (rc == SMDP_OK) rc = alloc_once(&rl->newid, std::max<int64_t>(1, n));
@@ -818,7 +768,32 @@ int ensure_forest(smdp_graph* g) {
 }
 
 static bool use_relabel(const smdp_params& p, const smdp_graph* g) {
-  return p.layout == SMDP_LAYOUT_HOT || (p.layout == SMDP_LAYOUT_AUTO && !g->transient && g->m >= (1 << 20));
+  return p.layout == SMDP_LAYOUT_HOT || p.layout == SMDP_LAYOUT_BUCKETS ||
+         (p.layout == SMDP_LAYOUT_AUTO && !g->transient && g->m >= (1 << 20));
+}
+
+int run_delta(smdp_graph* g, Relabel* rl, int32_t src, const smdp_params& p, int grid, float* ms_out,
+              int* launches_out);
+
+// Common end of a frontier run: out-arcs of reached vertices, the result kind,
+// the per-round log (log_rounds >= 0: rounds logged by the unordered kernels)
+// and the statistics.
+static int finish_frontier(smdp_graph* g, const int32_t* row, int32_t inf, bool bfs, bool relabel, float ms,
+                           int launches, int log_rounds, bool two_phase, smdp_stats* st) {
+  smdp_ctx* ctx = g->ctx;
+  hipLaunchKernelGGL(k_reached, dim3(std::min<int64_t>((g->n + 255) / 256, 2048)), dim3(256), 0, ctx->stream, row,
+                     g->dist, g->n, inf, g->cnt);
+  SMDP_HIP(hipGetLastError());
+  if (log_rounds >= 0) SMDP_CHECK(collect_round_log(ctx, log_rounds, two_phase));
+  if (st) {
+    SMDP_CHECK(fill_stats(g, st, ms, launches));
+    SMDP_CHECK(prof_collect(ctx, st));
+  } else {
+    prof_reset(ctx);
+  }
+  g->last_res = bfs ? SMDP_RES_LEVEL : SMDP_RES_DIST;
+  g->dist_perm = relabel;
+  return SMDP_OK;
 }
 
 // Runs SSSP (bfs = false) or BFS from src with the given solution; results stay
@@ -831,6 +806,7 @@ int run_frontier(smdp_graph* g, int32_t src, int sol, bool bfs, smdp_stats* st) 
   if (!bfs && g->m > 0 && !g->w) return SMDP_ERR_ARG;
   smdp_ctx* ctx = g->ctx;
   SMDP_HIP(hipSetDevice(ctx->device));
+  begin_run(g);
   const smdp_params& p = params_of(ctx, bfs ? SMDP_APP_BFS : SMDP_APP_SSSP);
   const bool tuned = sol == SMDP_SOL_TUNED;
   SMDP_CHECK(ensure_frontier(g, tuned ? p.chunk : 0));
@@ -878,6 +854,16 @@ int run_frontier(smdp_graph* g, int32_t src, int sol, bool bfs, smdp_stats* st) 
   const int32_t inf = bfs ? SMDP_BFS_UNDEF : SMDP_SSSP_INF;
   const bool two_phase = sol == SMDP_SOL_GRID || tuned;
   hipStream_t s = ctx->stream;
+  // SSSP on the renumbered copy runs in distance buckets unless asked for
+  // unordered rounds or a weight is negative (buckets need w >= 0)
+  const bool buckets = relabel && !bfs && p.layout != SMDP_LAYOUT_HOT &&
+                       static_cast<Relabel*>(g->relabel)->minw >= 0;
+  if (buckets) {
+    float ms = 0;
+    int launches = 0;
+    SMDP_CHECK(run_delta(g, static_cast<Relabel*>(g->relabel), src, p, grid, &ms, &launches));
+    return finish_frontier(g, row, inf, false, relabel, ms, launches, -1, false, st);
+  }
   prof_reset(ctx);
 
   int launches = 0;
@@ -912,19 +898,7 @@ int run_frontier(smdp_graph* g, int32_t src, int sol, bool bfs, smdp_stats* st) 
   SMDP_HIP(hipEventSynchronize(ctx->ev_end));
   float ms = 0;
   SMDP_HIP(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end));
-  hipLaunchKernelGGL(k_reached, dim3(std::min<int64_t>((g->n + 255) / 256, 2048)), dim3(256), 0, s, row, g->dist,
-                     g->n, inf, g->cnt);
-  SMDP_HIP(hipGetLastError());
-  g->last_res = bfs ? SMDP_RES_LEVEL : SMDP_RES_DIST;
-  g->dist_perm = relabel;
-  if (a.rlog) SMDP_CHECK(collect_round_log(ctx, r, two_phase));
-  if (st) {
-    SMDP_CHECK(fill_stats(g, st, ms, launches));
-    SMDP_CHECK(prof_collect(ctx, st));
-  } else {
-    prof_reset(ctx);
-  }
-  return SMDP_OK;
+  return finish_frontier(g, row, inf, bfs, relabel, ms, launches, a.rlog ? r : -1, two_phase, st);
 }
 
 // Host-array drop-in: upload, run, copy back (SSSP_GPU / BFS_REC_GPU shape).

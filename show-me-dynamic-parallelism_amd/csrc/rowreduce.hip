@@ -609,7 +609,8 @@ struct RowBufs {
   float2* mpart = nullptr;
   int64_t mpart_cap = 0;
   float* contrib2 = nullptr;
-  int64_t ndang = -1;
+  int64_t ndang = -1;         // out-degree-0 vertices, counted for out-degrees version ndang_version
+  int32_t ndang_version = -1;
   void* binned = nullptr;  // binned layout of PageRank (binned.hip)
   void* merge = nullptr;   // merge-path SpMV plan (merge.hip)
 };
@@ -890,6 +891,7 @@ int run_spmv(smdp_graph* g, int sol, smdp_stats* st) {
   if (!g || !g->val || !g->x) return SMDP_ERR_ARG;
   smdp_ctx* ctx = g->ctx;
   SMDP_HIP(hipSetDevice(ctx->device));
+  begin_run(g);
   RArgs a;
   RowBufs* b;
   SMDP_CHECK(prepare_args(g, SMDP_APP_SPMV, sol, &a, &b));
@@ -930,6 +932,7 @@ int run_pagerank(smdp_graph* g, const smdp_pr_opts* o, int sol, smdp_stats* st) 
     return SMDP_ERR_ARG;
   smdp_ctx* ctx = g->ctx;
   SMDP_HIP(hipSetDevice(ctx->device));
+  begin_run(g);
   RArgs a;
   RowBufs* b;
   SMDP_CHECK(prepare_args(g, SMDP_APP_PAGERANK, sol, &a, &b));
@@ -937,10 +940,11 @@ int run_pagerank(smdp_graph* g, const smdp_pr_opts* o, int sol, smdp_stats* st) 
   SMDP_CHECK(alloc_once(&g->rank[1], g->n));
   SMDP_CHECK(alloc_once(&g->contrib, g->n));
   SMDP_CHECK(alloc_once(&b->contrib2, g->n));
-  if (b->ndang < 0) {
+  if (b->ndang < 0 || b->ndang_version != g->outdeg_version) {
     std::vector<int32_t> od((size_t)g->n);
     SMDP_HIP(hipMemcpy(od.data(), g->outdeg, sizeof(int32_t) * g->n, hipMemcpyDeviceToHost));
     b->ndang = std::count(od.begin(), od.end(), 0);
+    b->ndang_version = g->outdeg_version;
   }
   const float T = o->total_rank;
   const float init = o->init_zero ? 0.f : T / (float)g->n;

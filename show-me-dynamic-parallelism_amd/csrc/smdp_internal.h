@@ -30,6 +30,9 @@ struct Ctl {
   int32_t pad[27];
   int32_t tiles[kClaimParts * kClaimStride];   // partitioned tile counters (phase 1)
   int32_t chunks[kClaimParts * kClaimStride];  // partitioned grid-queue counters (phase 2)
+  // bucketed SSSP: the next bucket seen by this round's settle pass, stored
+  // as INT32_MAX - bucket (0 = none) in kClaimParts copies (atomicMax)
+  int32_t nbk[kClaimParts * kClaimStride];
 };
 static_assert(sizeof(Ctl) % 128 == 0, "Ctl is made of 128-byte lines");
 constexpr int kCtlInts = (int)(sizeof(Ctl) / sizeof(int32_t));
@@ -76,7 +79,8 @@ struct Counters {
   unsigned long long edges2;   // arcs scanned by phase-2 (grid queue) kernels
   unsigned long long atom_try; // atomicMin attempts after the plain read said "improves"
   unsigned long long atom_win; // atomicMin calls that lowered the value
-  unsigned long long pad[4];
+  unsigned long long skipped;  // bucketed SSSP: arcs read whose relaxation belongs to another pass
+  unsigned long long pad[3];
 };
 static_assert(sizeof(Counters) == 128, "one 128-B line per counter copy");
 
@@ -158,6 +162,7 @@ struct smdp_graph {
   int32_t msrc_groups = 0;
   int last_res = -1;
   int val_version = 0;      // bumped by smdp_graph_set_values
+  int outdeg_version = 0;   // bumped by smdp_graph_set_outdeg
   void* rowbufs = nullptr;  // row-reduce family buffers (rowreduce.hip)
   void* colorbufs = nullptr; // coloring buffers (color.hip)
   void* relabel = nullptr;   // renumbered copy for the tuned frontier (frontier.hip)
@@ -257,6 +262,25 @@ void parallel_ranges(int64_t count, F body) {
   for (auto& x : th) x.join();
 }
 
+// Renumbered copy of a resident graph for the tuned frontier solution
+// (frontier.hip builds it; delta.hip partitions its rows).  Vertices are
+// renumbered by falling in-degree; rows keep their arcs in order until a
+// bucketed SSSP partitions each row into light arcs (w < delta) first, heavy
+// arcs after, with lsplit[v] the first heavy arc of row v.
+struct Relabel {
+  int32_t* row = nullptr;
+  int32_t* col = nullptr;
+  int32_t* w = nullptr;
+  int32_t* newid = nullptr;          // caller id -> copy id (device)
+  uint32_t* hasout = nullptr;        // bit v set when row v of the copy has arcs
+  std::vector<int32_t> h_newid;      // the same on the host (source mapping)
+  int32_t minw = 0;                  // smallest arc weight (bucketing needs minw >= 0)
+  int32_t* lsplit = nullptr;         // bucketed SSSP: first heavy arc of each row
+  int32_t lsplit_delta = 0;          // the bucket width lsplit was built for (0: not built)
+  int32_t* phase = nullptr;          // bucketed SSSP: 4 phase descriptors (delta.hip)
+  void* hctl = nullptr;              // bucketed SSSP: pinned host copy of one control slot
+};
+
 // Renumbered frontier copy (frontier.hip): freed with the graph; the
 // un-permuting gather writes the first `count` results in caller numbering.
 void relabel_free(smdp_graph* g);
@@ -272,5 +296,14 @@ void colorbufs_free(smdp_graph* g);
 
 // Copies counters into st and finishes timings.
 int fill_stats(smdp_graph* g, smdp_stats* st, double ms_kernel, uint64_t launches);
+
+// Called by every run before it writes a device buffer: until the run
+// succeeds and names its result kind, smdp_graph_read reports no result
+// (dist, stamp and the queues are shared by the apps, so a failed or
+// different run leaves nothing that can be read as a previous result).
+inline void begin_run(smdp_graph* g) {
+  g->last_res = -1;
+  g->dist_perm = false;
+}
 
 }  // namespace smdp

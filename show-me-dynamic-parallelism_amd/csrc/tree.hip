@@ -278,6 +278,7 @@ static int run_tree(smdp_graph* g, int32_t root, smdp_stats* st) {
   if (!g || g->n <= 0 || root < 0 || root >= g->n) return SMDP_ERR_ARG;
   smdp_ctx* ctx = g->ctx;
   SMDP_HIP(hipSetDevice(ctx->device));
+  begin_run(g);
   SMDP_CHECK(ensure_forest(g));
   if (g->forest != 1) return SMDP_ERR_ARG;  // a vertex with two in-arcs: not a tree
   const int64_t n = g->n;

@@ -75,12 +75,14 @@ static std::map<std::string, smdp_params> load_cache(const char* path) {
     char key[2048];
     smdp_params p;
     memset(&p, 0, sizeof p);
-    if (sscanf(line,
-               " \"%2047[^\"]\": {\"heavy_threshold\": %d, \"split_threshold\": %d, \"chunk\": %d, "
-               "\"wg_per_cu\": %d, \"batch_rounds\": %d, \"max_rounds\": %d, \"layout\": %d}",
-               key, &p.heavy_threshold, &p.split_threshold, &p.chunk, &p.wg_per_cu, &p.batch_rounds,
-               &p.max_rounds, &p.layout) == 8)
-      out[key] = p;
+    // entries written before the bucket width existed have no "delta" (0: default)
+    const int got = sscanf(line,
+                           " \"%2047[^\"]\": {\"heavy_threshold\": %d, \"split_threshold\": %d, \"chunk\": %d, "
+                           "\"wg_per_cu\": %d, \"batch_rounds\": %d, \"max_rounds\": %d, \"layout\": %d, "
+                           "\"delta\": %d, \"persistent\": %d}",
+                           key, &p.heavy_threshold, &p.split_threshold, &p.chunk, &p.wg_per_cu, &p.batch_rounds,
+                           &p.max_rounds, &p.layout, &p.delta, &p.persistent);
+    if (got >= 8) out[key] = p;
   }
   fclose(f);
   return out;
@@ -97,9 +99,10 @@ static void save_cache(const char* path, const std::map<std::string, smdp_params
     const smdp_params& p = kv.second;
     fprintf(f,
             "  \"%s\": {\"heavy_threshold\": %d, \"split_threshold\": %d, \"chunk\": %d, \"wg_per_cu\": %d, "
-            "\"batch_rounds\": %d, \"max_rounds\": %d, \"layout\": %d}%s\n",
+            "\"batch_rounds\": %d, \"max_rounds\": %d, \"layout\": %d, \"delta\": %d, "
+            "\"persistent\": %d}%s\n",
             kv.first.c_str(), p.heavy_threshold, p.split_threshold, p.chunk, p.wg_per_cu, p.batch_rounds,
-            p.max_rounds, p.layout, ++i < m.size() ? "," : "");
+            p.max_rounds, p.layout, p.delta, p.persistent, ++i < m.size() ? "," : "");
   }
   fprintf(f, "}\n");
   fclose(f);
@@ -149,6 +152,20 @@ static std::vector<smdp_params> candidates(int app, const smdp_params& base) {
   };
   switch (app) {
     case SMDP_APP_SSSP:
+      // bucket widths on the bucketed layout, beside the unordered rounds
+      for (int delta : {32, 64, 128})
+        for (int split : {512, 2048})
+          for (int wg : {4, 8}) {
+            add(base.heavy_threshold, split, 1024, wg);
+            out.back().layout = SMDP_LAYOUT_BUCKETS;
+            out.back().delta = delta;
+          }
+      for (int split : {512, 2048})
+        for (int wg : {4, 8}) {
+          add(base.heavy_threshold, split, 1024, wg);
+          out.back().layout = SMDP_LAYOUT_HOT;
+        }
+      break;
     case SMDP_APP_BFS:
       for (int split : {512, 2048, 8192})
         for (int chunk : {1024, 4096})

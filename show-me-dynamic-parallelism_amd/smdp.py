@@ -19,7 +19,7 @@ ERR_ARG, ERR_NOMEM, ERR_DEVICE, ERR_SOLUTION, ERR_OVERFLOW, ERR_IO, ERR_FORMAT, 
 SOL_FLAT, SOL_NAIVE, SOL_WAVE, SOL_BLOCK, SOL_GRID, SOL_HIER, SOL_TUNED = range(7)
 SOLUTIONS = {"flat": 0, "naive": 1, "wave": 2, "block": 3, "grid": 4, "hier": 5, "tuned": 6}
 APP_SSSP, APP_BFS, APP_SPMV, APP_PAGERANK, APP_COLOR, APP_MSSSP = range(6)
-LAYOUT_AUTO, LAYOUT_CSR, LAYOUT_BINNED, LAYOUT_HOT, LAYOUT_XCD8 = 0, 1, 2, 3, 8
+LAYOUT_AUTO, LAYOUT_CSR, LAYOUT_BINNED, LAYOUT_HOT, LAYOUT_BUCKETS, LAYOUT_XCD8 = 0, 1, 2, 3, 4, 8
 RES_DIST, RES_LEVEL, RES_Y, RES_RANK, RES_COLOR, RES_MDIST, RES_HEIGHT, RES_DESC = range(8)
 FMT_DIMACS9, FMT_DIMACS10, FMT_SLNDC = range(3)
 
@@ -41,7 +41,7 @@ class Csr(C.Structure):
 class Params(C.Structure):
     _fields_ = [("heavy_threshold", C.c_int32), ("split_threshold", C.c_int32), ("chunk", C.c_int32),
                 ("wg_per_cu", C.c_int32), ("batch_rounds", C.c_int32), ("max_rounds", C.c_int32),
-                ("layout", C.c_int32)]
+                ("layout", C.c_int32), ("delta", C.c_int32), ("persistent", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -53,7 +53,8 @@ class Stats(C.Structure):
                 ("m_reached", C.c_uint64), ("residual", C.c_double), ("dangling", C.c_double),
                 ("kernel_launches", C.c_uint64), ("ms_dominant", C.c_double),
                 ("dominant_launches", C.c_uint64), ("ms_phase2", C.c_double), ("phase2_launches", C.c_uint64),
-                ("edges_phase2", C.c_uint64), ("atomics_tried", C.c_uint64), ("atomics_won", C.c_uint64)]
+                ("edges_phase2", C.c_uint64), ("atomics_tried", C.c_uint64), ("atomics_won", C.c_uint64),
+                ("arcs_skipped", C.c_uint64)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
