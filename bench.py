@@ -72,15 +72,30 @@ def sssp_section(ctx, g, dg, args, smdp, orc, np):
     ctx.set_profiling(True)
     pst = dg.sssp(src, sol)
     ctx.set_profiling(False)
-    e1 = pst.edges_relaxed - pst.edges_phase2
-    bytes1 = 12 * e1 + 20 * pst.vertices_expanded  # col + w + dist[v]; 2 offsets + dist[u] + queue in/out
-    bytes2 = 12 * pst.edges_phase2
-    k1 = {"kernel": "k_expand_tile<false>", "bytes": bytes1, "ms": pst.ms_dominant,
-          "launches": pst.dominant_launches}
-    k2 = {"kernel": "k_drain<false>", "bytes": bytes2, "ms": pst.ms_phase2, "launches": pst.phase2_launches}
-    dom = k1 if k1["ms"] >= k2["ms"] else k2
+    bucketed = params.layout == smdp.LAYOUT_BUCKETS or (params.layout == smdp.LAYOUT_AUTO and g.m >= 1 << 20)
+    if bucketed:
+        # one phase = k_delta_step (tiles) + k_delta_drain (grid chunks); a
+        # relaxed arc moves 12 B (col, w, dist[v]), an arc read but left to the
+        # bucket's other pass 8 B (col, w), an expanded vertex 20 B (two row
+        # offsets, dist[u], queue in/out).  Both kernels form the pass.
+        name = "k_delta_step + k_delta_drain"
+        bytes_all = 12 * pst.edges_relaxed + 8 * pst.arcs_skipped + 20 * pst.vertices_expanded
+        ms_all = pst.ms_dominant + pst.ms_phase2
+        dom = {"kernel": name, "bytes": bytes_all, "ms": ms_all, "launches": pst.dominant_launches}
+        k1 = {"kernel": "k_delta_step", "bytes": None, "ms": pst.ms_dominant, "launches": pst.dominant_launches}
+        k2 = {"kernel": "k_delta_drain", "bytes": None, "ms": pst.ms_phase2, "launches": pst.phase2_launches}
+        traffic = _sum_traffic("k_delta_step", "k_delta_drain")
+        bytes1, bytes2 = bytes_all, 0
+    else:
+        e1 = pst.edges_relaxed - pst.edges_phase2
+        bytes1 = 12 * e1 + 20 * pst.vertices_expanded  # col + w + dist[v]; 2 offsets + dist[u] + queue in/out
+        bytes2 = 12 * pst.edges_phase2
+        k1 = {"kernel": "k_expand_tile<false>", "bytes": bytes1, "ms": pst.ms_dominant,
+              "launches": pst.dominant_launches}
+        k2 = {"kernel": "k_drain<false>", "bytes": bytes2, "ms": pst.ms_phase2, "launches": pst.phase2_launches}
+        dom = k1 if k1["ms"] >= k2["ms"] else k2
+        traffic = traffic_for(dom["kernel"])
     ach = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9 if dom["ms"] > 0 else 0.0
-    traffic = traffic_for(dom["kernel"])
     roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": dom["kernel"], "launches": dom["launches"],
@@ -88,9 +103,11 @@ def sssp_section(ctx, g, dg, args, smdp, orc, np):
             "algorithmic_bytes_per_launch": int(dom["bytes"] / max(1, dom["launches"]))}
     kernels = []
     for k in (k1, k2):
-        a = k["bytes"] / (k["ms"] * 1e-3) / 1e9 if k["ms"] > 0 else 0.0
-        kernels.append({"kernel": k["kernel"], "ms_total": round(k["ms"], 4), "launches": k["launches"],
-                        "achieved_GBs": round(a, 1), "frac": round(a / HBM_PEAK_GBS, 4)})
+        entry = {"kernel": k["kernel"], "ms_total": round(k["ms"], 4), "launches": k["launches"]}
+        if k["bytes"] is not None:
+            a = k["bytes"] / (k["ms"] * 1e-3) / 1e9 if k["ms"] > 0 else 0.0
+            entry.update(achieved_GBs=round(a, 1), frac=round(a / HBM_PEAK_GBS, 4))
+        kernels.append(entry)
     whole = (bytes1 + bytes2) / (ms_step * 1e-3) / 1e9
     inflation = st.edges_relaxed / max(1, st.m_reached)
     roof["frac_min_bytes"] = round(roof["frac"] / max(inflation, 1e-9), 4)
@@ -104,8 +121,9 @@ def sssp_section(ctx, g, dg, args, smdp, orc, np):
             "rounds": st.rounds, "edges_relaxed": st.edges_relaxed, "m_reached": st.m_reached,
             "inflation": round(st.edges_relaxed / max(1, st.m_reached), 3),
             "vertices_expanded": st.vertices_expanded, "kernel_launches": st.kernel_launches,
-            "exact_vs_dijkstra": ok, "atomics_tried": st.atomics_tried, "atomics_won": st.atomics_won, "whole_solve_GBs": round(whole, 1), "kernels": kernels,
-            "params": params.as_dict()}
+            "exact_vs_dijkstra": ok, "atomics_tried": st.atomics_tried, "atomics_won": st.atomics_won,
+            "arcs_skipped": st.arcs_skipped, "whole_solve_GBs": round(whole, 1), "kernels": kernels,
+            "layout": "buckets" if bucketed else "unordered rounds", "params": params.as_dict()}
     return gteps, ms_step, roof, info, st
 
 
@@ -421,8 +439,9 @@ def only_section(args, smdp, orc, np):
     ctx.set_params(smdp.APP_PAGERANK, layout=args.layout)
     if args.params:
         kw = json.loads(args.params)
-        for app in (smdp.APP_SSSP, smdp.APP_PAGERANK):
-            ctx.set_params(app, **kw)
+        app = {"sssp": smdp.APP_SSSP, "pagerank": smdp.APP_PAGERANK, "spmv": smdp.APP_SPMV,
+               "color": smdp.APP_COLOR, "bfs": smdp.APP_BFS}.get(args.only, smdp.APP_MSSSP)
+        ctx.set_params(app, **kw)
     if args.only in ("sssp", "pagerank"):
         g = smdp.gen_rmat(args.scale, 16, wlo=1, whi=255, seed=3)
         if args.only == "sssp":

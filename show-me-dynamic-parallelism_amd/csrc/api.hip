@@ -275,7 +275,10 @@ int smdp_create(int device, smdp_ctx** out) {
   for (int a = 0; a < SMDP_APP_COUNT; ++a) c->params[a] = default_params(a);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev_begin) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess ||
-      hipHostMalloc((void**)&c->pinned, 64 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&c->pinned, 64 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->progress, 64 * sizeof(int32_t), hipHostMallocMapped | hipHostMallocCoherent) !=
+          hipSuccess ||
+      hipHostGetDevicePointer((void**)&c->progress_dev, c->progress, 0) != hipSuccess) {
     smdp_destroy(c);
     return SMDP_ERR_DEVICE;
   }
@@ -288,6 +291,7 @@ void smdp_destroy(smdp_ctx* c) {
   (void)hipSetDevice(c->device);
   prof_reset(c);
   if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->progress) (void)hipHostFree(c->progress);
   for (int i = 0; i < 2; ++i) {
     if (c->stage[i]) (void)hipHostFree(c->stage[i]);
     if (c->stage_ev[i]) (void)hipEventDestroy(c->stage_ev[i]);

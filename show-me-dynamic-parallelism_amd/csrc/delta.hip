@@ -24,6 +24,15 @@
 //                smallest distance beyond bucket b (distances it reads and
 //                candidates that won their atomicMin), which names the next
 //                bucket, so empty buckets cost nothing.
+// When a settle pass sees few vertices left beyond its bucket (at most
+// n >> kTailShift, and at most 1/kTailRatio of those settled so far: a small
+// frontier near the source is not a tail), the rest of the solve is one
+// open-ended bucket: its light
+// phases relax whole rows with no upper limit (unordered rounds, as in the
+// reference) until no distance changes.  The tail of a skewed graph spreads
+// a few thousand vertices over many sparse buckets, each of which would cost
+// three phases.
+//
 // A vertex is expanded only while its distance is in the current bucket, and
 // it relaxes each arc in at most one of the light and settle passes with its
 // final distance (plus light re-expansions inside the bucket), so on RMAT-22
@@ -49,9 +58,20 @@
 namespace smdp {
 
 constexpr int kDIlp = 4;          // arcs per thread per step
-constexpr int kDSmallSplit = 32;  // split degree when the light queue is short
-constexpr int kDDrainTile = 8;    // grid-queue entries per drain tile (a power of two, at most 64)
+#ifndef SMDP_DELTA_SMALL_SPLIT
+#define SMDP_DELTA_SMALL_SPLIT 32
+#endif
+constexpr int kDSmallSplit = SMDP_DELTA_SMALL_SPLIT;  // split degree when the light queue is short
+#ifndef SMDP_DELTA_DRAIN_TILE
+#define SMDP_DELTA_DRAIN_TILE 8
+#endif
+constexpr int kDDrainTile = SMDP_DELTA_DRAIN_TILE;  // grid-queue entries per drain tile (a power of two, at most 64)
 constexpr int kRunPhases = 1024;  // phases per persistent launch (the host checks the solve between launches)
+constexpr int kTailShift = 8;     // a settle that leaves at most n >> kTailShift vertices ends in one open bucket
+constexpr int kTailRatio = 32;    // ... if at least kTailRatio times as many are settled
+constexpr int kFeedFirst = 8;     // phases enqueued before the first progress report is read
+constexpr int kFeed = 4;          // phases enqueued at a time after that ...
+constexpr int kFeedSlack = 3;     // ... whenever fewer than this many are still queued ahead of the device
 
 enum : int32_t { kSweepLight = 0, kLight = 1, kSettle = 2, kDone = 3 };
 
@@ -59,15 +79,20 @@ enum : int32_t { kSweepLight = 0, kLight = 1, kSettle = 2, kDone = 3 };
 struct DPhase {
   int32_t mode;
   int32_t bucket;
-  int32_t nin;  // entries: queue length (light) or vertices (sweeps)
-  int32_t pad[29];
+  int32_t nin;   // entries: queue length (light) or vertices (sweeps)
+  int32_t tail;  // 1: the bucket has no upper limit (the last one)
+  int32_t settled;  // vertices expanded by the settle passes before this phase
+  int32_t pad[27];
 };
 static_assert(sizeof(DPhase) == 128, "one line per descriptor");
 constexpr int kPhaseInts = (int)(sizeof(DPhase) / sizeof(int32_t));
 
 // Phase after `prev`: qlen = light-queue entries pushed by prev, nb = the
-// largest next-bucket code stored by prev (INT32_MAX - bucket, 0 = none).
-__host__ __device__ inline DPhase next_phase(const DPhase& prev, int32_t qlen, int32_t nb, int32_t n) {
+// largest next-bucket code stored by prev (INT32_MAX - bucket, 0 = none),
+// rem = the vertices prev's settle pass saw beyond its bucket, sel = those it
+// expanded.
+__host__ __device__ inline DPhase next_phase(const DPhase& prev, int32_t qlen, int32_t nb, int32_t rem, int32_t sel,
+                                             int32_t n) {
   DPhase d = prev;
   switch (prev.mode) {
     case kSweepLight:
@@ -75,16 +100,21 @@ __host__ __device__ inline DPhase next_phase(const DPhase& prev, int32_t qlen, i
       if (qlen > 0) {
         d.mode = kLight;
         d.nin = qlen;
+      } else if (prev.tail) {
+        d.mode = kDone;
+        d.nin = 0;
       } else {
         d.mode = kSettle;
         d.nin = n;
       }
       break;
     case kSettle:
+      d.settled = prev.settled + sel;
       if (nb > 0) {
         d.mode = kSweepLight;
         d.bucket = INT32_MAX - nb;
         d.nin = n;
+        d.tail = rem <= (n >> kTailShift) && (int64_t)rem * kTailRatio <= d.settled;
       } else {
         d.mode = kDone;
         d.nin = 0;
@@ -112,12 +142,13 @@ struct DArgs {
   Ctl* ctl;
   DPhase* ph;
   Counters* cnt;
+  int32_t small_q;  // a light queue shorter than this cuts entries of >= kDSmallSplit arcs into chunks
   unsigned long long* rlog;  // profiling: per phase (entries, arcs relaxed), or nullptr
+  int32_t* progress;         // host memory: p + 1 when phase p starts, -(p + 1) when it is the kDone phase
   int32_t n;
   int32_t shift;    // delta = 1 << shift
   int32_t split;    // an entry with at least this many arcs is cut into grid chunks
   int32_t chunk;    // arcs per grid chunk
-  int32_t small_q;  // a light queue shorter than this splits entries of >= kDSmallSplit arcs
 };
 
 __device__ __forceinline__ bool d_has_arcs(const DArgs& a, int v) { return (a.hasout[v >> 5] >> (v & 31)) & 1u; }
@@ -131,14 +162,20 @@ __device__ __forceinline__ DPhase phase_of(const DArgs& a, int p) {
     d.mode = kSweepLight;
     d.bucket = 0;  // the source, at distance 0
     d.nin = a.n;
+    d.tail = 0;
+    d.settled = 0;
     return d;
   }
   const DPhase prev = a.ph[(p - 1) & 3];
   const Ctl* c = a.ctl + (p & 3);
-  int nb = 0;
+  int nb = 0, rem = 0, sel = 0;
   if (prev.mode == kSettle)
-    for (int k = 0; k < kClaimParts; ++k) nb = max(nb, c->nbk[k * kClaimStride]);
-  return next_phase(prev, c->qlen, nb, a.n);
+    for (int k = 0; k < kClaimParts; ++k) {
+      nb = max(nb, c->nbk[k * kClaimStride]);
+      rem += c->rem[k * kClaimStride];
+      sel += c->sel[k * kClaimStride];
+    }
+  return next_phase(prev, c->qlen, nb, rem, sel, a.n);
 }
 
 // Candidate relaxations of one step, packed to the low lanes through the
@@ -147,7 +184,8 @@ __device__ __forceinline__ DPhase phase_of(const DArgs& a, int p) {
 // settle phase keeps the smallest winning distance of a vertex with arcs.
 template <int K>
 __device__ __forceinline__ void relax_packed_d(const DArgs& a, const int* v, const int* alt, const bool* want,
-                                               bool settle, int label, int32_t* s_pair, Sink& out, int& nbmin) {
+                                               bool settle, int label, int32_t* s_pair, Sink& out, int& nbmin,
+                                               int& nrem) {
   const int lane = lane_id();
   int total = 0;
 #pragma unroll
@@ -178,8 +216,12 @@ __device__ __forceinline__ void relax_packed_d(const DArgs& a, const int* v, con
       if (aa < old) {
         ++out.wins;
         if (arcs) {
-          if (settle) nbmin = min(nbmin, aa);
-          else if (st != label) push = atomicExch(&a.stamp[vv], label) != label;
+          if (settle) {
+            nbmin = min(nbmin, aa);
+            ++nrem;
+          } else if (st != label) {
+            push = atomicExch(&a.stamp[vv], label) != label;
+          }
         }
       }
     }
@@ -191,18 +233,35 @@ __device__ __forceinline__ void relax_packed_d(const DArgs& a, const int* v, con
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// The settle pass's smallest distance beyond the bucket, over the workgroup,
-// into one of the kClaimParts next-bucket copies of slot `c`.  All threads.
-__device__ __forceinline__ void store_next_bucket(Ctl* c, int nbmin, int shift, int* s_red) {
+// The settle pass's smallest distance beyond the bucket and its count of
+// vertices seen beyond it, over the workgroup, into the kClaimParts copies of
+// slot `c` (copy blockIdx % kClaimParts).  All threads; s_red needs
+// 2 * SMDP_WAVES_PER_WG ints.
+__device__ __forceinline__ void store_next_bucket(Ctl* c, int nbmin, int nrem, int nsel, int shift, int* s_red) {
+  constexpr int W = SMDP_WAVES_PER_WG;
   int m = nbmin;
 #pragma unroll
   for (int off = SMDP_WAVE / 2; off > 0; off >>= 1) m = min(m, __shfl_xor(m, off));
-  if (lane_id() == 0) s_red[threadIdx.x / SMDP_WAVE] = m;
+  const int r = wave_sum(nrem), q = wave_sum(nsel);
+  if (lane_id() == 0) {
+    s_red[threadIdx.x / SMDP_WAVE] = m;
+    s_red[W + threadIdx.x / SMDP_WAVE] = r;
+    s_red[2 * W + threadIdx.x / SMDP_WAVE] = q;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int k = 1; k < SMDP_WAVES_PER_WG; ++k) m = min(m, s_red[k]);
-    if (m != INT32_MAX) atomicMax(&c->nbk[(blockIdx.x % kClaimParts) * kClaimStride], INT32_MAX - (m >> shift));
+    int rr = s_red[W], qq = s_red[2 * W];
+    for (int k = 1; k < W; ++k) {
+      m = min(m, s_red[k]);
+      rr += s_red[W + k];
+      qq += s_red[2 * W + k];
+    }
+    const int part = (blockIdx.x % kClaimParts) * kClaimStride;
+    if (m != INT32_MAX) atomicMax(&c->nbk[part], INT32_MAX - (m >> shift));
+    if (rr) atomicAdd(&c->rem[part], rr);
+    if (qq) atomicAdd(&c->sel[part], qq);
   }
+  __syncthreads();
 }
 
 __device__ __forceinline__ void log_phase(const DArgs& a, int p, int L, unsigned long long wave_arcs) {
@@ -244,7 +303,7 @@ struct DShared {
   int32_t start[SMDP_WG];  // first arc of each entry of the tile
   int32_t du[SMDP_WG];     // its distance
   int32_t off[SMDP_WG];    // exclusive scan of the arcs expanded in the tile
-  int32_t scan[SMDP_WAVES_PER_WG];
+  int32_t scan[3 * SMDP_WAVES_PER_WG];
   int32_t list[kSweepList];  // sweep: the selected vertices of a group
   int32_t ldu[kSweepList];   // and their distances
   int32_t out[SMDP_WAVES_PER_WG * kOutCap];
@@ -262,8 +321,9 @@ struct DTile {
   int lim;
   int label;
   bool settle;
+  bool tail;  // the open last bucket: light phases scan whole rows
   unsigned long long edges, skipped, pg, verts;
-  int nbmin;
+  int nbmin, nrem, nsel;
 };
 
 // One phase's state in this workgroup; out must be initialised on the
@@ -272,11 +332,16 @@ __device__ __forceinline__ void tile_init(const DArgs& a, const DPhase& d, int p
   t.s_pair = sh.pair + (threadIdx.x / SMDP_WAVE) * 2 * kDIlp * SMDP_WAVE;
   t.cin = a.ctl + (p & 3);
   t.cout = a.ctl + ((p + 1) & 3);
+  // a short light queue has fewer tiles than workgroups: its entries of a
+  // few dozen arcs go to the grid queue, which every workgroup drains
   t.split = (d.mode == kLight && d.nin < a.small_q) ? min(a.split, kDSmallSplit) : a.split;
-  t.lim = (d.bucket + 1) << a.shift;  // first distance beyond the bucket
+  t.tail = d.tail != 0;
+  t.lim = t.tail ? INT32_MAX : (d.bucket + 1) << a.shift;  // first distance beyond the bucket
   t.label = p + 1;
   t.settle = d.mode == kSettle;
   t.nbmin = INT32_MAX;
+  t.nrem = 0;
+  t.nsel = 0;
 }
 
 // Relaxes the arcs [s_start + j - s_off] of the NE staged entries,
@@ -284,8 +349,8 @@ __device__ __forceinline__ void tile_init(const DArgs& a, const DPhase& d, int p
 // before its atomics.  Every thread of the workgroup calls it.
 template <int NE>
 __device__ __forceinline__ void relax_staged(const DArgs& a, DTile& t, const int32_t* s_start, const int32_t* s_du,
-                                             const int32_t* s_off, int tot, Sink& out) {
-  for (int jb = 0; jb < tot; jb += kDIlp * SMDP_WG) {
+                                             const int32_t* s_off, int j0, int tot, Sink& out) {
+  for (int jb = j0; jb < tot; jb += kDIlp * SMDP_WG) {
     int j[kDIlp], own[kDIlp], k[kDIlp], v[kDIlp], alt[kDIlp], seen[kDIlp];
     bool rel[kDIlp], want[kDIlp];
 #pragma unroll
@@ -309,17 +374,19 @@ __device__ __forceinline__ void relax_staged(const DArgs& a, DTile& t, const int
     for (int q = 0; q < kDIlp; ++q) seen[q] = rel[q] ? a.dist[v[q]] : 0;
 #pragma unroll
     for (int q = 0; q < kDIlp; ++q) want[q] = rel[q] && alt[q] < seen[q];
-    relax_packed_d<kDIlp>(a, v, alt, want, t.settle, t.label, t.s_pair, out, t.nbmin);
+    relax_packed_d<kDIlp>(a, v, alt, want, t.settle, t.label, t.s_pair, out, t.nbmin, t.nrem);
   }
 }
 
-// Expands one tile: this thread's entry is (u, s, deg, du) with deg = 0 for
-// none.  Entries of >= split arcs go to the grid queue as chunks; the others
-// are relaxed edge-balanced.  Every thread of the workgroup calls it.
+// Expands part `part` of `parts` of one tile: this thread's entry is (u, s,
+// deg, du) with deg = 0 for none.  Entries of >= split arcs go to the grid
+// queue as chunks (by part 0); the others are relaxed edge-balanced, each
+// part taking an equal share of the tile's arcs.  Every thread of the
+// workgroup calls it.
 __device__ __forceinline__ void expand_tile(const DArgs& a, DShared& sh, DTile& t, int u, int s, int deg, int du,
-                                            Sink& out) {
+                                            Sink& out, int part = 0, int parts = 1) {
   const int lane = lane_id();
-  const int nch = deg >= t.split ? (deg + a.chunk - 1) / a.chunk : 0;
+  const int nch = (deg >= t.split && part == 0) ? (deg + a.chunk - 1) / a.chunk : 0;
   const int incl = wave_inclusive_scan(nch);
   const int wtot = __shfl(incl, SMDP_WAVE - 1);
   if (wtot > 0) {
@@ -333,10 +400,11 @@ __device__ __forceinline__ void expand_tile(const DArgs& a, DShared& sh, DTile& 
   sh.start[threadIdx.x] = s;
   sh.du[threadIdx.x] = du;
   int tot;
-  const int ex = wg_exclusive_scan(nch ? 0 : deg, sh.scan, &tot);
+  const int ex = wg_exclusive_scan(deg >= t.split ? 0 : deg, sh.scan, &tot);
   sh.off[threadIdx.x] = ex;
   __syncthreads();
-  relax_staged<SMDP_WG>(a, t, sh.start, sh.du, sh.off, tot, out);
+  relax_staged<SMDP_WG>(a, t, sh.start, sh.du, sh.off, (int)((int64_t)tot * part / parts),
+                        (int)((int64_t)tot * (part + 1) / parts), out);
   __syncthreads();
 }
 
@@ -363,12 +431,17 @@ __device__ __forceinline__ void phase_tiles(const DArgs& a, const DPhase& d, int
         dv[k] = SMDP_SSSP_INF;
         if (t0 + k * (int)gridDim.x < ntiles && v < L && d_has_arcs(a, v)) dv[k] = a.dist[v];
       }
+      // selected: in the bucket (in the open last bucket, at or beyond it)
 #pragma unroll
       for (int k = 0; k < kSweepPer; ++k) {
         const int b = dv[k] >> a.shift;
         if (dv[k] != SMDP_SSSP_INF) {
-          if (b == bucket) ++cnt;
-          else if (settle && b > bucket) t.nbmin = min(t.nbmin, dv[k]);
+          if (b == bucket || (t.tail && b > bucket)) {
+            ++cnt;
+          } else if (settle && b > bucket) {
+            t.nbmin = min(t.nbmin, dv[k]);
+            ++t.nrem;
+          }
         }
       }
       int total;
@@ -377,13 +450,15 @@ __device__ __forceinline__ void phase_tiles(const DArgs& a, const DPhase& d, int
       for (int g = 0; g < kSweepPer; g += kSweepGroup) {
         int c = 0;
 #pragma unroll
-        for (int k = 0; k < kSweepGroup; ++k) c += dv[g + k] != SMDP_SSSP_INF && (dv[g + k] >> a.shift) == bucket;
+        for (int k = 0; k < kSweepGroup; ++k)
+          c += dv[g + k] != SMDP_SSSP_INF && ((dv[g + k] >> a.shift) == bucket || (t.tail && (dv[g + k] >> a.shift) > bucket));
         int gtot;
         int pos = wg_exclusive_scan(c, sh.scan, &gtot);
         if (gtot == 0) continue;
 #pragma unroll
         for (int k = 0; k < kSweepGroup; ++k) {
-          if (dv[g + k] != SMDP_SSSP_INF && (dv[g + k] >> a.shift) == bucket) {
+          if (dv[g + k] != SMDP_SSSP_INF &&
+              ((dv[g + k] >> a.shift) == bucket || (t.tail && (dv[g + k] >> a.shift) > bucket))) {
             sh.list[pos] = (t0 + (g + k) * (int)gridDim.x) * SMDP_WG + (int)threadIdx.x;
             sh.ldu[pos] = dv[g + k];
             ++pos;
@@ -397,8 +472,9 @@ __device__ __forceinline__ void phase_tiles(const DArgs& a, const DPhase& d, int
             u = sh.list[i];
             du = sh.ldu[i];
             s = a.row[u];
-            deg = (settle ? a.row[u + 1] : a.lsplit[u]) - s;
+            deg = (settle || t.tail ? a.row[u + 1] : a.lsplit[u]) - s;
             ++t.verts;
+            ++t.nsel;
           }
           expand_tile(a, sh, t, u, s, deg, du, out);
         }
@@ -407,29 +483,26 @@ __device__ __forceinline__ void phase_tiles(const DArgs& a, const DPhase& d, int
   } else {
     const int32_t* qin = (p & 1) ? a.q1 : a.q0;
     const int ntiles = (L + SMDP_WG - 1) / SMDP_WG;
-    int part = 0;  // claim cursor of thread 0
+    int cur = 0;  // claim cursor of thread 0
     for (;;) {
       if (threadIdx.x == 0) {
         int f = 0, e = 0;
-        sh.tile = claim_items(t.cin->tiles, ntiles, 1, part, &f, &e) ? f : -1;
-        sh.tend = e;
+        sh.tile = claim_items(t.cin->tiles, ntiles, 1, cur, &f, &e) ? f : -1;
       }
       __syncthreads();
-      const int first = sh.tile, tend = sh.tend;
+      const int tile = sh.tile;
       __syncthreads();
-      if (first < 0) break;
-      for (int tile = first; tile < tend; ++tile) {
-        const int i = tile * SMDP_WG + (int)threadIdx.x;
-        int u = 0, s = 0, deg = 0, du = 0;
-        if (i < L) {
-          u = qin[i];
-          s = a.row[u];
-          deg = a.lsplit[u] - s;
-          du = a.dist[u];
-          ++t.verts;
-        }
-        expand_tile(a, sh, t, u, s, deg, du, out);
+      if (tile < 0) break;
+      const int i = tile * SMDP_WG + (int)threadIdx.x;
+      int u = 0, s = 0, deg = 0, du = 0;
+      if (i < L) {
+        u = qin[i];
+        s = a.row[u];
+        deg = (t.tail ? a.row[u + 1] : a.lsplit[u]) - s;
+        du = a.dist[u];
+        ++t.verts;
       }
+      expand_tile(a, sh, t, u, s, deg, du, out);
     }
   }
 }
@@ -457,7 +530,7 @@ __device__ __forceinline__ unsigned long long phase_chunks(const DArgs& a, const
       int len = 0, st = 0, du = 0;
       if ((int)threadIdx.x < ept && idx < H) {
         const Chunk c = a.hq[idx];
-        const int vend = t.settle ? a.row[c.v + 1] : a.lsplit[c.v];
+        const int vend = t.settle || t.tail ? a.row[c.v + 1] : a.lsplit[c.v];
         const int end = vend - c.first > a.chunk ? c.first + a.chunk : vend;
         len = end - c.first;
         st = c.first;
@@ -472,7 +545,7 @@ __device__ __forceinline__ unsigned long long phase_chunks(const DArgs& a, const
       if (threadIdx.x == kDDrainTile - 1) sh.tot = incl;
     }
     __syncthreads();
-    relax_staged<kDDrainTile>(a, t, sh.start, sh.du, sh.off, sh.tot, out);
+    relax_staged<kDDrainTile>(a, t, sh.start, sh.du, sh.off, 0, sh.tot, out);
     __syncthreads();
   }
   return t.edges - e0;
@@ -482,7 +555,7 @@ __device__ __forceinline__ unsigned long long phase_chunks(const DArgs& a, const
 // pass records its next-bucket minimum.  All threads.
 __device__ __forceinline__ void end_pass(const DArgs& a, DShared& sh, DTile& t, Sink& out) {
   wg_flush(out, sh.flush);
-  if (t.settle) store_next_bucket(t.cout, t.nbmin, a.shift, sh.scan);
+  if (t.settle) store_next_bucket(t.cout, t.nbmin, t.nrem, t.nsel, a.shift, sh.scan);
 }
 
 // The workgroup's work counters, one atomic per counter.  All threads.
@@ -499,6 +572,9 @@ __device__ __forceinline__ void phase_begin(const DArgs& a, const DPhase& d, int
     if (threadIdx.x == 0) {
       a.ph[p & 3] = d;
       if (d.mode != kDone) atomicMax(&cnt_of(a.cnt)->rounds, (unsigned long long)(p + 1));
+      if (a.progress)
+        __hip_atomic_store(a.progress, d.mode == kDone ? -(p + 1) : p + 1, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
     }
   }
 }
@@ -686,11 +762,12 @@ int run_delta(smdp_graph* g, Relabel* rl, int32_t src, const smdp_params& p, int
   a.ph = reinterpret_cast<DPhase*>(rl->phase);
   a.cnt = g->cnt;
   a.rlog = ctx->profiling ? ctx->rlog : nullptr;
+  a.progress = nullptr;
+  a.small_q = grid * SMDP_WG;
   a.n = g->n;
   a.shift = shift;
   a.split = p.split_threshold;
   a.chunk = p.chunk;
-  a.small_q = grid * SMDP_WG;
   hipStream_t s = ctx->stream;
   if (a.rlog) SMDP_HIP(hipMemsetAsync(a.rlog, 0, sizeof(unsigned long long) * 2 * kRoundLogCap, s));
   prof_reset(ctx);
@@ -717,9 +794,9 @@ int run_delta(smdp_graph* g, Relabel* rl, int32_t src, const smdp_params& p, int
     ph = 0;
     if (!persistent) break;
     // all phases in launches of up to kRunPhases phases each
-    a.small_q = pgrid * SMDP_WG;
     int32_t* hf = ctx->pinned;
     bool gave_up = false;
+    a.small_q = pgrid * SMDP_WG;
     for (;;) {
       SMDP_CHECK(prof_mark(ctx, 0));
       hipLaunchKernelGGL(k_delta_run, dim3(pgrid), dim3(SMDP_WG), 0, s, a, ph, ph + kRunPhases, bar, flags);
@@ -748,33 +825,57 @@ int run_delta(smdp_graph* g, Relabel* rl, int32_t src, const smdp_params& p, int
     if (attempt > 0) return SMDP_ERR_DEVICE;
     persistent = false;
     prof_reset(ctx);
+    a.small_q = grid * SMDP_WG;
   }
   if (!persistent) {
-    a.small_q = grid * SMDP_WG;
-    DPhase* hph = reinterpret_cast<DPhase*>(rl->hctl);
-    Ctl* hc = reinterpret_cast<Ctl*>(hph + 1);
-    int batch = std::max(8, p.batch_rounds);
-    for (;;) {
-      for (int j = 0; j < batch; ++j, ++ph) {
+    // One launch pair per phase.  The step kernel reports each phase it starts
+    // in host memory, so the host keeps a few phases queued ahead of the
+    // device (never letting the queue drain while it checks) and stops
+    // enqueueing at the phase that finds the solve done; a blocking check per
+    // batch left the device idle ~35 us each time, and batches sized blindly
+    // ran up to 25 phases past the end.
+      a.progress = ctx->progress_dev;
+    volatile int32_t* prog = ctx->progress;
+    *prog = 0;
+    int enq = 0;
+    auto enqueue = [&](int k) -> int {
+      for (int j = 0; j < k; ++j, ++enq) {
         SMDP_CHECK(prof_mark(ctx, 0));
-        hipLaunchKernelGGL(k_delta_step, dim3(grid), dim3(SMDP_WG), 0, s, a, ph);
+        hipLaunchKernelGGL(k_delta_step, dim3(grid), dim3(SMDP_WG), 0, s, a, enq);
         SMDP_CHECK(prof_mark(ctx, 0));
         SMDP_CHECK(prof_mark(ctx, 1));
-        hipLaunchKernelGGL(k_delta_drain, dim3(grid), dim3(SMDP_WG), 0, s, a, ph);
+        hipLaunchKernelGGL(k_delta_drain, dim3(grid), dim3(SMDP_WG), 0, s, a, enq);
         SMDP_CHECK(prof_mark(ctx, 1));
         launches += 2;
       }
       SMDP_HIP(hipGetLastError());
-      // the last phase's descriptor and what it appended decide whether another follows
-      SMDP_HIP(hipMemcpyAsync(hph, a.ph + ((ph - 1) & 3), sizeof(DPhase), hipMemcpyDeviceToHost, s));
-      SMDP_HIP(hipMemcpyAsync(hc, g->ctl + (ph & 3), sizeof(Ctl), hipMemcpyDeviceToHost, s));
-      SMDP_HIP(hipStreamSynchronize(s));
-      int nb = 0;
-      for (int k = 0; k < kClaimParts; ++k) nb = std::max(nb, hc->nbk[k * kClaimStride]);
-      if (hph->mode == kDone || next_phase(*hph, hc->qlen, nb, g->n).mode == kDone) break;
-      if (ph >= p.max_rounds) return SMDP_ERR_OVERFLOW;
-      batch = std::min(batch * 2, 32);
+      return SMDP_OK;
+    };
+    SMDP_CHECK(enqueue(kFeedFirst));
+    int seen = 0;
+    double t_seen = now_ms();
+    for (;;) {
+      const int v = *prog;
+      if (v < 0) break;  // the kDone phase started: nothing more to enqueue
+      if (v >= enq - kFeedSlack) {
+        if (enq >= p.max_rounds) {
+          SMDP_HIP(hipStreamSynchronize(s));
+          return SMDP_ERR_OVERFLOW;
+        }
+        SMDP_CHECK(enqueue(kFeed));
+        continue;
+      }
+      if (v != seen) {
+        seen = v;
+        t_seen = now_ms();
+      } else if (now_ms() - t_seen > 100.0) {
+        // no progress report for a while: a fault stops the stream
+        const hipError_t e = hipStreamQuery(s);
+        if (e != hipSuccess && e != hipErrorNotReady) return hip_fail(e, "bucketed SSSP phases", __LINE__);
+        t_seen = now_ms();
+      }
     }
+    ph = enq;
   }
   SMDP_HIP(hipEventRecord(ctx->ev_end, s));
   SMDP_HIP(hipEventSynchronize(ctx->ev_end));

@@ -767,7 +767,12 @@ int ensure_forest(smdp_graph* g) {
   return rc;
 }
 
+// The renumbered copy is built on the host once per resident graph; a graph
+// uploaded by a drop-in call for one solve never pays for it (RMAT-22: a
+// drop-in solve took 21 ms without it and 575 ms with it), so it runs the
+// unordered rounds on the graph as given.
 static bool use_relabel(const smdp_params& p, const smdp_graph* g) {
+  if (g->transient) return false;
   return p.layout == SMDP_LAYOUT_HOT || p.layout == SMDP_LAYOUT_BUCKETS ||
          (p.layout == SMDP_LAYOUT_AUTO && !g->transient && g->m >= (1 << 20));
 }

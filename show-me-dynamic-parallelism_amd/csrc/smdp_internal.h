@@ -33,6 +33,10 @@ struct Ctl {
   // bucketed SSSP: the next bucket seen by this round's settle pass, stored
   // as INT32_MAX - bucket (0 = none) in kClaimParts copies (atomicMax)
   int32_t nbk[kClaimParts * kClaimStride];
+  // bucketed SSSP: vertices the settle pass saw beyond its bucket (an upper
+  // bound: a vertex lowered twice counts twice), in kClaimParts copies
+  int32_t rem[kClaimParts * kClaimStride];
+  int32_t sel[kClaimParts * kClaimStride];  // bucketed SSSP: vertices the settle pass expanded
 };
 static_assert(sizeof(Ctl) % 128 == 0, "Ctl is made of 128-byte lines");
 constexpr int kCtlInts = (int)(sizeof(Ctl) / sizeof(int32_t));
@@ -114,6 +118,10 @@ struct smdp_ctx {
   std::vector<hipEvent_t> prof_ev;  // pairs of events around profiled launches
   std::vector<int> prof_kind;       // kernel kind of each pair
   int32_t* pinned = nullptr;        // small host readback buffer (hipHostMalloc)
+  // Host memory kernels write progress into while they run (coherent, mapped):
+  // progress is the host view, progress_dev the device address of the same words.
+  int32_t* progress = nullptr;
+  int32_t* progress_dev = nullptr;
   int32_t* stage[2] = {nullptr, nullptr};  // pinned staging of large uploads (hipHostMalloc, lazily)
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
   unsigned long long* rlog = nullptr;  // per-round (queue length, arcs) when profiling

@@ -154,8 +154,8 @@ static std::vector<smdp_params> candidates(int app, const smdp_params& base) {
     case SMDP_APP_SSSP:
       // bucket widths on the bucketed layout, beside the unordered rounds
       for (int delta : {32, 64, 128})
-        for (int split : {512, 2048})
-          for (int wg : {4, 8}) {
+        for (int split : {128, 256, 512})
+          for (int wg : {2, 4}) {
             add(base.heavy_threshold, split, 1024, wg);
             out.back().layout = SMDP_LAYOUT_BUCKETS;
             out.back().delta = delta;
