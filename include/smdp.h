@@ -266,6 +266,27 @@ void smdp_batch_destroy(smdp_batch* b);
 int smdp_autotune(smdp_graph* gr, int app, int32_t src, const char* cache_path,
                   smdp_params* chosen);
 
+/* ---- an entry point that allocates nothing --------------------------------*/
+/* smdp_sssp (same solutions, same distances) on a CSR the caller already holds
+ * in device memory: d_row[n+1], d_col[m], d_w[m] and the output d_dist[n] are
+ * DEVICE pointers owned by the caller.  No counterpart in the reference; it is
+ * for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The call allocates
+ * nothing: every working buffer (frontier queues, stamps, grid queue, control
+ * blocks, counters) comes from one fixed 12 MB device array that the library
+ * holds on each GPU it runs on until the process ends (set up when the first
+ * context on that GPU is created), so n <= SMDP_NOALLOC_MAX_N and
+ * m <= SMDP_NOALLOC_MAX_M, else SMDP_ERR_ARG.  The CSR is checked on the device
+ * first (SMDP_ERR_ARG when row is not monotone from 0 to m or a column is out of
+ * range).  All calls on one GPU share that array, so calls that overlap on one
+ * GPU (from other threads or contexts) race.  Profiling is ignored, and
+ * SMDP_SOL_TUNED runs unordered rounds on the graph as given (no renumbered
+ * copy, no buckets). */
+#define SMDP_NOALLOC_MAX_N (1 << 19)
+#define SMDP_NOALLOC_MAX_M (1 << 23)
+int smdp_sssp_device_noalloc(smdp_ctx* ctx, int32_t n, int32_t m, const int32_t* d_row, const int32_t* d_col,
+                             const int32_t* d_w, int32_t src, int solution, int32_t* d_dist, smdp_stats* st);
+
 /* ---- host graph utilities (no device needed) ------------------------------*/
 /* A host graph owned by the library. */
 typedef struct {

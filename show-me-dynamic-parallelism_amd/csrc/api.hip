@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "smdp_internal.h"
 
@@ -106,13 +107,17 @@ int prof_collect(smdp_ctx* ctx, smdp_stats* st) {
   return SMDP_OK;
 }
 
-int read_counters(const Counters* dev, hipStream_t s, Counters* out) {
-  Counters parts[kCntParts];
-  SMDP_HIP(hipMemcpyAsync(parts, dev, sizeof parts, hipMemcpyDeviceToHost, s));
-  SMDP_HIP(hipStreamSynchronize(s));
+// The copies go through the context's pinned buffer (allocated with the
+// context), so reading counters allocates nothing.
+int read_counters(smdp_ctx* ctx, const Counters* dev, Counters* out) {
+  static_assert(kCntParts * sizeof(Counters) <= kPinnedInts * sizeof(int32_t), "pinned buffer holds the counters");
+  const Counters* parts = reinterpret_cast<const Counters*>(ctx->pinned);
+  SMDP_HIP(hipMemcpyAsync(ctx->pinned, dev, kCntParts * sizeof(Counters), hipMemcpyDeviceToHost, ctx->stream));
+  SMDP_HIP(hipStreamSynchronize(ctx->stream));
   Counters c;
   memset(&c, 0, sizeof c);
-  for (const Counters& p : parts) {
+  for (int k = 0; k < kCntParts; ++k) {
+    const Counters& p = parts[k];
     c.edges += p.edges;
     c.verts += p.verts;
     for (int i = 0; i < 4; ++i) c.push[i] += p.push[i];
@@ -131,7 +136,7 @@ int read_counters(const Counters* dev, hipStream_t s, Counters* out) {
 int fill_stats(smdp_graph* g, smdp_stats* st, double ms_kernel, uint64_t launches) {
   if (!st) return SMDP_OK;
   Counters c;
-  SMDP_CHECK(read_counters(g->cnt, g->ctx->stream, &c));
+  SMDP_CHECK(read_counters(g->ctx, g->cnt, &c));
   st->ms_kernel = ms_kernel;
   st->edges_relaxed = c.edges;
   st->vertices_expanded = c.verts;
@@ -232,6 +237,35 @@ static int upload4(smdp_ctx* ctx, T** d, const T* s, int64_t cnt) {
   return SMDP_OK;
 }
 
+// Working memory of the entry points that allocate nothing
+// (smdp_sssp_device_noalloc): one fixed array per GPU, held by every GPU the
+// library runs on until the process ends.  In host code `scratch` only names
+// the symbol; its device address on each GPU comes from hipGetSymbolAddress,
+// taken once per device (with that device current) when the first context on
+// it is created, and is the only handle host code uses.
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static void* g_scratch_ptr[kMaxDevices] = {nullptr};
+static std::mutex g_scratch_mu;
+
+int scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return SMDP_ERR_ARG;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return SMDP_OK;
+  void* ptr = nullptr;
+  SMDP_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));  // the device is current (smdp_create)
+  if (!ptr) return SMDP_ERR_DEVICE;
+  g_scratch_ptr[device] = ptr;
+  return SMDP_OK;
+}
+
+void* scratch_ptr(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+
 }  // namespace smdp
 
 using namespace smdp;
@@ -275,12 +309,17 @@ int smdp_create(int device, smdp_ctx** out) {
   for (int a = 0; a < SMDP_APP_COUNT; ++a) c->params[a] = default_params(a);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev_begin) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess ||
-      hipHostMalloc((void**)&c->pinned, 64 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->pinned, kPinnedInts * sizeof(int32_t), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&c->progress, 64 * sizeof(int32_t), hipHostMallocMapped | hipHostMallocCoherent) !=
           hipSuccess ||
       hipHostGetDevicePointer((void**)&c->progress_dev, c->progress, 0) != hipSuccess) {
     smdp_destroy(c);
     return SMDP_ERR_DEVICE;
+  }
+  const int rc = scratch_setup(device);
+  if (rc != SMDP_OK) {
+    smdp_destroy(c);
+    return rc;
   }
   *out = c;
   return SMDP_OK;

@@ -571,7 +571,7 @@ int run_multi_source(smdp_graph* g, const int32_t* srcs, int32_t k, int32_t* hos
     }
     ms_total += ms;
     Counters c;
-    if (read_counters(g->cnt, s, &c) != SMDP_OK) {
+    if (read_counters(ctx, g->cnt, &c) != SMDP_OK) {
       rc = SMDP_ERR_DEVICE;
       break;
     }
@@ -837,7 +837,7 @@ int smdp_batch_run(smdp_batch* b, smdp_stats* st) {
   std::vector<int32_t> status(b->ng);
   SMDP_HIP(hipMemcpy(status.data(), b->d_status, sizeof(int32_t) * b->ng, hipMemcpyDeviceToHost));
   Counters c;
-  SMDP_CHECK(read_counters(b->d_cnt, s, &c));
+  SMDP_CHECK(read_counters(ctx, b->d_cnt, &c));
   int rc = SMDP_OK;
   uint64_t rounds = 0;
   for (int32_t v : status) {

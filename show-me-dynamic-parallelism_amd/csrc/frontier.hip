@@ -906,6 +906,95 @@ int run_frontier(smdp_graph* g, int32_t src, int sol, bool bfs, smdp_stats* st) 
   return finish_frontier(g, row, inf, bfs, relabel, ms, launches, a.rlog ? r : -1, two_phase, st);
 }
 
+// ---- SSSP on a caller's device CSR with no allocation -------------------------
+// Every working buffer is carved from the library's fixed per-GPU array
+// (scratch_ptr), 256-byte aligned, in this order.
+namespace noalloc {
+constexpr int64_t kMaxN = SMDP_NOALLOC_MAX_N;
+constexpr int64_t kMaxM = SMDP_NOALLOC_MAX_M;
+constexpr int64_t kHqCap = kMaxN + kMaxM / 64 + 64;  // chunk >= 64 (smdp_set_params)
+constexpr int64_t align(int64_t b) { return (b + 255) / 256 * 256; }
+constexpr int64_t kCtl = 0;
+constexpr int64_t kCnt = kCtl + align(4 * (int64_t)sizeof(Ctl));
+constexpr int64_t kFlag = kCnt + align(kCntParts * (int64_t)sizeof(Counters));
+constexpr int64_t kStamp = kFlag + 256;
+constexpr int64_t kQ0 = kStamp + align(4 * kMaxN);
+constexpr int64_t kQ1 = kQ0 + align(4 * kMaxN);
+constexpr int64_t kHasout = kQ1 + align(4 * kMaxN);
+constexpr int64_t kHq = kHasout + align(4 * ((kMaxN + 31) / 32));
+constexpr int64_t kEnd = kHq + align((int64_t)sizeof(Chunk) * kHqCap);
+static_assert(kEnd <= kNoAllocScratchBytes, "the working buffers fit the fixed array");
+}  // namespace noalloc
+
+// flag = 1 when row is not non-decreasing from row[0] = 0 to row[n] = m or a
+// column is outside [0, n).  Reads row[0..n] and col[0..m) only.
+__global__ void k_check_csr(const int32_t* __restrict__ row, const int32_t* __restrict__ col, int32_t n, int32_t m,
+                            int32_t* flag) {
+  bool bad = false;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t r = row[i];
+    if ((i == 0 && r != 0) || (i == n && r != m) || (i < n && row[i + 1] < r)) bad = true;
+  }
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += (int64_t)gridDim.x * blockDim.x)
+    if ((uint32_t)col[e] >= (uint32_t)n) bad = true;
+  if (bad) atomicOr(flag, 1);
+}
+
+__global__ void k_clear_flag(int32_t* flag) { *flag = 0; }
+
+int sssp_noalloc(smdp_ctx* ctx, int32_t n, int32_t m, const int32_t* d_row, const int32_t* d_col,
+                 const int32_t* d_w, int32_t src, int sol, int32_t* d_dist, smdp_stats* st) {
+  if (!ctx || !d_row || !d_dist || n <= 0 || m < 0 || (m > 0 && (!d_col || !d_w))) return SMDP_ERR_ARG;
+  if (n > noalloc::kMaxN || m > noalloc::kMaxM) return SMDP_ERR_ARG;  // beyond the fixed array
+  if (src < 0 || src >= n) return SMDP_ERR_ARG;
+  if (sol < 0 || sol >= SMDP_SOL_COUNT || sol == SMDP_SOL_HIER) return SMDP_ERR_SOLUTION;
+  SMDP_HIP(hipSetDevice(ctx->device));
+  char* base = static_cast<char*>(scratch_ptr(ctx->device));
+  if (!base) return SMDP_ERR_DEVICE;
+  hipStream_t s = ctx->stream;
+  int32_t* flag = reinterpret_cast<int32_t*>(base + noalloc::kFlag);
+  hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, s, flag);
+  hipLaunchKernelGGL(k_check_csr, dim3((unsigned)std::min<int64_t>(((int64_t)std::max(n + 1, m) + 255) / 256, 2048)),
+                     dim3(256), 0, s, d_row, d_col, n, m, flag);
+  SMDP_HIP(hipGetLastError());
+  SMDP_HIP(hipMemcpyAsync(ctx->pinned, flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  SMDP_HIP(hipStreamSynchronize(s));
+  if (ctx->pinned[0]) return SMDP_ERR_ARG;
+
+  smdp_graph g;  // every buffer is the caller's or the fixed array's: nothing to free
+  g.ctx = ctx;
+  g.n = n;
+  g.m = m;
+  g.row = const_cast<int32_t*>(d_row);
+  g.col = const_cast<int32_t*>(d_col);
+  g.w = const_cast<int32_t*>(d_w);
+  g.dist = d_dist;
+  g.stamp = reinterpret_cast<int32_t*>(base + noalloc::kStamp);
+  g.q[0] = reinterpret_cast<int32_t*>(base + noalloc::kQ0);
+  g.q[1] = reinterpret_cast<int32_t*>(base + noalloc::kQ1);
+  g.hq = reinterpret_cast<Chunk*>(base + noalloc::kHq);
+  g.hq_cap = noalloc::kHqCap;
+  g.ctl = reinterpret_cast<Ctl*>(base + noalloc::kCtl);
+  g.cnt = reinterpret_cast<Counters*>(base + noalloc::kCnt);
+  g.transient = true;  // no renumbered copy
+  g.forest = 0;
+  if (sol == SMDP_SOL_TUNED) {
+    // the has-arcs bits are rebuilt every call (the array keeps the last call's contents)
+    g.hasout = reinterpret_cast<uint32_t*>(base + noalloc::kHasout);
+    const int64_t nw = ((int64_t)n + 31) / 32;
+    hipLaunchKernelGGL(k_hasout, dim3((unsigned)std::min<int64_t>((nw + 255) / 256, 4096)), dim3(256), 0, s,
+                       d_row, n, g.hasout);
+    SMDP_HIP(hipGetLastError());
+  }
+  // k_init_frontier clears the control blocks and counters and sets stamp; the
+  // queues and the grid queue are written before they are read
+  const int prof = ctx->profiling;
+  ctx->profiling = 0;  // profiling would create events
+  const int rc = run_frontier(&g, src, sol, false, st);
+  ctx->profiling = prof;
+  return rc;
+}
+
 // Host-array drop-in: upload, run, copy back (SSSP_GPU / BFS_REC_GPU shape).
 int frontier_dropin(smdp_ctx* ctx, const smdp_csr* h, int32_t src, int sol, bool bfs, int32_t* out,
                     smdp_stats* st) {
@@ -941,6 +1030,11 @@ int smdp_sssp(smdp_ctx* ctx, const smdp_csr* g, int32_t src, int solution, int32
 
 int smdp_bfs(smdp_ctx* ctx, const smdp_csr* g, int32_t src, int solution, int32_t* level, smdp_stats* st) {
   return frontier_dropin(ctx, g, src, solution, true, level, st);
+}
+
+int smdp_sssp_device_noalloc(smdp_ctx* ctx, int32_t n, int32_t m, const int32_t* d_row, const int32_t* d_col,
+                             const int32_t* d_w, int32_t src, int solution, int32_t* d_dist, smdp_stats* st) {
+  return sssp_noalloc(ctx, n, m, d_row, d_col, d_w, src, solution, d_dist, st);
 }
 
 }  // extern "C"

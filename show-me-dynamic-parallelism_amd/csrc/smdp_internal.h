@@ -96,7 +96,7 @@ constexpr int kCntParts = 32;
 __device__ __forceinline__ Counters* cnt_of(Counters* c) { return c + (blockIdx.x & (kCntParts - 1)); }
 // Copies the kCntParts copies from the device (synchronously on stream s)
 // and combines them: sums, and the maximum of rounds and done.
-int read_counters(const Counters* dev, hipStream_t s, Counters* out);
+int read_counters(smdp_ctx* ctx, const Counters* dev, Counters* out);
 
 // Entry of the grid queue: vertex/row and the first edge of its chunk.
 struct Chunk {
@@ -131,6 +131,17 @@ struct smdp_ctx {
 
 namespace smdp {
 constexpr int kRoundLogCap = 4096;
+constexpr int kPinnedInts = 2048;  // smdp_ctx::pinned: readback of small results and of the work counters
+
+// Fixed working memory of smdp_sssp_device_noalloc (api.hip holds the array):
+// control slots, counters, a flag, then stamp, two queues and the has-arcs
+// bits for n <= SMDP_NOALLOC_MAX_N vertices, and the grid queue for
+// n + m / 64 + 64 chunks with m <= SMDP_NOALLOC_MAX_M.
+constexpr int64_t kNoAllocScratchBytes = int64_t(12) << 20;
+// Sets up the array's device address for `device` (once; the device must be
+// current) / returns it (nullptr before set-up).
+int scratch_setup(int device);
+void* scratch_ptr(int device);
 }
 
 struct smdp_graph {

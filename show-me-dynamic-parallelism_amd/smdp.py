@@ -121,6 +121,7 @@ def _declare(L):
         "smdp_pagerank_run": (C.c_int, [vp, P(PrOpts), C.c_int, P(Stats)]),
         "smdp_color_run": (C.c_int, [vp, C.c_int, C.c_int, P(Stats)]),
         "smdp_sssp_multi_source_run": (C.c_int, [vp, vp, i32, P(Stats)]),
+        "smdp_sssp_device_noalloc": (C.c_int, [vp, i32, i32, vp, vp, vp, i32, C.c_int, vp, P(Stats)]),
         "smdp_graph_read": (C.c_int, [vp, C.c_int, vp, C.c_int64]),
         "smdp_batch_create": (C.c_int, [vp, P(Csr), i32, vp, P(vp)]),
         "smdp_batch_run": (C.c_int, [vp, P(Stats)]),
