@@ -467,6 +467,7 @@ void smdp_graph_destroy(smdp_graph* g) {
     if (p) (void)hipFree(p);
   rowbufs_free(g);
   colorbufs_free(g);
+  msplan_free(g);
   relabel_free(g);
   delete g;
 }
@@ -526,7 +527,14 @@ int smdp_graph_read(smdp_graph* g, int what, void* out, int64_t count) {
       what = SMDP_RES_HEIGHT;
       break;
     case SMDP_RES_MDIST:
-      return SMDP_ERR_ARG;  // multi-source results are read by the multi-source call
+      // [k][n] of the last multi-source run, source-major
+      if (g->last_res != SMDP_RES_MDIST) return SMDP_ERR_ARG;
+      {
+        const int32_t* r = nullptr;
+        SMDP_CHECK(msplan_result(g, &r, &avail));
+        src = r;
+      }
+      break;
     default: return SMDP_ERR_ARG;
   }
   // Only the result of the last run is defined (dist/level/colour share a buffer).

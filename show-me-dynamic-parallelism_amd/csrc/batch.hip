@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "device_common.h"
+#include "grid_barrier.h"
 #include "smdp_internal.h"
 
 namespace smdp {
@@ -100,8 +101,6 @@ __global__ void k_ms_init(int32_t* mdist, int32_t* stamp, int32_t n, const int32
   if (blockIdx.x == 0) {
     int32_t* c = reinterpret_cast<int32_t*>(ctl);
     for (int i = threadIdx.x; i < 4 * kCtlInts; i += blockDim.x) c[i] = 0;
-    unsigned long long* p = reinterpret_cast<unsigned long long*>(cnt);
-    for (int i = threadIdx.x; i < (int)(kCntParts * sizeof(Counters) / 8); i += blockDim.x) p[i] = 0;
   }
 }
 
@@ -116,22 +115,19 @@ __global__ void k_ms_seed(int32_t* mdist, int32_t* stamp, const int32_t* __restr
   }
 }
 
-__global__ __launch_bounds__(SMDP_WG) void k_ms_expand(MArgs a, int r) {
-  ctl_clear(a.ctl, (r + 2) & 3);
+// Round r's expansion pass (all threads of the workgroup): waves take queue
+// entries; a vertex with >= split arcs goes to the grid queue as chunks.
+// Queue entries go out through `out` (flushed at the end).
+__device__ __forceinline__ void ms_expand_pass(const MArgs& a, int r, int L, WaveOut<kMsOut>& out,
+                                               unsigned long long& edges, unsigned long long& verts,
+                                               unsigned long long& wins, unsigned long long& pg) {
   Ctl* cin = a.ctl + (r & 3);
   Ctl* cout = a.ctl + ((r + 1) & 3);
-  const int L = cin->qlen;
-  if (L == 0) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&cnt_of(a.cnt)->rounds, (unsigned long long)(r + 1));
   const int32_t* qin = (r & 1) ? a.q1 : a.q0;
   int32_t* qout = (r & 1) ? a.q0 : a.q1;
   const int label = r + 1;
   const int lane = lane_id();
   const int wave = threadIdx.x / SMDP_WAVE;
-  __shared__ int32_t s_out[SMDP_WAVES_PER_WG * kMsOut];
-  WaveOut<kMsOut> out;
-  out.init(s_out + wave * kMsOut);
-  unsigned long long edges = 0, verts = 0, wins = 0, pg = 0;
   const int gw = blockIdx.x * SMDP_WAVES_PER_WG + wave;
   const int nw = gridDim.x * SMDP_WAVES_PER_WG;
   for (int i = gw; i < L; i += nw) {
@@ -151,31 +147,16 @@ __global__ __launch_bounds__(SMDP_WG) void k_ms_expand(MArgs a, int r) {
     ms_arcs(a, s, e, du, label, out, qout, &cout->qlen, edges, wins);
   }
   out.flush(qout, &cout->qlen);
-  edges = wave_sum(edges);
-  verts = wave_sum(verts);
-  wins = wave_sum(wins);
-  pg = wave_sum(pg);
-  if (lane == 0) {
-    if (edges) atomicAdd(&cnt_of(a.cnt)->edges, edges);
-    if (verts) atomicAdd(&cnt_of(a.cnt)->verts, verts);
-    if (wins) atomicAdd(&cnt_of(a.cnt)->atom_win, wins);
-    if (pg) atomicAdd(&cnt_of(a.cnt)->push[3], pg);
-  }
 }
 
-__global__ __launch_bounds__(SMDP_WG) void k_ms_drain(MArgs a, int r) {
+// Round r's chunk pass (all threads): waves claim the H grid-queue chunks.
+__device__ __forceinline__ void ms_drain_pass(const MArgs& a, int r, int H, WaveOut<kMsOut>& out,
+                                              unsigned long long& edges, unsigned long long& wins) {
   Ctl* cin = a.ctl + (r & 3);
   Ctl* cout = a.ctl + ((r + 1) & 3);
-  const int H = cin->hlen;
-  if (H == 0) return;
   int32_t* qout = (r & 1) ? a.q0 : a.q1;
   const int label = r + 1;
   const int lane = lane_id();
-  const int wave = threadIdx.x / SMDP_WAVE;
-  __shared__ int32_t s_out[SMDP_WAVES_PER_WG * kMsOut];
-  WaveOut<kMsOut> out;
-  out.init(s_out + wave * kMsOut);
-  unsigned long long edges = 0, wins = 0;
   int part = 0;  // lane 0: claim cursor of this wave
   for (;;) {
     int idx = -1;
@@ -191,15 +172,105 @@ __global__ __launch_bounds__(SMDP_WG) void k_ms_drain(MArgs a, int r) {
     ms_arcs(a, c.first, e, du, label, out, qout, &cout->qlen, edges, wins);
   }
   out.flush(qout, &cout->qlen);
+}
+
+__device__ __forceinline__ void ms_counters(const MArgs& a, unsigned long long edges, unsigned long long verts,
+                                            unsigned long long wins, unsigned long long pg,
+                                            unsigned long long edges2) {
   edges = wave_sum(edges);
+  verts = wave_sum(verts);
   wins = wave_sum(wins);
-  if (lane == 0) {
-    if (edges) {
-      atomicAdd(&cnt_of(a.cnt)->edges, edges);
-      atomicAdd(&cnt_of(a.cnt)->edges2, edges);
-    }
+  pg = wave_sum(pg);
+  edges2 = wave_sum(edges2);
+  if (lane_id() == 0) {
+    if (edges) atomicAdd(&cnt_of(a.cnt)->edges, edges);
+    if (verts) atomicAdd(&cnt_of(a.cnt)->verts, verts);
     if (wins) atomicAdd(&cnt_of(a.cnt)->atom_win, wins);
+    if (pg) atomicAdd(&cnt_of(a.cnt)->push[3], pg);
+    if (edges2) atomicAdd(&cnt_of(a.cnt)->edges2, edges2);
   }
+}
+
+__global__ __launch_bounds__(SMDP_WG) void k_ms_expand(MArgs a, int r) {
+  ctl_clear(a.ctl, (r + 2) & 3);
+  const int L = a.ctl[r & 3].qlen;
+  if (L == 0) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&cnt_of(a.cnt)->rounds, (unsigned long long)(r + 1));
+  __shared__ int32_t s_out[SMDP_WAVES_PER_WG * kMsOut];
+  WaveOut<kMsOut> out;
+  out.init(s_out + (threadIdx.x / SMDP_WAVE) * kMsOut);
+  unsigned long long edges = 0, verts = 0, wins = 0, pg = 0;
+  ms_expand_pass(a, r, L, out, edges, verts, wins, pg);
+  ms_counters(a, edges, verts, wins, pg, 0);
+}
+
+__global__ __launch_bounds__(SMDP_WG) void k_ms_drain(MArgs a, int r) {
+  const int H = a.ctl[r & 3].hlen;
+  if (H == 0) return;
+  __shared__ int32_t s_out[SMDP_WAVES_PER_WG * kMsOut];
+  WaveOut<kMsOut> out;
+  out.init(s_out + (threadIdx.x / SMDP_WAVE) * kMsOut);
+  unsigned long long edges = 0, wins = 0;
+  ms_drain_pass(a, r, H, out, edges, wins);
+  ms_counters(a, edges, 0, wins, 0, edges);
+}
+
+// Every round of one group of sources in one launch whose workgroups are all
+// resident (the grid is sized to what the device holds), with a grid barrier
+// after the expansion pass and, when the round pushed chunks, after the chunk
+// pass.  Every workgroup reads the queue lengths after a barrier, so all take
+// the same barriers and stop together.  flags[0] = 1 when a barrier gave up
+// waiting; flags[1 + group] = the rounds run, or -1 when rmax was reached.
+// The launch needs no host check between rounds, so it can be captured in a
+// hipGraph with the initialisation and replayed.
+__global__ __launch_bounds__(SMDP_WG) void k_ms_run(MArgs a, uint32_t* bar, int32_t* flags, int group, int rmax) {
+  __shared__ int32_t s_out[SMDP_WAVES_PER_WG * kMsOut];
+  __shared__ int s_ok;
+  WaveOut<kMsOut> out;
+  out.init(s_out + (threadIdx.x / SMDP_WAVE) * kMsOut);
+  unsigned long long edges = 0, verts = 0, wins = 0, pg = 0, edges2 = 0;
+  uint32_t epoch = 0;
+  int r = 0;
+  bool capped = false;
+  for (;; ++r) {
+    const int L = __hip_atomic_load(&a.ctl[r & 3].qlen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (L == 0) break;
+    if (r >= rmax) {
+      capped = true;
+      break;
+    }
+    if (blockIdx.x == 0) {
+      ctl_clear(a.ctl, (r + 2) & 3);  // round r + 1's output slot; no workgroup of round r uses it
+      if (threadIdx.x == 0) atomicMax(&cnt_of(a.cnt)->rounds, (unsigned long long)(r + 1));
+    }
+    ms_expand_pass(a, r, L, out, edges, verts, wins, pg);
+    if (!grid_barrier(bar, ++epoch, &s_ok)) {
+      if (threadIdx.x == 0) atomicExch(&flags[0], 1);
+      break;
+    }
+    const int H = __hip_atomic_load(&a.ctl[r & 3].hlen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (H > 0) {
+      const unsigned long long e0 = edges;
+      ms_drain_pass(a, r, H, out, edges, wins);
+      edges2 += edges - e0;
+      if (!grid_barrier(bar, ++epoch, &s_ok)) {
+        if (threadIdx.x == 0) atomicExch(&flags[0], 1);
+        break;
+      }
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) flags[1 + group] = capped ? -1 : r;
+  ms_counters(a, edges, verts, wins, pg, edges2);
+}
+
+// Resets a group's barrier lines (every group's persistent launch counts its
+// epochs from 1) and, for the first group, the batch's flags and counters.
+__global__ void k_ms_reset(uint32_t* bar, int32_t* flags, int32_t nflags, Counters* cnt, int first) {
+  for (int i = threadIdx.x; i < kBarLines * 32; i += blockDim.x) bar[i] = 0;
+  if (!first) return;
+  for (int i = threadIdx.x; i < nflags; i += blockDim.x) flags[i] = 0;
+  unsigned long long* p = reinterpret_cast<unsigned long long*>(cnt);
+  for (int i = threadIdx.x; i < (int)(kCntParts * sizeof(Counters) / 8); i += blockDim.x) p[i] = 0;
 }
 
 // [n][64] group table -> rows [s0, s0+k) of the caller's [k_total][n] output.
@@ -483,29 +554,53 @@ __global__ void k_mg_status_clear(int32_t* status, int32_t ng) {
 
 // ------------------------------------------------------------------ host
 
-int run_multi_source(smdp_graph* g, const int32_t* srcs, int32_t k, int32_t* host_out, smdp_stats* st) {
-  if (!g || !srcs || k <= 0 || g->n <= 0) return SMDP_ERR_ARG;
-  if (g->m > 0 && !g->w) return SMDP_ERR_ARG;
-  for (int32_t i = 0; i < k; ++i)
-    if (srcs[i] < 0 || srcs[i] >= g->n) return SMDP_ERR_ARG;
-  smdp_ctx* ctx = g->ctx;
-  SMDP_HIP(hipSetDevice(ctx->device));
-  const smdp_params& p = params_of(ctx, SMDP_APP_MSSSP);
-  const int64_t n = g->n;
-  SMDP_CHECK(ensure(&g->mdist, &g->mdist_cap, n * kLanes));
-  SMDP_CHECK(alloc_once(&g->stamp, n));
-  SMDP_CHECK(alloc_once(&g->q[0], n));
-  SMDP_CHECK(alloc_once(&g->q[1], n));
-  SMDP_CHECK(ensure(&g->hq, &g->hq_cap, n + g->m / std::max(1, p.chunk) + 64));
-  int32_t* d_out = nullptr;   // [k][n] on the device
-  int32_t* d_srcs = nullptr;
-  if (hipMalloc((void**)&d_out, sizeof(int32_t) * (size_t)k * n) != hipSuccess) return SMDP_ERR_NOMEM;
-  if (hipMalloc((void**)&d_srcs, sizeof(int32_t) * k) != hipSuccess) {
-    (void)hipFree(d_out);
-    return SMDP_ERR_NOMEM;
+// Per-graph state of the batched multi-source solve: the sources, the
+// [k][n] results, the persistent launches' barrier lines and flags, and the
+// hipGraph that runs every group of sources.  Captured once for a (k,
+// parameters) pair; a later call with the same k writes its sources into
+// d_srcs and replays the graph.
+struct MsPlan {
+  int32_t k = 0;
+  int32_t groups = 0;
+  int32_t* d_srcs = nullptr;  // k
+  int32_t* mres = nullptr;    // [k][n], source-major
+  int64_t mres_cap = 0;
+  uint32_t* bar = nullptr;    // kBarLines * 32 barrier words
+  int32_t* flags = nullptr;   // [0] a barrier gave up; [1 + j] rounds of group j (-1: round cap)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int grid = 0, split = 0, chunk = 0, rmax = 0;
+  // the graph's buffers the capture baked in: another app may reallocate the
+  // shared grid queue, and a replay must never use a freed buffer
+  const void* bufs[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
   }
-  int rc = SMDP_OK;
-  hipStream_t s = ctx->stream;
+};
+
+void msplan_free(smdp_graph* g) {
+  MsPlan* pl = static_cast<MsPlan*>(g->msplan);
+  if (!pl) return;
+  pl->drop_graph();
+  void* ptrs[] = {pl->d_srcs, pl->mres, pl->bar, pl->flags};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  delete pl;
+  g->msplan = nullptr;
+}
+
+int msplan_result(smdp_graph* g, const int32_t** out, int64_t* count) {
+  MsPlan* pl = static_cast<MsPlan*>(g->msplan);
+  if (!pl || !pl->mres) return SMDP_ERR_ARG;
+  *out = pl->mres;
+  *count = (int64_t)pl->k * g->n;
+  return SMDP_OK;
+}
+
+static MArgs ms_args(smdp_graph* g, const smdp_params& p) {
   MArgs a;
   a.row = g->row;
   a.col = g->col;
@@ -519,85 +614,172 @@ int run_multi_source(smdp_graph* g, const int32_t* srcs, int32_t k, int32_t* hos
   a.cnt = g->cnt;
   a.split = p.split_threshold;
   a.chunk = p.chunk;
+  return a;
+}
+
+// The same groups with host-driven rounds (two launches per round and a
+// check of the queue length per batch of rounds), for a device that cannot
+// hold the persistent grid.  Results go to the plan's mres.
+static int ms_rounds_host(smdp_graph* g, MsPlan* pl, const smdp_params& p, uint64_t* rounds, uint64_t* launches) {
+  smdp_ctx* ctx = g->ctx;
+  hipStream_t s = ctx->stream;
+  const MArgs a = ms_args(g, p);
   const int grid = grid_size(ctx, p);
-  double ms_total = 0;
-  uint64_t edges = 0, verts = 0, rounds = 0, wins = 0, edges2 = 0, launches = 0;
-  prof_reset(ctx);
-  if (hipMemcpy(d_srcs, srcs, sizeof(int32_t) * k, hipMemcpyHostToDevice) != hipSuccess) rc = SMDP_ERR_DEVICE;
-  for (int32_t s0 = 0; s0 < k && rc == SMDP_OK; s0 += kLanes) {
-    const int kk = std::min<int32_t>(kLanes, k - s0);
-    if (hipEventRecord(ctx->ev_begin, s) != hipSuccess) {
-      rc = SMDP_ERR_DEVICE;
-      break;
-    }
-    hipLaunchKernelGGL(k_ms_init, dim3(2048), dim3(256), 0, s, g->mdist, g->stamp, g->n, d_srcs + s0, kk, g->q[0],
+  hipLaunchKernelGGL(k_ms_reset, dim3(1), dim3(256), 0, s, pl->bar, pl->flags, 1 + pl->groups, g->cnt, 1);
+  ++*launches;
+  for (int j = 0; j < pl->groups; ++j) {
+    const int s0 = j * kLanes, kk = std::min<int32_t>(kLanes, pl->k - s0);
+    hipLaunchKernelGGL(k_ms_init, dim3(2048), dim3(256), 0, s, g->mdist, g->stamp, g->n, pl->d_srcs + s0, kk, g->q[0],
                        g->ctl, g->cnt);
-    hipLaunchKernelGGL(k_ms_seed, dim3(1), dim3(64), 0, s, g->mdist, g->stamp, d_srcs + s0, kk, g->q[0], g->ctl);
-    launches += 2;
+    hipLaunchKernelGGL(k_ms_seed, dim3(1), dim3(64), 0, s, g->mdist, g->stamp, pl->d_srcs + s0, kk, g->q[0], g->ctl);
+    *launches += 2;
     int r = 0;
     RoundBatcher rb(p.batch_rounds, 16);
     for (;;) {
-      for (int j = 0; j < rb.batch; ++j, ++r) {
-        prof_mark(ctx, 0);
+      for (int i = 0; i < rb.batch; ++i, ++r) {
+        SMDP_CHECK(prof_mark(ctx, 0));
         hipLaunchKernelGGL(k_ms_expand, dim3(grid), dim3(SMDP_WG), 0, s, a, r);
-        prof_mark(ctx, 0);
-        prof_mark(ctx, 1);
+        SMDP_CHECK(prof_mark(ctx, 0));
+        SMDP_CHECK(prof_mark(ctx, 1));
         hipLaunchKernelGGL(k_ms_drain, dim3(grid), dim3(SMDP_WG), 0, s, a, r);
-        prof_mark(ctx, 1);
-        launches += 2;
+        SMDP_CHECK(prof_mark(ctx, 1));
+        *launches += 2;
       }
-      if (hipGetLastError() != hipSuccess ||
-          hipMemcpyAsync(ctx->pinned, &g->ctl[r & 3].qlen, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
-          hipStreamSynchronize(s) != hipSuccess) {
-        rc = SMDP_ERR_DEVICE;
-        break;
-      }
+      SMDP_HIP(hipGetLastError());
+      SMDP_HIP(hipMemcpyAsync(ctx->pinned, &g->ctl[r & 3].qlen, 4, hipMemcpyDeviceToHost, s));
+      SMDP_HIP(hipStreamSynchronize(s));
       if (ctx->pinned[0] == 0) break;
-      if (r >= p.max_rounds) {
-        rc = SMDP_ERR_OVERFLOW;
-        break;
-      }
+      if (r >= pl->rmax) return SMDP_ERR_OVERFLOW;
       rb.advance(ctx->pinned[0], r);
     }
-    if (rc != SMDP_OK) break;
-    hipLaunchKernelGGL(k_ms_transpose, dim3((unsigned)((n + 63) / 64)), dim3(64, 16), 0, s, g->mdist, g->n, kk,
-                       d_out + (int64_t)s0 * n);
-    ++launches;
-    float ms = 0;
-    if (hipEventRecord(ctx->ev_end, s) != hipSuccess || hipEventSynchronize(ctx->ev_end) != hipSuccess ||
-        hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end) != hipSuccess) {
-      rc = SMDP_ERR_DEVICE;
-      break;
-    }
-    ms_total += ms;
-    Counters c;
-    if (read_counters(ctx, g->cnt, &c) != SMDP_OK) {
-      rc = SMDP_ERR_DEVICE;
-      break;
-    }
-    edges += c.edges;
-    edges2 += c.edges2;
-    verts += c.verts;
-    rounds += c.rounds;
-    wins += c.atom_win;
+    *rounds += (uint64_t)r;
+    hipLaunchKernelGGL(k_ms_transpose, dim3((unsigned)((g->n + 63) / 64)), dim3(64, 16), 0, s, g->mdist, g->n, kk,
+                       pl->mres + (int64_t)s0 * g->n);
+    ++*launches;
   }
-  if (rc == SMDP_OK && host_out &&
-      hipMemcpy(host_out, d_out, sizeof(int32_t) * (size_t)k * n, hipMemcpyDeviceToHost) != hipSuccess)
-    rc = SMDP_ERR_DEVICE;
-  (void)hipFree(d_out);
-  (void)hipFree(d_srcs);
-  if (rc != SMDP_OK) return rc;
+  SMDP_HIP(hipGetLastError());
+  return SMDP_OK;
+}
+
+// Batched SSSP from k sources; results stay in the plan's mres ([k][n]) and,
+// with host_out, are copied there.
+int run_multi_source(smdp_graph* g, const int32_t* srcs, int32_t k, int32_t* host_out, smdp_stats* st) {
+  if (!g || !srcs || k <= 0 || g->n <= 0) return SMDP_ERR_ARG;
+  if (g->m > 0 && !g->w) return SMDP_ERR_ARG;
+  for (int32_t i = 0; i < k; ++i)
+    if (srcs[i] < 0 || srcs[i] >= g->n) return SMDP_ERR_ARG;
+  smdp_ctx* ctx = g->ctx;
+  SMDP_HIP(hipSetDevice(ctx->device));
+  begin_run(g);
+  const smdp_params& p = params_of(ctx, SMDP_APP_MSSSP);
+  const int64_t n = g->n;
+  SMDP_CHECK(ensure(&g->mdist, &g->mdist_cap, n * kLanes));
+  SMDP_CHECK(alloc_once(&g->stamp, n));
+  SMDP_CHECK(alloc_once(&g->q[0], n));
+  SMDP_CHECK(alloc_once(&g->q[1], n));
+  SMDP_CHECK(ensure(&g->hq, &g->hq_cap, n + g->m / std::max(1, p.chunk) + 64));
+  if (!g->msplan) g->msplan = new MsPlan();
+  MsPlan* pl = static_cast<MsPlan*>(g->msplan);
+  hipStream_t s = ctx->stream;
+
+  // The persistent launch needs all its workgroups resident at once.
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_ms_run), SMDP_WG, 0) !=
+      hipSuccess)
+    per_cu = 0;
+  const int pgrid = std::min(grid_size(ctx, p), per_cu * ctx->cus);
+  const int rmax = (int)std::min<int64_t>(p.max_rounds, n + 1);  // more rounds than vertices: a negative cycle
+  const int groups = (k + kLanes - 1) / kLanes;
+  const void* bufs[7] = {g->mdist, g->stamp, g->q[0], g->q[1], g->hq, g->ctl, g->cnt};
+  if (pl->k != k || pl->grid != pgrid || pl->split != p.split_threshold || pl->chunk != p.chunk || pl->rmax != rmax ||
+      !pl->exec || !std::equal(bufs, bufs + 7, pl->bufs)) {
+    pl->drop_graph();
+    if (pl->k < k || !pl->d_srcs) {
+      if (pl->d_srcs) (void)hipFree(pl->d_srcs);
+      pl->d_srcs = nullptr;
+      SMDP_CHECK(alloc_once(&pl->d_srcs, k));
+    }
+    if (pl->groups < groups || !pl->flags) {
+      if (pl->flags) (void)hipFree(pl->flags);
+      pl->flags = nullptr;
+      SMDP_CHECK(alloc_once(&pl->flags, 1 + groups));
+    }
+    SMDP_CHECK(alloc_once(&pl->bar, (int64_t)kBarLines * 32));
+    SMDP_CHECK(ensure(&pl->mres, &pl->mres_cap, (int64_t)k * n));
+    pl->k = k;
+    pl->groups = groups;
+    pl->grid = pgrid;
+    pl->split = p.split_threshold;
+    pl->chunk = p.chunk;
+    pl->rmax = rmax;
+    std::copy(bufs, bufs + 7, pl->bufs);
+    if (pgrid >= 1) {
+      const MArgs a = ms_args(g, p);
+      SMDP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      for (int j = 0; j < groups; ++j) {
+        const int s0 = j * kLanes, kk = std::min<int32_t>(kLanes, k - s0);
+        // kernels clear the barrier lines, flags and counters (no memset nodes)
+        hipLaunchKernelGGL(k_ms_reset, dim3(1), dim3(256), 0, s, pl->bar, pl->flags, 1 + groups, g->cnt, j == 0);
+        hipLaunchKernelGGL(k_ms_init, dim3(2048), dim3(256), 0, s, g->mdist, g->stamp, g->n, pl->d_srcs + s0, kk,
+                           g->q[0], g->ctl, g->cnt);
+        hipLaunchKernelGGL(k_ms_seed, dim3(1), dim3(64), 0, s, g->mdist, g->stamp, pl->d_srcs + s0, kk, g->q[0],
+                           g->ctl);
+        hipLaunchKernelGGL(k_ms_run, dim3(pgrid), dim3(SMDP_WG), 0, s, a, pl->bar, pl->flags, j, rmax);
+        hipLaunchKernelGGL(k_ms_transpose, dim3((unsigned)((n + 63) / 64)), dim3(64, 16), 0, s, g->mdist, g->n, kk,
+                           pl->mres + (int64_t)s0 * n);
+      }
+      const hipError_t e = hipStreamEndCapture(s, &pl->graph);
+      if (e != hipSuccess) return hip_fail(e, "capture of the multi-source batch", __LINE__);
+      SMDP_HIP(hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0));
+    }
+  }
+  SMDP_HIP(hipMemcpyAsync(pl->d_srcs, srcs, sizeof(int32_t) * k, hipMemcpyHostToDevice, s));
+  prof_reset(ctx);
+  uint64_t rounds = 0, launches = 0;
+  bool host_rounds = !pl->exec;
+  SMDP_HIP(hipEventRecord(ctx->ev_begin, s));
+  if (!host_rounds) {
+    SMDP_CHECK(prof_mark(ctx, 0));
+    SMDP_HIP(hipGraphLaunch(pl->exec, s));
+    SMDP_CHECK(prof_mark(ctx, 0));
+    launches += 5 * (uint64_t)groups;
+    SMDP_HIP(hipMemcpyAsync(ctx->pinned, pl->flags, sizeof(int32_t) * (1 + groups), hipMemcpyDeviceToHost, s));
+    SMDP_HIP(hipStreamSynchronize(s));
+    if (ctx->pinned[0]) {
+      // a grid barrier gave up (the workgroups were not all resident): run
+      // the same groups with host-driven rounds
+      host_rounds = true;
+      prof_reset(ctx);
+      SMDP_HIP(hipEventRecord(ctx->ev_begin, s));
+    } else {
+      for (int j = 0; j < groups; ++j) {
+        if (ctx->pinned[1 + j] < 0) return SMDP_ERR_OVERFLOW;
+        rounds += (uint64_t)ctx->pinned[1 + j];
+      }
+    }
+  }
+  if (host_rounds) SMDP_CHECK(ms_rounds_host(g, pl, p, &rounds, &launches));
+  SMDP_HIP(hipEventRecord(ctx->ev_end, s));
+  SMDP_HIP(hipEventSynchronize(ctx->ev_end));
+  float ms = 0;
+  SMDP_HIP(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end));
+  Counters c;
+  SMDP_CHECK(read_counters(ctx, g->cnt, &c));
+  if (host_out)
+    SMDP_HIP(hipMemcpy(host_out, pl->mres, sizeof(int32_t) * (size_t)k * n, hipMemcpyDeviceToHost));
   g->msrc_k = k;
+  g->last_res = SMDP_RES_MDIST;
   if (st) {
     memset(st, 0, sizeof *st);
-    st->ms_kernel = ms_total;
-    st->edges_relaxed = edges;
-    st->edges_phase2 = edges2;
-    st->vertices_expanded = verts;
+    st->ms_kernel = ms;
+    st->edges_relaxed = c.edges;
+    st->edges_phase2 = c.edges2;
+    st->vertices_expanded = c.verts;
     st->rounds = rounds;
-    st->atomics_won = wins;
+    st->atomics_won = c.atom_win;
+    st->q_push[3] = c.push[3];
     st->kernel_launches = launches;
-    prof_collect(ctx, st);
+    SMDP_CHECK(prof_collect(ctx, st));
   } else {
     prof_reset(ctx);
   }

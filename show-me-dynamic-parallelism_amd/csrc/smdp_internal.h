@@ -178,7 +178,7 @@ struct smdp_graph {
   int32_t* mdist = nullptr;
   int64_t mdist_cap = 0;
   int32_t msrc_k = 0;
-  int32_t msrc_groups = 0;
+  void* msplan = nullptr;   // batched multi-source state and its captured graph (batch.hip)
   int last_res = -1;
   int val_version = 0;      // bumped by smdp_graph_set_values
   int outdeg_version = 0;   // bumped by smdp_graph_set_outdeg
@@ -312,6 +312,9 @@ int prof_collect(smdp_ctx* ctx, smdp_stats* st);
 
 void rowbufs_free(smdp_graph* g);
 void colorbufs_free(smdp_graph* g);
+void msplan_free(smdp_graph* g);
+// The last multi-source run's [k][n] results (device) and their count.
+int msplan_result(smdp_graph* g, const int32_t** out, int64_t* count);
 
 // Copies counters into st and finishes timings.
 int fill_stats(smdp_graph* g, smdp_stats* st, double ms_kernel, uint64_t launches);

@@ -21,6 +21,62 @@ def test_multi_source_parity(ctx, k):
     assert st.rounds >= 1
 
 
+def test_multi_source_resident_read_and_replays(ctx):
+    """The resident call keeps [k][n] on the device for smdp_graph_read; later
+    calls with the same k replay the captured graph with new sources, with
+    eager work (which may reallocate the graph's shared grid queue) between
+    replays."""
+    g = smdp.gen_rmat(13, 16, wlo=1, whi=255, seed=7)
+    with smdp.Graph(ctx, g) as dg:
+        for rep in range(4):
+            srcs = smdp.gen_sources(g, 40, 100 + rep)
+            st = dg.sssp_multi_source(srcs)
+            d = dg.read(smdp.RES_MDIST, count=40 * g.n).reshape(40, g.n)
+            for i, s in enumerate(srcs):
+                assert np.array_equal(d[i], orc.dijkstra(g.row, g.col, g.w, int(s))), (rep, i)
+            assert st.rounds >= 1 and st.edges_relaxed > 0
+            # eager work on the same graph between replays: an SSSP with small
+            # chunks needs a larger grid queue than the batch captured
+            old = ctx.get_params(smdp.APP_SSSP)
+            ctx.set_params(smdp.APP_SSSP, layout=smdp.LAYOUT_HOT, split_threshold=64, chunk=64)
+            dg.sssp(int(srcs[0]), smdp.SOL_TUNED)
+            ctx.set_params(smdp.APP_SSSP, **old.as_dict())
+            assert np.array_equal(dg.read(smdp.RES_DIST), orc.dijkstra(g.row, g.col, g.w, int(srcs[0])))
+            with pytest.raises(smdp.SmdpError):
+                dg.read(smdp.RES_MDIST, count=40 * g.n)  # the last run was an SSSP
+
+
+def test_results_of_another_run_are_not_readable(ctx):
+    """A run invalidates what an earlier run of another kind left readable,
+    and a failed run leaves nothing readable."""
+    t = smdp.gen_zipf_tree(5000, 1.5, 64, 3)
+    row, col = t.row, t.col
+    w = np.ones(len(col), np.int32)
+    with smdp.Graph(ctx, smdp.HostGraph(row, col, w)) as dg:
+        dg.tree(0)
+        dg.read(smdp.RES_DESC)
+        dg.sssp_multi_source(np.array([0, 1], np.int32))
+        with pytest.raises(smdp.SmdpError):
+            dg.read(smdp.RES_DESC)
+    # colour, then an SSSP that fails on a negative cycle: no colours to read
+    rowc, colc, wc = random_graph(500, 3000, 5, wlo=1, whi=9)
+    wc = wc.copy()
+    wc[:] = -1  # every cycle is negative
+    old = ctx.get_params(smdp.APP_SSSP)
+    with smdp.Graph(ctx, smdp.HostGraph(rowc, colc, wc)) as dg:
+        dg.color(False, smdp.SOL_TUNED)
+        dg.read(smdp.RES_COLOR)
+        try:
+            ctx.set_params(smdp.APP_SSSP, max_rounds=50)
+            with pytest.raises(smdp.SmdpError) as e:
+                dg.sssp(0, smdp.SOL_FLAT)
+            assert e.value.code == smdp.ERR_OVERFLOW
+        finally:
+            ctx.set_params(smdp.APP_SSSP, **old.as_dict())
+        with pytest.raises(smdp.SmdpError):
+            dg.read(smdp.RES_COLOR)
+
+
 def test_multi_source_duplicates_and_chunks(ctx):
     row, col, w = random_graph(3000, 40000, 12, 1, 30)
     g = smdp.HostGraph(row, col, w)

@@ -98,8 +98,12 @@ def test_batched_rmat21_and_small_graphs(ctx):
     print("batched rmat21", flush=True)
     srcs = smdp.gen_sources(g, 64, 4)
     outs, st = ctx.sssp_multi_source(g, srcs)
-    for i in (0, 17, 42, 63):
-        assert np.array_equal(outs[i], orc.dijkstra(g.row, g.col, g.w, int(srcs[i])))
+    # every one of the 64 sources (the oracle's Dijkstra runs on host threads)
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=16) as ex:
+        refs = list(ex.map(lambda s: orc.dijkstra(g.row, g.col, g.w, int(s)), srcs))
+    for i in range(64):
+        assert np.array_equal(outs[i], refs[i]), i
     graphs = [smdp.gen_small_graph(i, seed=4) for i in range(256)]
     outs, _ = ctx.sssp_multi_graph(graphs, np.zeros(256, np.int32))
     for gi, di in zip(graphs, outs):
