@@ -66,5 +66,8 @@ def test_autotune_other_apps(ctx, app, tmp_path):
             else:
                 srcs = smdp.gen_sources(g, 8, 1)
                 dg.sssp_multi_source(srcs)
+                d = dg.read(smdp.RES_MDIST, count=8 * g.n).reshape(8, g.n)
+                for i, s in enumerate(srcs):
+                    assert np.array_equal(d[i], orc.dijkstra(g.row, g.col, g.w, int(s))), i
     finally:
         ctx.set_params(app, **old.as_dict())

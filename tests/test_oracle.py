@@ -225,6 +225,35 @@ def test_pagerank_iteration_matched_and_init():
     assert it2 < 100 and res2 < 1e-6
 
 
+@pytest.mark.parametrize("seed,d", [(0, 0.85), (1, 0.5), (2, 0.99)])
+def test_pagerank_k_steps_are_powers_of_the_google_matrix(seed, d):
+    """From ranks summing to T, k pull iterations with dangling mass spread
+    uniformly are G^k r0 for the Google matrix G = d P + (1 - d)/N 11^T, where
+    column u of P is 1/outdeg(u) on u's targets (arcs counted with their
+    multiplicity) and 1/N everywhere when u has no arcs.  A dense matrix power
+    pins every iterate, not only the fixed point: a dangling mass taken from
+    the new ranks, or damping left off it, moves them."""
+    import smdp
+    row, col, _ = random_graph(40, 120, 900 + seed)  # dangling vertices and duplicate arcs
+    n = len(row) - 1
+    assert (np.diff(row) == 0).any()
+    P = np.zeros((n, n))
+    for u in range(n):
+        deg = row[u + 1] - row[u]
+        if deg == 0:
+            P[:, u] = 1.0 / n
+        for e in range(row[u], row[u + 1]):
+            P[col[e], u] += 1.0 / deg
+    G = d * P + (1.0 - d) / n * np.ones((n, n))
+    T = 3.0
+    r0 = np.full(n, T / n)
+    ig = smdp.invert(smdp.HostGraph(row, col))
+    for k in range(1, 7):
+        rk, it, _ = orc.pagerank(ig.row, ig.col, np.diff(row), d=d, T=T, fixed_iters=k)
+        assert it == k
+        assert np.max(np.abs(rk - np.linalg.matrix_power(G, k) @ r0)) <= 1e-13 * T, k
+
+
 # ---- OpenMP twins (CPU baseline timing) -----------------------------------
 
 @pytest.mark.parametrize("seed", [0, 1])
