@@ -154,6 +154,69 @@ def cpu_baseline(g, src, orc, np, budget_s=8.0):
                                      "solves": n_dij}}
 
 
+def c1_section(ctx, smdp, orc, np):
+    """The reference's own small case: 16384 vertices, 8 uniform out-arcs each,
+    weights U[1,100], source 0, written as a DIMACS9 .gr file and read back
+    with format 0 (sssp.cpp:100-154, global.cpp:29-64), solved on the GPU
+    through the drop-in call and through the gpu-sssp command, with the
+    oracle's worklist Bellman-Ford timed on the host beside it."""
+    import subprocess
+    import tempfile
+    src_g = smdp.gen_uniform(16384, 8, 1, 100, 1)
+    tmp = tempfile.mkdtemp()
+    path = os.path.join(tmp, "uniform16k.gr")
+    smdp.write_dimacs9(path, src_g)
+    t_read = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        g = smdp.read_graph(path, 0)
+        t_read.append(time.perf_counter() - t0)
+    same = bool(np.array_equal(g.row, src_g.row) and np.array_equal(g.col, src_g.col)
+                and np.array_equal(g.w, src_g.w) and g.source == src_g.source)
+    ref_read_ms = None
+    if orc.ref_available():
+        t0 = time.perf_counter()
+        rg = orc.ref_read(path, 0)
+        ref_read_ms = round((time.perf_counter() - t0) * 1e3, 2)
+        same = same and bool(np.array_equal(rg["row"], g.row) and np.array_equal(rg["col"], g.col)
+                             and np.array_equal(rg["w"], g.w))
+    src = g.source
+    sol = smdp.SOL_TUNED
+    ctx.sssp(g, src, sol)
+    runs = [ctx.sssp(g, src, sol) for _ in range(10)]
+    e2e = sorted(st.ms_total for _, st in runs)[5]
+    dev = sorted(st.ms_kernel for _, st in runs)[5]
+    dist, st = runs[-1]
+    ref = orc.dijkstra(g.row, g.col, g.w, src)
+    # CPU: the reference algorithm restated, one thread, about 2 s of solves
+    n_cpu, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < 2.0:
+        d_cpu, rounds_cpu, _ = orc.sssp_worklist(g.row, g.col, g.w, src)
+        n_cpu += 1
+    t_cpu = (time.perf_counter() - t0) / n_cpu
+    cli = {}
+    exe = os.path.join(ROOT, "show-me-dynamic-parallelism_amd", "bin", "gpu-sssp")
+    if os.path.exists(exe):
+        r = subprocess.run([exe, "-i", path, "-f", "0", "-s", "6", "-v"], capture_output=True, text=True,
+                           timeout=120)
+        for key, pat in (("s_kernel", "HIP kernel execution"), ("s_total", "HIP total"),
+                         ("s_read", "Read data and AdjList to CSR")):
+            for ln in (r.stdout + r.stderr).splitlines():
+                if ln.startswith(pat):
+                    cli[key] = float(ln.split()[-1])
+        cli["returncode"] = r.returncode
+    os.remove(path)
+    os.rmdir(tmp)
+    return {"workload": "uniform16k .gr, 16384 vertices x 8 arcs, weights U[1,100], source 0",
+            "file_read_ms": round(sorted(t_read)[2] * 1e3, 2), "reference_reader_ms": ref_read_ms,
+            "read_back_identical": same, "gpu_ms_device": round(dev, 4), "gpu_ms_end_to_end_dropin": round(e2e, 3),
+            "rounds": st.rounds, "exact_vs_dijkstra": bool(np.array_equal(dist, ref)),
+            "cpu": {"ms": round(t_cpu * 1e3, 3), "cores": 1, "kind": "port", "rounds": rounds_cpu,
+                    "exact": bool(np.array_equal(d_cpu, ref)),
+                    "sample": f"{n_cpu} solves of worklist Bellman-Ford (oracle orc_sssp_worklist)"},
+            "gpu_vs_cpu_end_to_end": round(t_cpu * 1e3 / max(e2e, 1e-9), 2), "cli_gpu_sssp_f0": cli}
+
+
 def pagerank_section(ctx, g, args, smdp, orc, np):
     ig = smdp.invert(g)
     od = g.outdegree()
@@ -595,6 +658,11 @@ def main():
     if not args.skip_cpu:
         line["cpu_baseline"] = cpu_baseline(g, g.source, orc, np)
         log(f"cpu baseline: {line['cpu_baseline']['value']} GTEPS")
+    try:
+        line["c1"] = c1_section(ctx, smdp, orc, np)
+        log(f"c1: {line['c1']}")
+    except smdp.SmdpError as e:
+        line["c1"] = {"error": str(e)}
     if not args.skip_pagerank:
         try:
             line["pagerank"] = pagerank_section(ctx, g, args, smdp, orc, np)

@@ -4,7 +4,11 @@
 // damping 0.5 and total rank 1000 (pagerank_wrapper.cu:7, pagerank.h:15) and
 // its initial ranks T/N (pagerank.cpp:103-106); the stop rule is an FP64 L1
 // residual (--tol, relative to the total rank) or --max-iter iterations.
+// Under -v the same number of iterations is repeated on the host in FP64 and
+// the L1 and L-inf distances of the device ranks from it are printed.
+#include <math.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include "cli_common.h"
 
@@ -30,6 +34,33 @@ static void usage() {
   fprintf(stderr, "                 6 - tuned: LDS-staged row tiles + grid chunks\n");
   fprintf(stderr, "    --device,-e <number>               select the device\n");
   fprintf(stderr, "    --damping <d>  --total-rank <T>  --tol <l1>  --max-iter <k>  --init-zero\n");
+}
+
+// `iters` FP64 pull iterations with the device's update rule: the mass of
+// out-degree-0 vertices is spread evenly, r' = d*(sum_in r/outdeg + dang/N)
+// + (1-d)*T/N.
+static void host_pagerank_f64(const smdp_hgraph* ig, const int32_t* outdeg, double d, double T, int init_zero,
+                              uint64_t iters, double* r) {
+  const int32_t n = ig->n;
+  double* nr = (double*)malloc(sizeof(double) * (n > 0 ? n : 1));
+  double* contrib = (double*)malloc(sizeof(double) * (n > 0 ? n : 1));
+  for (int32_t v = 0; v < n; ++v) r[v] = init_zero ? 0.0 : T / n;
+  for (uint64_t it = 0; it < iters; ++it) {
+    double dang = 0;
+    for (int32_t u = 0; u < n; ++u) {
+      if (outdeg[u] == 0) dang += r[u];
+      contrib[u] = outdeg[u] ? r[u] / outdeg[u] : 0.0;
+    }
+    const double base = (1.0 - d) * T / n + d * dang / n;
+    for (int32_t v = 0; v < n; ++v) {
+      double s = 0;
+      for (int32_t e = ig->row[v]; e < ig->row[v + 1]; ++e) s += contrib[ig->col[e]];
+      nr[v] = d * s + base;
+    }
+    memcpy(r, nr, sizeof(double) * n);
+  }
+  free(nr);
+  free(contrib);
 }
 
 int main(int argc, char** argv) {
@@ -65,6 +96,17 @@ int main(int argc, char** argv) {
     fprintf(stdout, "HIP kernel execution:\t\t%lf\n", st.ms_kernel / 1e3);
     fprintf(stdout, "HIP total (malloc + H2D + kernels + D2H):\t\t%lf\n", st.ms_total / 1e3);
     fprintf(stdout, "L1 residual: %.3e  dangling mass: %.6e\n", st.residual, st.dangling);
+    double* ref = (double*)malloc(sizeof(double) * (g->n > 0 ? g->n : 1));
+    host_pagerank_f64(ig, outdeg, o.damping, o.total_rank, o.init_zero, st.rounds, ref);
+    double l1 = 0, linf = 0;
+    for (int32_t v = 0; v < g->n; ++v) {
+      const double e = fabs((double)rank[v] - ref[v]);
+      l1 += e;
+      linf = fmax(linf, e);
+    }
+    fprintf(stdout, "vs FP64 host (%llu iterations): L1 %.3e  L-inf %.3e  (total rank %g)\n",
+            (unsigned long long)st.rounds, l1, linf, (double)o.total_rank);
+    free(ref);
     cli_print_counters(stdout, st);
   }
   if (c.debug) {

@@ -17,6 +17,7 @@
 #include <mutex>
 
 #include "smdp_internal.h"
+#include "stage_team.h"
 
 namespace smdp {
 
@@ -169,58 +170,17 @@ static int staged_upload(smdp_ctx* ctx, int32_t* dst, const int32_t* src, int64_
     }
     if (!ctx->stage_ev[i]) SMDP_HIP(hipEventCreateWithFlags(&ctx->stage_ev[i], hipEventDisableTiming));
   }
-  const int64_t nchunks = (count + kStageInts - 1) / kStageInts;
   const int nt = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)std::thread::hardware_concurrency(), 16));
-  std::atomic<int> arrived{0};
-  std::atomic<int64_t> phase{0};
-  std::atomic<int> err{SMDP_OK};
-  // all nt threads meet; thread 0 runs `step` in between (HIP calls stay on one thread)
-  auto meet = [&](int t, int64_t& ph, auto step) {
-    if (t == 0) {
-      while (arrived.load(std::memory_order_acquire) != nt - 1) std::this_thread::yield();
-      arrived.store(0, std::memory_order_relaxed);
-      step();
-      phase.store(++ph, std::memory_order_release);
-    } else {
-      arrived.fetch_add(1, std::memory_order_acq_rel);
-      while (phase.load(std::memory_order_acquire) == ph) std::this_thread::yield();
-      ++ph;
-    }
-  };
-  auto worker = [&](int t) {
-    int64_t ph = 0;
-    for (int64_t c = 0; c < nchunks; ++c) {
-      const int64_t off = c * kStageInts, len = std::min(kStageInts, count - off);
-      int32_t* buf = ctx->stage[c & 1];
-      // the buffer is free once the copy issued from it two chunks ago is done
-      meet(t, ph, [&] {
-        if (c >= 2 && err.load() == SMDP_OK && hipEventSynchronize(ctx->stage_ev[c & 1]) != hipSuccess)
-          err = SMDP_ERR_DEVICE;
+  const bool ok = stage_team(
+      nt, src, count, kStageInts, ctx->stage, check_n, bad,
+      [&](int b) { return hipEventSynchronize(ctx->stage_ev[b]) == hipSuccess; },
+      [&](int64_t off, int b, int64_t len) {
+        return hipMemcpyAsync(dst + off, ctx->stage[b], sizeof(int32_t) * (size_t)len, hipMemcpyHostToDevice,
+                              ctx->stream) == hipSuccess &&
+               hipEventRecord(ctx->stage_ev[b], ctx->stream) == hipSuccess;
       });
-      const int64_t b = len * t / nt, e = len * (t + 1) / nt;
-      if (e > b) {
-        memcpy(buf + b, src + off + b, sizeof(int32_t) * (size_t)(e - b));
-        if (check_n >= 0) {
-          bool local = false;
-          for (int64_t k = b; k < e; ++k) local |= (uint32_t)buf[k] >= (uint64_t)check_n;
-          if (local) *bad = true;
-        }
-      }
-      meet(t, ph, [&] {
-        if (err.load() == SMDP_OK &&
-            (hipMemcpyAsync(dst + off, buf, sizeof(int32_t) * (size_t)len, hipMemcpyHostToDevice, ctx->stream) !=
-                 hipSuccess ||
-             hipEventRecord(ctx->stage_ev[c & 1], ctx->stream) != hipSuccess))
-          err = SMDP_ERR_DEVICE;
-      });
-    }
-  };
-  std::vector<std::thread> team;
-  for (int t = 1; t < nt; ++t) team.emplace_back(worker, t);
-  worker(0);
-  for (auto& x : team) x.join();
   SMDP_HIP(hipStreamSynchronize(ctx->stream));
-  return err.load();
+  return ok ? SMDP_OK : SMDP_ERR_DEVICE;
 }
 
 // One array of 4-byte entries (values, x, out-degrees, priorities).

@@ -71,9 +71,14 @@ def test_gpu_spmv_and_pagerank_cli(data_test_path):
     r = _gpu_run("gpu-SpMV", data_test_path, "-s", "4", "-v")
     err = float(re.search(r"L-inf error vs FP64: (\S+)", r.stdout).group(1))
     assert err == 0.0  # the reference's inputs make every sum exact
-    r = _gpu_run("gpu-pg", data_test_path, "-s", "6", "-d")
-    ranks = np.array([float(x) for x in re.findall(r"Node \d+: (\S+)", r.stdout)])
-    assert len(ranks) == 20 and abs(ranks.sum() - 1000.0) < 1e-2
+    for sol in ("0", "4", "6"):
+        r = _gpu_run("gpu-pg", data_test_path, "-s", sol, "-v", "-d")
+        ranks = np.array([float(x) for x in re.findall(r"Node \d+: (\S+)", r.stdout)])
+        assert len(ranks) == 20 and abs(ranks.sum() - 1000.0) < 1e-2
+        # FP32 ranks of size ~50 against the FP64 host iterates: a few ulps each
+        m = re.search(r"vs FP64 host \((\d+) iterations\): L1 (\S+)  L-inf (\S+)", r.stdout)
+        assert m and int(m.group(1)) > 0, r.stdout
+        assert float(m.group(2)) < 1e-3 and float(m.group(3)) < 1e-4, m.group(0)
 
 
 @pytest.mark.gpu
@@ -91,3 +96,25 @@ def test_gpu_sssp_cli_counters(golden, data_test_path, sol):
     assert sum(int(x) for x in m.groups()[:3]) > 0
     k = re.search(r"Kernel launches: (\d+)  atomics tried: (\d+)  won: (\d+)", r.stdout)
     assert k and int(k.group(1)) > 0 and int(k.group(3)) <= int(k.group(2))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sol", ["0", "3", "6"])
+def test_gpu_sssp_cli_uniform16k_gr(tmp_path, sol):
+    """The reference's small SSSP case end to end through files: 16384
+    vertices with 8 uniform out-arcs each, weights U[1,100], written as a
+    DIMACS9 .gr by smdp-gen and read back by gpu-sssp -f 0 (sssp.cpp:100-154);
+    every distance printed matches Dijkstra on the same file."""
+    import orc
+    import smdp
+    path = tmp_path / "uniform16k.gr"
+    assert run("smdp-gen", "uniform", "16384", "8", "1", "100", "1", str(path)).returncode == 0
+    g = smdp.read_graph(str(path), 0)
+    assert (g.n, g.m, g.source) == (16384, 131072, 0)
+    r = _gpu_run("gpu-sssp", str(path), "-s", sol, "-d")
+    got = re.findall(r"From Node (\d+) -> (\d+) is (\d+)", r.stdout)
+    assert len(got) == g.n
+    ref = orc.dijkstra(g.row, g.col, g.w, g.source)
+    dist = np.array([int(c) for _, _, c in got])
+    to = np.array([int(b) for _, b, _ in got]) - 1
+    assert np.array_equal(dist, ref[to])
