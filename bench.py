@@ -549,9 +549,13 @@ def only_section(args, smdp, orc, np):
                     dg.sssp(g.source, smdp.SOL_TUNED)
                     q, arcs, ms = ctx.round_log()
                     ctx.set_profiling(False)
+                    modes = ("sweep-light", "light", "settle", "done")
                     for i in range(len(q)):
                         if q[i] or arcs[i]:
-                            log(f"  round {i}: queue {q[i]} arcs {arcs[i]} {ms[i]:.3f} ms "
+                            # bucketed phases log mode << 60 | bucket << 32 | entries
+                            tag = (f"{modes[q[i] >> 60]} bucket {(q[i] >> 32) & 0xFFFFFFF} "
+                                   if ctx.get_params(smdp.APP_SSSP).layout == smdp.LAYOUT_BUCKETS else "")
+                            log(f"  round {i}: {tag}queue {q[i] & 0xFFFFFFFF} arcs {arcs[i]} {ms[i]:.3f} ms "
                                 f"{arcs[i] / max(ms[i], 1e-9) / 1e6:.1f} Garcs/s")
         else:
             ig = smdp.invert(g)
@@ -589,6 +593,8 @@ def only_section(args, smdp, orc, np):
             for _ in range(args.warmup + args.steps):
                 st = dg.bfs(0, smdp.SOL_TUNED)
             log(f"bfs zipf tree 4M: {st.ms_kernel:.3f} ms, {st.rounds} rounds")
+    elif args.only == "c1":
+        log(json.dumps(c1_section(ctx, smdp, orc, np)))
     elif args.only == "tree":
         t = smdp.gen_zipf_tree(4194304, 1.5, 4096, 5)
         with smdp.Graph(ctx, t) as dg:
@@ -617,7 +623,7 @@ def main():
                     help="PageRank arc layout: 0 auto, 1 plain CSR, 2 binned, 8 XCD blocks")
     ap.add_argument("--params", default="", help="profiling runs: JSON object of smdp_params fields to set")
     ap.add_argument("--round-log", action="store_true", help="profiling runs: print SSSP per-round log")
-    ap.add_argument("--only", choices=["sssp", "pagerank", "batched", "copy", "spmv", "color", "bfs", "tree"],
+    ap.add_argument("--only", choices=["sssp", "pagerank", "batched", "copy", "spmv", "color", "bfs", "tree", "c1"],
                     help="profiling runs: one section only, no JSON contract line")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -11,9 +11,9 @@
 // and each arc is relaxed about once.
 //
 // A bucket b (distances in [b*delta, (b+1)*delta)) is processed in phases:
-//   light sweep  every vertex is read; those in bucket b relax the arcs of
-//                the light part of their row (w < delta, stored first) whose
-//                candidate stays in bucket b;
+//   light sweep  every vertex is read; those in bucket b relax the light
+//                arcs of their row (w < delta) whose candidate stays in
+//                bucket b;
 //   light        vertices lowered within bucket b by the previous phase are
 //                expanded the same way from a queue (one entry per vertex per
 //                phase, deduplicated by a phase stamp), until a phase lowers
@@ -38,6 +38,16 @@
 // final distance (plus light re-expansions inside the bucket), so on RMAT-22
 // the arcs relaxed fall to about 1.06 per reachable arc.
 //
+// Each row is stored in weight bands: kBands bands of width delta / kBands
+// for the light arcs, lightest first, then the heavy arcs, with the band
+// starts in a table of kBands ints per row.  A vertex at distance du relaxes
+// in a light pass only arcs with w < lim - du and in the settle pass only
+// arcs with w >= lim - du (lim = the bucket's end), so each pass reads the
+// bands that hold its arcs and filters one partial band.  Reading whole light
+// parts instead read about four arcs per relaxed one in the light phases
+// near the source of an RMAT graph, where hubs are expanded again each time
+// their distance falls.
+//
 // Phases are launched back to back; each kernel computes its own phase from
 // the previous phase's descriptor and counters (values no kernel of the
 // current phase writes), so the host enqueues phases blindly and reads the
@@ -57,9 +67,12 @@
 
 namespace smdp {
 
-constexpr int kDIlp = 4;          // arcs per thread per step
+#ifndef SMDP_DELTA_ILP
+#define SMDP_DELTA_ILP 4
+#endif
+constexpr int kDIlp = SMDP_DELTA_ILP;  // arcs per thread per step
 #ifndef SMDP_DELTA_SMALL_SPLIT
-#define SMDP_DELTA_SMALL_SPLIT 32
+#define SMDP_DELTA_SMALL_SPLIT 256
 #endif
 constexpr int kDSmallSplit = SMDP_DELTA_SMALL_SPLIT;  // split degree when the light queue is short
 #ifndef SMDP_DELTA_DRAIN_TILE
@@ -97,7 +110,12 @@ __host__ __device__ inline DPhase next_phase(const DPhase& prev, int32_t qlen, i
   switch (prev.mode) {
     case kSweepLight:
     case kLight:
-      if (qlen > 0) {
+      if (qlen > n) {
+        // more pushes than the queue holds (a vertex can be queued twice in a
+        // race): its entries are incomplete, so the bucket is swept again
+        d.mode = kSweepLight;
+        d.nin = n;
+      } else if (qlen > 0) {
         d.mode = kLight;
         d.nin = qlen;
       } else if (prev.tail) {
@@ -132,7 +150,7 @@ struct DArgs {
   const int32_t* __restrict__ row;
   const int32_t* __restrict__ col;
   const int32_t* __restrict__ w;
-  const int32_t* __restrict__ lsplit;  // first heavy arc of each row (light arcs, w < delta, come first)
+  const int32_t* __restrict__ band;    // kBands band starts per row (light arcs by weight band, heavy last)
   const uint32_t* __restrict__ hasout; // bit v set when row v has arcs
   int32_t* dist;
   int32_t* stamp;  // phase label of the last light-queue push
@@ -147,11 +165,32 @@ struct DArgs {
   int32_t* progress;         // host memory: p + 1 when phase p starts, -(p + 1) when it is the kDone phase
   int32_t n;
   int32_t shift;    // delta = 1 << shift
+  int32_t bshift;   // band width = 1 << bshift (delta / kBands, at least 1)
   int32_t split;    // an entry with at least this many arcs is cut into grid chunks
   int32_t chunk;    // arcs per grid chunk
 };
 
 __device__ __forceinline__ bool d_has_arcs(const DArgs& a, int v) { return (a.hasout[v >> 5] >> (v & 31)) & 1u; }
+
+// First arc of band j (0..kBands) of row u; band kBands is the heavy part.
+__device__ __forceinline__ int band_start(const DArgs& a, int u, int j) {
+  return j == 0 ? a.row[u] : a.band[(int64_t)u * kBands + j - 1];
+}
+
+// A vertex at distance du in a bucket ending at lim relaxes, in a light pass,
+// the arcs with w < T = lim - du (0 < T <= delta): bands [0, ceil(T / width)),
+// the last of them only in part.  Returns that range's end.
+__device__ __forceinline__ int light_end(const DArgs& a, int u, int du, int lim) {
+  const int T = lim - du;
+  const int c = min(kBands, max(0, (T + (1 << a.bshift) - 1) >> a.bshift));
+  return band_start(a, u, c);
+}
+
+// ... and in the settle pass the arcs with w >= T: from band floor(T / width)
+// (in part) to the end of the row.  Returns that range's start.
+__device__ __forceinline__ int settle_start(const DArgs& a, int u, int du, int lim) {
+  return band_start(a, u, min(kBands, max(0, (lim - du) >> a.bshift)));
+}
 
 // Phase p, computed by every workgroup of phase p from phase p-1's descriptor
 // (written by workgroup 0 of phase p-1) and from slot p & 3, which phase p-1
@@ -180,8 +219,13 @@ __device__ __forceinline__ DPhase phase_of(const DArgs& a, int p) {
 
 // Candidate relaxations of one step, packed to the low lanes through the
 // wave's LDS area (s_pair: 2 * K * 64 ints) so that each lane issues one
-// atomic.  Light phases queue the winners for the next light phase; the
-// settle phase keeps the smallest winning distance of a vertex with arcs.
+// atomic.  Every candidate is below the distance its thread read before, so
+// the vertex's distance falls in this phase, through this candidate or a
+// lower one: the atomicMin returns nothing (the wave does not wait for it),
+// and every candidate counts as a winner.  Light phases queue the vertex for
+// the next light phase (once per phase, by a device-coherent stamp check; a
+// rare race queues it twice, which costs one more expansion); the settle phase keeps
+// the smallest candidate distance of a vertex with arcs.
 template <int K>
 __device__ __forceinline__ void relax_packed_d(const DArgs& a, const int* v, const int* alt, const bool* want,
                                                bool settle, int label, int32_t* s_pair, Sink& out, int& nbmin,
@@ -209,19 +253,17 @@ __device__ __forceinline__ void relax_packed_d(const DArgs& a, const int* v, con
       vv = s_pair[2 * i];
       const int aa = s_pair[2 * i + 1];
       ++out.tries;
-      const int old = atomicMin(&a.dist[vv], aa);
-      // read while the atomic is in flight (the stamp is only a filter)
-      const bool arcs = d_has_arcs(a, vv);
-      const int st = settle ? 0 : a.stamp[vv];
-      if (aa < old) {
-        ++out.wins;
-        if (arcs) {
-          if (settle) {
-            nbmin = min(nbmin, aa);
-            ++nrem;
-          } else if (st != label) {
-            push = atomicExch(&a.stamp[vv], label) != label;
-          }
+      ++out.wins;
+      __hip_atomic_fetch_min(&a.dist[vv], aa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (d_has_arcs(a, vv)) {
+        if (settle) {
+          nbmin = min(nbmin, aa);
+          ++nrem;
+        } else if (__hip_atomic_load(&a.stamp[vv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != label) {
+          // device-coherent read and write: a plain read could see another
+          // compute unit's stale cached stamp and queue the vertex again
+          __hip_atomic_store(&a.stamp[vv], label, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          push = true;
         }
       }
     }
@@ -264,9 +306,13 @@ __device__ __forceinline__ void store_next_bucket(Ctl* c, int nbmin, int nrem, i
   __syncthreads();
 }
 
-__device__ __forceinline__ void log_phase(const DArgs& a, int p, int L, unsigned long long wave_arcs) {
+// Profiling log of phase p: word 0 = mode << 60 | bucket << 32 | entries
+// (written by the step kernel, d != nullptr), word 1 += arcs relaxed.
+__device__ __forceinline__ void log_phase(const DArgs& a, int p, const DPhase* d, unsigned long long wave_arcs) {
   if (!a.rlog || p >= kRoundLogCap) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0 && L >= 0) a.rlog[2 * p] = (unsigned long long)L;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && d)
+    a.rlog[2 * p] = (unsigned long long)d->mode << 60 | (unsigned long long)(uint32_t)d->bucket << 32 |
+                    (uint32_t)d->nin;
   if (lane_id() == 0 && wave_arcs) atomicAdd(&a.rlog[2 * p + 1], wave_arcs);
 }
 
@@ -408,101 +454,116 @@ __device__ __forceinline__ void expand_tile(const DArgs& a, DShared& sh, DTile& 
   __syncthreads();
 }
 
-// Pass 1 of phase d (all threads of the workgroup).  Sweeps: tiles of
-// SMDP_WG consecutive vertices dealt grid-stride (the renumbering puts the
-// rows with most arcs first, so contiguous ranges per workgroup would give
-// the first workgroups most of the work), kSweepPer tiles read at once (most
-// hold no vertex of the bucket and cost one round of loads); the selected
-// vertices are compacted in LDS and expanded in tiles.  Light phases: tiles
-// of SMDP_WG queue entries claimed from the phase's counters.
+// A vertex at distance dv is expanded by a pass over `bucket`: it lies in the
+// bucket (in the open last bucket, at or beyond it).
+__device__ __forceinline__ bool selected(int dv, int bucket, bool tail, int shift) {
+  const int b = dv >> shift;
+  return b == bucket || (tail && b > bucket);
+}
+
+// Expands the `count` vertices of sh.list / sh.ldu in tiles of SMDP_WG: whole
+// rows in the open last bucket, else the settle pass's or the light pass's
+// bands.  All threads.
+__device__ __forceinline__ void expand_list(const DArgs& a, DShared& sh, DTile& t, int count, bool settle,
+                                            Sink& out) {
+  for (int e0 = 0; e0 < count; e0 += SMDP_WG) {
+    const int i = e0 + (int)threadIdx.x;
+    int u = 0, s = 0, deg = 0, du = 0;
+    if (i < count) {
+      u = sh.list[i];
+      du = sh.ldu[i];
+      if (t.tail) {
+        s = a.row[u];
+        deg = a.row[u + 1] - s;
+      } else if (settle) {
+        s = settle_start(a, u, du, t.lim);
+        deg = a.row[u + 1] - s;
+      } else {
+        s = a.row[u];
+        deg = light_end(a, u, du, t.lim) - s;
+      }
+      ++t.verts;
+      ++t.nsel;
+    }
+    expand_tile(a, sh, t, u, s, deg, du, out);
+  }
+}
+
+// Pass 1 of phase d (all threads of the workgroup).  Its entries are the
+// vertices 0 .. n-1 (sweeps) or the light queue's entries, dealt in tiles of
+// SMDP_WG consecutive entries grid-stride (the renumbering puts the rows with
+// most arcs first, so contiguous ranges per workgroup would give the first
+// workgroups most of the work).  kSweepPer tiles are read at once, so a
+// thread has that many entries' loads in flight (most sweep tiles hold no
+// vertex of the bucket and cost one round of loads); the selected vertices
+// are compacted in LDS and expanded in tiles.
 __device__ __forceinline__ void phase_tiles(const DArgs& a, const DPhase& d, int p, DShared& sh, DTile& t,
                                             Sink& out) {
   const int L = d.nin;
   const int bucket = d.bucket;
   const bool settle = d.mode == kSettle;
-  if (d.mode != kLight) {
-    const int ntiles = (L + SMDP_WG - 1) / SMDP_WG;
-    for (int t0 = blockIdx.x; t0 < ntiles; t0 += kSweepPer * gridDim.x) {
-      int dv[kSweepPer];
-      int cnt = 0;
+  const int32_t* qin = d.mode == kLight ? ((p & 1) ? a.q1 : a.q0) : nullptr;  // queued vertices have arcs
+  const int ntiles = (L + SMDP_WG - 1) / SMDP_WG;
+  for (int t0 = blockIdx.x; t0 < ntiles; t0 += kSweepPer * gridDim.x) {
+    int vv[kSweepPer], dv[kSweepPer];
+    int cnt = 0;
 #pragma unroll
-      for (int k = 0; k < kSweepPer; ++k) {
-        const int v = (t0 + k * (int)gridDim.x) * SMDP_WG + (int)threadIdx.x;
-        dv[k] = SMDP_SSSP_INF;
-        if (t0 + k * (int)gridDim.x < ntiles && v < L && d_has_arcs(a, v)) dv[k] = a.dist[v];
-      }
-      // selected: in the bucket (in the open last bucket, at or beyond it)
+    for (int k = 0; k < kSweepPer; ++k) {
+      const int i = (t0 + k * (int)gridDim.x) * SMDP_WG + (int)threadIdx.x;
+      const bool in = t0 + k * (int)gridDim.x < ntiles && i < L;
+      vv[k] = qin ? (in ? qin[i] : -1) : (in ? i : -1);
+    }
 #pragma unroll
-      for (int k = 0; k < kSweepPer; ++k) {
-        const int b = dv[k] >> a.shift;
-        if (dv[k] != SMDP_SSSP_INF) {
-          if (b == bucket || (t.tail && b > bucket)) {
-            ++cnt;
-          } else if (settle && b > bucket) {
-            t.nbmin = min(t.nbmin, dv[k]);
-            ++t.nrem;
-          }
-        }
-      }
-      int total;
-      (void)wg_exclusive_scan(cnt, sh.scan, &total);
-      if (total == 0) continue;
-      for (int g = 0; g < kSweepPer; g += kSweepGroup) {
-        int c = 0;
+    for (int k = 0; k < kSweepPer; ++k) {
+      dv[k] = SMDP_SSSP_INF;
+      if (vv[k] >= 0 && (qin || d_has_arcs(a, vv[k]))) dv[k] = a.dist[vv[k]];
+    }
+    // selected: in the bucket (in the open last bucket, at or beyond it)
 #pragma unroll
-        for (int k = 0; k < kSweepGroup; ++k)
-          c += dv[g + k] != SMDP_SSSP_INF && ((dv[g + k] >> a.shift) == bucket || (t.tail && (dv[g + k] >> a.shift) > bucket));
-        int gtot;
-        int pos = wg_exclusive_scan(c, sh.scan, &gtot);
-        if (gtot == 0) continue;
-#pragma unroll
-        for (int k = 0; k < kSweepGroup; ++k) {
-          if (dv[g + k] != SMDP_SSSP_INF &&
-              ((dv[g + k] >> a.shift) == bucket || (t.tail && (dv[g + k] >> a.shift) > bucket))) {
-            sh.list[pos] = (t0 + (g + k) * (int)gridDim.x) * SMDP_WG + (int)threadIdx.x;
-            sh.ldu[pos] = dv[g + k];
-            ++pos;
-          }
-        }
-        __syncthreads();
-        for (int e0 = 0; e0 < gtot; e0 += SMDP_WG) {
-          const int i = e0 + (int)threadIdx.x;
-          int u = 0, s = 0, deg = 0, du = 0;
-          if (i < gtot) {
-            u = sh.list[i];
-            du = sh.ldu[i];
-            s = a.row[u];
-            deg = (settle || t.tail ? a.row[u + 1] : a.lsplit[u]) - s;
-            ++t.verts;
-            ++t.nsel;
-          }
-          expand_tile(a, sh, t, u, s, deg, du, out);
-        }
+    for (int k = 0; k < kSweepPer; ++k) {
+      if (dv[k] == SMDP_SSSP_INF) continue;
+      if (selected(dv[k], bucket, t.tail, a.shift)) {
+        ++cnt;
+      } else if (settle && (dv[k] >> a.shift) > bucket) {
+        t.nbmin = min(t.nbmin, dv[k]);
+        ++t.nrem;
       }
     }
-  } else {
-    const int32_t* qin = (p & 1) ? a.q1 : a.q0;
-    const int ntiles = (L + SMDP_WG - 1) / SMDP_WG;
-    int cur = 0;  // claim cursor of thread 0
-    for (;;) {
-      if (threadIdx.x == 0) {
-        int f = 0, e = 0;
-        sh.tile = claim_items(t.cin->tiles, ntiles, 1, cur, &f, &e) ? f : -1;
+    int total;
+    int pos = wg_exclusive_scan(cnt, sh.scan, &total);
+    if (total == 0) continue;
+    if (total <= kSweepList) {
+      // the whole batch's selection fits one list (sparse buckets)
+#pragma unroll
+      for (int k = 0; k < kSweepPer; ++k) {
+        if (dv[k] != SMDP_SSSP_INF && selected(dv[k], bucket, t.tail, a.shift)) {
+          sh.list[pos] = vv[k];
+          sh.ldu[pos] = dv[k];
+          ++pos;
+        }
       }
       __syncthreads();
-      const int tile = sh.tile;
-      __syncthreads();
-      if (tile < 0) break;
-      const int i = tile * SMDP_WG + (int)threadIdx.x;
-      int u = 0, s = 0, deg = 0, du = 0;
-      if (i < L) {
-        u = qin[i];
-        s = a.row[u];
-        deg = (t.tail ? a.row[u + 1] : a.lsplit[u]) - s;
-        du = a.dist[u];
-        ++t.verts;
+      expand_list(a, sh, t, total, settle, out);
+      continue;
+    }
+    for (int g = 0; g < kSweepPer; g += kSweepGroup) {
+      int c = 0;
+#pragma unroll
+      for (int k = 0; k < kSweepGroup; ++k)
+        c += dv[g + k] != SMDP_SSSP_INF && selected(dv[g + k], bucket, t.tail, a.shift);
+      int gtot;
+      int gpos = wg_exclusive_scan(c, sh.scan, &gtot);
+      if (gtot == 0) continue;
+#pragma unroll
+      for (int k = 0; k < kSweepGroup; ++k) {
+        if (dv[g + k] != SMDP_SSSP_INF && selected(dv[g + k], bucket, t.tail, a.shift)) {
+          sh.list[gpos] = vv[g + k];
+          sh.ldu[gpos] = dv[g + k];
+          ++gpos;
+        }
       }
-      expand_tile(a, sh, t, u, s, deg, du, out);
+      __syncthreads();
+      expand_list(a, sh, t, gtot, settle, out);
     }
   }
 }
@@ -516,11 +577,12 @@ __device__ __forceinline__ unsigned long long phase_chunks(const DArgs& a, const
   while (ept > 1 && (H + ept / 2 - 1) / (ept / 2) <= (int)gridDim.x) ept >>= 1;
   const int ntiles = (H + ept - 1) / ept;
   const unsigned long long e0 = t.edges;
-  int part = 0;
+  uint32_t dead = 0;  // claim state of wave 0
   for (;;) {
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < SMDP_WAVE) {
       int f = 0, e = 0;
-      sh.tile = claim_items(t.cin->chunks, ntiles, 1, part, &f, &e) ? f : -1;
+      const bool got = claim_items_wave(t.cin->chunks, ntiles, 1, dead, &f, &e);
+      if (threadIdx.x == 0) sh.tile = got ? f : -1;
     }
     __syncthreads();
     const int tile = sh.tile;
@@ -530,11 +592,13 @@ __device__ __forceinline__ unsigned long long phase_chunks(const DArgs& a, const
       int len = 0, st = 0, du = 0;
       if ((int)threadIdx.x < ept && idx < H) {
         const Chunk c = a.hq[idx];
-        const int vend = t.settle || t.tail ? a.row[c.v + 1] : a.lsplit[c.v];
+        du = a.dist[c.v];
+        // a light entry's distance may have fallen since it was cut: its
+        // range only grows, and the vertex is queued again for the rest
+        const int vend = t.settle || t.tail ? a.row[c.v + 1] : light_end(a, c.v, du, t.lim);
         const int end = vend - c.first > a.chunk ? c.first + a.chunk : vend;
         len = end - c.first;
         st = c.first;
-        du = a.dist[c.v];
       }
       const int incl = wave_inclusive_scan(len);
       if ((int)threadIdx.x < kDDrainTile) {
@@ -590,9 +654,9 @@ __global__ __launch_bounds__(SMDP_WG) void k_delta_step(DArgs a, int p) {
   tile_init(a, d, p, sh, t);
   t.edges = t.skipped = t.pg = t.verts = 0;
   Sink out;
-  out.init(sh.out + (threadIdx.x / SMDP_WAVE) * kOutCap, (p & 1) ? a.q0 : a.q1, &t.cout->qlen);
+  out.init(sh.out + (threadIdx.x / SMDP_WAVE) * kOutCap, (p & 1) ? a.q0 : a.q1, &t.cout->qlen, a.n);
   phase_tiles(a, d, p, sh, t, out);
-  log_phase(a, p, d.mode == kLight ? d.nin : -1, wave_sum(t.edges));
+  log_phase(a, p, &d, wave_sum(t.edges));
   end_pass(a, sh, t, out);
   phase_counters(a, sh, t, out, 0);
 }
@@ -610,9 +674,9 @@ __global__ __launch_bounds__(SMDP_WG) void k_delta_drain(DArgs a, int p) {
   tile_init(a, d, p, sh, t);
   t.edges = t.skipped = t.pg = t.verts = 0;
   Sink out;
-  out.init(sh.out + (threadIdx.x / SMDP_WAVE) * kOutCap, (p & 1) ? a.q0 : a.q1, &t.cout->qlen);
+  out.init(sh.out + (threadIdx.x / SMDP_WAVE) * kOutCap, (p & 1) ? a.q0 : a.q1, &t.cout->qlen, a.n);
   const unsigned long long e2 = phase_chunks(a, d, H, sh, t, out);
-  log_phase(a, p, -1, wave_sum(e2));
+  log_phase(a, p, nullptr, wave_sum(e2));
   end_pass(a, sh, t, out);
   phase_counters(a, sh, t, out, e2);
 }
@@ -629,7 +693,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_delta_run(DArgs a, int p0, int pend
   t.edges = t.skipped = t.pg = t.verts = 0;
   unsigned long long e2 = 0;
   Sink out;
-  out.init(sh.out + (threadIdx.x / SMDP_WAVE) * kOutCap, a.q0, &a.ctl[0].qlen);
+  out.init(sh.out + (threadIdx.x / SMDP_WAVE) * kOutCap, a.q0, &a.ctl[0].qlen, a.n);
   uint32_t epoch = 0;
   int p = p0;
   for (; p < pend; ++p) {
@@ -656,50 +720,76 @@ __global__ __launch_bounds__(SMDP_WG) void k_delta_run(DArgs a, int p0, int pend
         break;
       }
     }
-    log_phase(a, p, d.mode == kLight ? d.nin : -1, wave_sum(t.edges - eb));
+    log_phase(a, p, &d, wave_sum(t.edges - eb));
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) flags[1] = p;
   phase_counters(a, sh, t, out, e2);
 }
 
-// Rows of the renumbered copy partitioned in place of order: light arcs (w <
-// delta) first, heavy arcs after, each in their original order; lsplit[u] =
-// first heavy arc of row u.  One wave per row.
-__global__ __launch_bounds__(SMDP_WG) void k_partition_rows(const int32_t* __restrict__ row,
-                                                            const int32_t* __restrict__ col_in,
-                                                            const int32_t* __restrict__ w_in, int32_t* col_out,
-                                                            int32_t* w_out, int32_t* lsplit, int32_t n,
-                                                            int32_t delta) {
+// Group of an arc for the band layout: its light band (w >> bshift, below
+// kBands) or kBands for a heavy arc (w >= delta).
+__device__ __forceinline__ int arc_group(int w, int delta, int bshift) { return w >= delta ? kBands : w >> bshift; }
+
+// Rows of the renumbered copy reordered by weight band: band j < kBands holds
+// the light arcs with j * width <= w < (j + 1) * width (width = 1 << bshift =
+// delta / kBands, at least 1), band kBands the heavy arcs (w >= delta); each
+// band keeps the arcs' original order.  band[u * kBands + j - 1] = first arc
+// of band j of row u.  One wave per row; lane j counts group j.
+__global__ __launch_bounds__(SMDP_WG) void k_band_rows(const int32_t* __restrict__ row,
+                                                       const int32_t* __restrict__ col_in,
+                                                       const int32_t* __restrict__ w_in, int32_t* col_out,
+                                                       int32_t* w_out, int32_t* band, int32_t n, int32_t delta,
+                                                       int32_t bshift) {
+  static_assert(kBands < SMDP_WAVE, "one lane per group");
   const int lane = lane_id();
   const int64_t nw = (int64_t)gridDim.x * SMDP_WAVES_PER_WG;
   for (int64_t u = (int64_t)blockIdx.x * SMDP_WAVES_PER_WG + threadIdx.x / SMDP_WAVE; u < n; u += nw) {
     const int s = row[u], e = row[u + 1];
-    int nl = 0;
+    int cnt = 0;
     for (int b = s; b < e; b += SMDP_WAVE) {
       const int k = b + lane;
-      nl += __popcll(ballot(k < e && w_in[k] < delta));
+      const int g = k < e ? arc_group(w_in[k], delta, bshift) : -1;
+#pragma unroll
+      for (int j = 0; j <= kBands; ++j) {
+        const int c = __popcll(ballot(g == j));
+        if (lane == j) cnt += c;
+      }
     }
-    int lo = s, hi = s + nl;
+    const int mine = lane <= kBands ? cnt : 0;
+    int base = s + wave_inclusive_scan(mine) - mine;  // lane j: first arc of group j
+    if (lane >= 1 && lane <= kBands) band[u * kBands + lane - 1] = base;
     for (int b = s; b < e; b += SMDP_WAVE) {
       const int k = b + lane;
       const bool in = k < e;
       const int wk = in ? w_in[k] : 0;
-      const bool light = in && wk < delta;
-      const uint64_t ml = ballot(light), mh = ballot(in && !light);
+      const int g = in ? arc_group(wk, delta, bshift) : -1;
+      int pos = 0;
+#pragma unroll
+      for (int j = 0; j <= kBands; ++j) {
+        const uint64_t m = ballot(g == j);
+        const int bj = __shfl(base, j);
+        if (g == j) pos = bj + mbcnt(m);
+        if (lane == j) base += __popcll(m);
+      }
       if (in) {
-        const int pos = light ? lo + mbcnt(ml) : hi + mbcnt(mh);
         col_out[pos] = col_in[k];
         w_out[pos] = wk;
       }
-      lo += __popcll(ml);
-      hi += __popcll(mh);
     }
-    if (lane == 0) lsplit[u] = s + nl;
   }
 }
 
-// Partitions the renumbered copy's rows for bucket width delta (once per
-// delta) and allocates the phase descriptors.
+// log2 of the band width for bucket width delta (a power of two).
+static int band_shift(int delta) {
+  int shift = 0;
+  while ((1 << shift) < delta) ++shift;
+  int b = 0;
+  while ((1 << b) < kBands) ++b;
+  return shift > b ? shift - b : 0;
+}
+
+// Reorders the renumbered copy's rows into weight bands for bucket width
+// delta (once per delta) and allocates the phase descriptors.
 static int ensure_buckets(smdp_graph* g, Relabel* rl, int delta) {
   hipStream_t s = g->ctx->stream;
   // 4 phase descriptors, the persistent launch's barrier lines and flags
@@ -708,8 +798,8 @@ static int ensure_buckets(smdp_graph* g, Relabel* rl, int delta) {
     rl->hctl = nullptr;
     return SMDP_ERR_NOMEM;
   }
-  if (rl->lsplit && rl->lsplit_delta == delta) return SMDP_OK;
-  SMDP_CHECK(alloc_once(&rl->lsplit, std::max<int64_t>(1, g->n)));
+  if (rl->band && rl->band_delta == delta) return SMDP_OK;
+  SMDP_CHECK(alloc_once(&rl->band, std::max<int64_t>(1, (int64_t)g->n * kBands)));
   const int64_t m = std::max<int64_t>(1, g->m);
   int32_t *col2 = nullptr, *w2 = nullptr;
   if (hipMalloc((void**)&col2, sizeof(int32_t) * m) != hipSuccess) return SMDP_ERR_NOMEM;
@@ -718,8 +808,8 @@ static int ensure_buckets(smdp_graph* g, Relabel* rl, int delta) {
     return SMDP_ERR_NOMEM;
   }
   const int64_t blocks = std::min<int64_t>(((int64_t)g->n + SMDP_WAVES_PER_WG - 1) / SMDP_WAVES_PER_WG, 16384);
-  hipLaunchKernelGGL(k_partition_rows, dim3((unsigned)std::max<int64_t>(1, blocks)), dim3(SMDP_WG), 0, s, rl->row,
-                     rl->col, rl->w, col2, w2, rl->lsplit, g->n, delta);
+  hipLaunchKernelGGL(k_band_rows, dim3((unsigned)std::max<int64_t>(1, blocks)), dim3(SMDP_WG), 0, s, rl->row,
+                     rl->col, rl->w, col2, w2, rl->band, g->n, delta, band_shift(delta));
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
     (void)hipFree(col2);
     (void)hipFree(w2);
@@ -729,7 +819,7 @@ static int ensure_buckets(smdp_graph* g, Relabel* rl, int delta) {
   (void)hipFree(rl->w);
   rl->col = col2;
   rl->w = w2;
-  rl->lsplit_delta = delta;
+  rl->band_delta = delta;
   return SMDP_OK;
 }
 
@@ -751,7 +841,7 @@ int run_delta(smdp_graph* g, Relabel* rl, int32_t src, const smdp_params& p, int
   a.row = rl->row;
   a.col = rl->col;
   a.w = rl->w;
-  a.lsplit = rl->lsplit;
+  a.band = rl->band;
   a.hasout = rl->hasout;
   a.dist = g->dist;
   a.stamp = g->stamp;
@@ -766,6 +856,7 @@ int run_delta(smdp_graph* g, Relabel* rl, int32_t src, const smdp_params& p, int
   a.small_q = grid * SMDP_WG;
   a.n = g->n;
   a.shift = shift;
+  a.bshift = band_shift(delta);
   a.split = p.split_threshold;
   a.chunk = p.chunk;
   hipStream_t s = ctx->stream;

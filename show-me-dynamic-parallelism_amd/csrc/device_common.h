@@ -54,10 +54,12 @@ template <int CAP>
 struct WaveOut {
   int32_t* buf;  // this wave's CAP-entry LDS region
   int n;
+  int32_t qcap;  // entries the global queue holds: later ones are counted, not stored
 
-  __device__ __forceinline__ void init(int32_t* region) {
+  __device__ __forceinline__ void init(int32_t* region, int32_t queue_cap = INT32_MAX) {
     buf = region;
     n = 0;
+    qcap = queue_cap;
   }
 
   __device__ __forceinline__ void flush(int32_t* __restrict__ q, int32_t* counter) {
@@ -72,7 +74,8 @@ struct WaveOut {
     int base = 0;
     if (lane_id() == leader) base = atomicAdd(counter, n);
     base = __shfl(base, leader);
-    for (int j = me; j < n; j += stride) q[base + j] = buf[j];
+    for (int j = me; j < n; j += stride)
+      if (base + j < qcap) q[base + j] = buf[j];
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

@@ -19,8 +19,9 @@ struct Sink {
   unsigned long long pushes;
   unsigned long long tries;  // atomicMin attempts (plain read said "improves")
   unsigned long long wins;   // atomicMin that lowered the value
-  __device__ __forceinline__ void init(int32_t* lds_region, int32_t* queue, int32_t* counter) {
-    wo.init(lds_region);
+  __device__ __forceinline__ void init(int32_t* lds_region, int32_t* queue, int32_t* counter,
+                                       int32_t queue_cap = INT32_MAX) {
+    wo.init(lds_region, queue_cap);
     q = queue;
     qlen = counter;
     pushes = tries = wins = 0;
@@ -81,9 +82,45 @@ __device__ __forceinline__ void wg_flush(Sink& out, int* s_n) {
   }
   __syncthreads();
   const int base = s_n[W] + s_n[wave];
-  for (int j = lane_id(); j < out.wo.n; j += SMDP_WAVE) out.q[base + j] = out.wo.buf[j];
+  for (int j = lane_id(); j < out.wo.n; j += SMDP_WAVE)
+    if (base + j < out.wo.qcap) out.q[base + j] = out.wo.buf[j];
   out.wo.n = 0;
   __syncthreads();
+}
+
+// claim_items for a whole wave: lanes 0 .. kClaimParts-1 read the range
+// counters at once (lane j: range (blockIdx.x + j) % kClaimParts), and the
+// first range with items left is claimed with one atomic, so an exhausted
+// grid costs one round of reads instead of one read per range in turn.
+// `dead` (bit j: range of lane j found exhausted) starts at 0 and is kept by
+// the caller between claims.  Every lane of the wave calls it with the same
+// arguments and gets the same result.
+__device__ __forceinline__ bool claim_items_wave(int32_t* ctr, int total, int grab, uint32_t& dead, int* first,
+                                                 int* end) {
+  const int lane = lane_id();
+  const int p = ((int)blockIdx.x + lane) % kClaimParts;
+  const int lo = (int)((int64_t)total * p / kClaimParts);
+  const int size = (int)((int64_t)total * (p + 1) / kClaimParts) - lo;
+  int32_t* c = ctr + p * kClaimStride;
+  for (;;) {
+    bool avail = false;
+    if (lane < kClaimParts && !((dead >> lane) & 1u))
+      avail = __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < size;
+    const uint64_t m = ballot(avail);
+    dead |= (uint32_t)~m & ((1u << kClaimParts) - 1u);
+    if (m == 0) return false;
+    const int leader = first_lane(m);
+    int t = 0;
+    if (lane == leader) t = atomicAdd(c, grab);
+    t = __shfl(t, leader);
+    const int llo = __shfl(lo, leader), lsize = __shfl(size, leader);
+    if (t < lsize) {
+      *first = llo + t;
+      *end = llo + min(t + grab, lsize);
+      return true;
+    }
+    dead |= 1u << leader;  // another workgroup took the last items first
+  }
 }
 
 // Word indices of the Counters fields, for wg_add_counters.

@@ -635,7 +635,7 @@ __global__ void k_unpermute(const int32_t* __restrict__ d, const int32_t* __rest
 void relabel_free(smdp_graph* g) {
   Relabel* rl = static_cast<Relabel*>(g->relabel);
   if (!rl) return;
-  void* ptrs[] = {rl->row, rl->col, rl->w, rl->newid, rl->hasout, rl->lsplit, rl->phase};
+  void* ptrs[] = {rl->row, rl->col, rl->w, rl->newid, rl->hasout, rl->band, rl->phase};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (rl->hctl) (void)hipHostFree(rl->hctl);

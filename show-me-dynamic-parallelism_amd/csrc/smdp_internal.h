@@ -284,8 +284,12 @@ void parallel_ranges(int64_t count, F body) {
 // Renumbered copy of a resident graph for the tuned frontier solution
 // (frontier.hip builds it; delta.hip partitions its rows).  Vertices are
 // renumbered by falling in-degree; rows keep their arcs in order until a
-// bucketed SSSP partitions each row into light arcs (w < delta) first, heavy
-// arcs after, with lsplit[v] the first heavy arc of row v.
+// bucketed SSSP partitions each row into kBands weight bands of its light
+// arcs (w < delta), lightest band first, then the heavy arcs, with
+// band[v * kBands + j - 1] the first arc of band j of row v (j = kBands: the
+// first heavy arc).
+constexpr int kBands = 8;  // weight bands of a row's light arcs (bucketed SSSP)
+
 struct Relabel {
   int32_t* row = nullptr;
   int32_t* col = nullptr;
@@ -294,8 +298,8 @@ struct Relabel {
   uint32_t* hasout = nullptr;        // bit v set when row v of the copy has arcs
   std::vector<int32_t> h_newid;      // the same on the host (source mapping)
   int32_t minw = 0;                  // smallest arc weight (bucketing needs minw >= 0)
-  int32_t* lsplit = nullptr;         // bucketed SSSP: first heavy arc of each row
-  int32_t lsplit_delta = 0;          // the bucket width lsplit was built for (0: not built)
+  int32_t* band = nullptr;           // bucketed SSSP: band starts, kBands per row
+  int32_t band_delta = 0;            // the bucket width band was built for (0: not built)
   int32_t* phase = nullptr;          // bucketed SSSP: 4 phase descriptors (delta.hip)
   void* hctl = nullptr;              // bucketed SSSP: pinned host copy of one control slot
 };
