@@ -315,7 +315,7 @@ def batched_section(ctx, args, smdp, orc, np):
     ms_k = pst.ms_dominant
     roof = _roof(per_visit * pst.edges_relaxed, ms_k)
     roof.update(kernel="k_ms_run (hipGraph replay)", launches=pst.dominant_launches,
-                avg_launch_us=round(ms_k * 1e3 / max(1, pst.dominant_launches), 3))
+                avg_launch_us=round(ms_k * 1e3 / max(1, pst.dominant_launches), 3), traffic=traffic_for("k_ms_run"))
     entry = {"graph": f"rmat-{args.batch_scale} ef16 w[1,255] seed 4", "sources": 64,
              "ms_batched_median_of_10": round(ms_batch, 3), "ms_64_plain": round(t_plain, 3),
              "speedup": round(t_plain / ms_batch, 2), "rounds": st.rounds, "arc_visits": st.edges_relaxed,
@@ -356,7 +356,7 @@ def batched_section(ctx, args, smdp, orc, np):
                           "arcs": int(sum(x.m for x in graphs)), "ms_batched_median_of_10": round(t_batch, 3),
                           "edges_relaxed": st.edges_relaxed,
                           "roofline": dict(_roof(12 * st.edges_relaxed + 20 * st.vertices_expanded, t_batch),
-                                           kernel="k_mg (hipGraph replay)"),
+                                           kernel="k_mg (hipGraph replay)", traffic=traffic_for("k_mg")),
                           "ms_256_plain_resident": round(t_plain, 3),
                           "ms_256_plain_with_upload": round(t_plain_upload * 1e3, 3),
                           "speedup": round(t_plain / t_batch, 2)}
@@ -431,7 +431,8 @@ def color_section(ctx, smdp, orc, np):
     return {"workload": "color_rmat23_sym", "vertices": g.n, "arcs": g.m, "ms_device_median": round(ms, 4),
             "rounds": int(st.rounds), "colors": int(c.max()), "exact_vs_oracle": bool(np.array_equal(c, ref)),
             "scanned_arcs": int(st.edges_relaxed), "gteps": round(g.m / (ms * 1e6), 3),
-            "roofline": _roof(12 * st.edges_relaxed + 16 * st.vertices_expanded, ms),
+            "roofline": dict(_roof(12 * st.edges_relaxed + 16 * st.vertices_expanded, ms),
+                             traffic_per_launch={"k_layer_short": traffic_for("k_layer_short")}),
             "cpu": _cpu_entry(t_cpu, t_par, nt, g.m,
                               "the closed form colour(v) = 1 + max over preceding neighbours: one pass in "
                               "precedence order (orc_color_closed) / by layers (orc_color_layers_par); the round "
@@ -452,7 +453,9 @@ def bfs_section(ctx, smdp, orc, np):
     t_par = time.perf_counter() - t0
     return {"workload": "bfs_zipf_tree_4m", "vertices": t.n, "arcs": t.m, "ms_device_median": round(ms, 4),
             "rounds": int(st.rounds), "exact_vs_oracle": bool(np.array_equal(lv, ref)),
-            "gteps": round(t.m / (ms * 1e6), 3), "roofline": _roof(8 * st.edges_relaxed + 12 * t.n, ms),
+            "gteps": round(t.m / (ms * 1e6), 3),
+            "roofline": dict(_roof(8 * st.edges_relaxed + 12 * t.n, ms),
+                             traffic_per_launch={"k_drain<true>": traffic_for("k_drain<true>")}),
             "cpu": _cpu_entry(t_cpu, t_par, nt, t.m, "one BFS from the root: FIFO (bfs_rec.cpp:102-121 restated, "
                               "orc_bfs_fifo) / level-synchronous (orc_bfs_par)")}
 
@@ -474,7 +477,8 @@ def tree_section(ctx, smdp, orc, np):
     return {"workload": "tree_zipf_4m", "vertices": t.n, "arcs": t.m, "ms_device_median": round(ms, 4),
             "levels": int(st.rounds), "tree_height": int(rth), "descendants_of_root": int(rd[0]),
             "exact_vs_oracle": bool(np.array_equal(h, rh) and np.array_equal(d, rd)),
-            "gteps": round(t.m / (ms * 1e6), 3), "roofline": _roof(alg, ms),
+            "gteps": round(t.m / (ms * 1e6), 3),
+            "roofline": dict(_roof(alg, ms), traffic_per_launch={k: traffic_for(k) for k in ("k_tree_down", "k_tree_up")}),
             "cpu": {"ms": round(t_cpu * 1e3, 1), "cores": 1, "kind": "port", "gteps": round(t.m / t_cpu / 1e9, 4),
                     "sample": "one pass: BFS order from the root, then that order backwards (orc_tree)"}}
 

@@ -1020,16 +1020,21 @@ static int run_color_layers(smdp_graph* g, int use_prio, smdp_stats* st) {
   a.grid = grid;
   a.rd = b->rd;
   a.bar = reinterpret_cast<uint32_t*>(b->bar);
-  // k_layer_short: all workgroups resident (cooperative launch).  Its grid
-  // barrier costs more per workgroup than a short layer gains from more of
-  // them: on RMAT-23, 256/128/64/32/16/8 workgroups took 48.6/43.0/39.1/
-  // 42.6/56.0/83.8 ms per colouring, so one per four CUs.
+  // k_layer_short: its workgroups meet at grid barriers, so all must be
+  // resident at once.  The grid is a quarter of the CUs (its barrier costs
+  // more per workgroup than a short layer gains from more of them: on
+  // RMAT-23, 256/128/64/32/16/8 workgroups took 48.6/43.0/39.1/42.6/56.0/
+  // 83.8 ms per colouring), capped at what the device holds, and launched
+  // after the stream's previous kernels finished, so it fits on the idle
+  // device; a barrier that waits too long (another process holding CUs)
+  // makes the run return SMDP_ERR_DEVICE instead of hanging.  A plain launch
+  // rather than a cooperative one: rocprofv3 ended with a segmentation fault
+  // in its exit handlers after cooperative launches.
   int per_cu = 0;
   SMDP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_layer_short),
                                                           SMDP_WG, 0));
   if (per_cu < 1) return SMDP_ERR_DEVICE;
-  const int short_grid = std::max(8, ctx->cus / 4);
-  void* short_args[] = {&a};
+  const int short_grid = std::min(std::max(8, ctx->cus / 4), per_cu * ctx->cus);
   hipStream_t s = ctx->stream;
   prof_reset(ctx);
   int launches = 0;
@@ -1043,7 +1048,7 @@ static int run_color_layers(smdp_graph* g, int use_prio, smdp_stats* st) {
   // chunks) followed by the run of short layers after it in one launch.  The
   // layer indices live on the device (rd), so steps are enqueued in batches
   // and the host only checks rd between batches.  A step enqueued after the
-  // last one with work still costs ~20 us (three launches, one cooperative),
+  // last one with work still costs ~20 us (three launches),
   // so batches stop growing at 8: on RMAT-23 (at most 148 steps with work)
   // caps of 64, 16 and 8 enqueued 188, 156 and 148 steps and took 38.9, 38.1
   // and 38.0 ms; sizing batches from the next layer's decay per step
@@ -1060,8 +1065,7 @@ static int run_color_layers(smdp_graph* g, int use_prio, smdp_stats* st) {
       hipLaunchKernelGGL(k_layer_drain, dim3(drain_grid), dim3(SMDP_WG), 0, s, a);
       SMDP_CHECK(prof_mark(ctx, 1));
       SMDP_CHECK(prof_mark(ctx, 0));
-      SMDP_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_layer_short), dim3(short_grid),
-                                          dim3(SMDP_WG), short_args, 0, s));
+      hipLaunchKernelGGL(k_layer_short, dim3(short_grid), dim3(SMDP_WG), 0, s, a);
       SMDP_CHECK(prof_mark(ctx, 0));
       launches += 3;
     }

@@ -82,9 +82,6 @@ constexpr int kDDrainTile = SMDP_DELTA_DRAIN_TILE;  // grid-queue entries per dr
 constexpr int kRunPhases = 1024;  // phases per persistent launch (the host checks the solve between launches)
 constexpr int kTailShift = 8;     // a settle that leaves at most n >> kTailShift vertices ends in one open bucket
 constexpr int kTailRatio = 32;    // ... if at least kTailRatio times as many are settled
-constexpr int kFeedFirst = 8;     // phases enqueued before the first progress report is read
-constexpr int kFeed = 4;          // phases enqueued at a time after that ...
-constexpr int kFeedSlack = 3;     // ... whenever fewer than this many are still queued ahead of the device
 
 enum : int32_t { kSweepLight = 0, kLight = 1, kSettle = 2, kDone = 3 };
 

@@ -59,6 +59,7 @@ struct FArgs {
   // and trees most reached vertices are such.  nullptr: queue every claim.
   const uint32_t* __restrict__ hasout;
   unsigned long long* rlog;  // profiling: per round (queue length, arcs), or nullptr
+  int32_t* progress;         // tuned: host memory, r + 1 when round r starts with work, -(r + 1) when it has none
 };
 
 // Profiling log: round r's queue length and arcs (one atomic per wave).
@@ -364,6 +365,8 @@ __global__ __launch_bounds__(SMDP_WG) void k_expand_tile(FArgs a, int r) {
   Ctl* cin = a.ctl + (r & 3);
   Ctl* cout = a.ctl + ((r + 1) & 3);
   const int L = cin->qlen;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && a.progress)
+    __hip_atomic_store(a.progress, L > 0 ? r + 1 : -(r + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   if (L == 0) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&cnt_of(a.cnt)->rounds, (unsigned long long)(r + 1));
   // workgroups past the tile count would only contend for the counter
@@ -843,6 +846,7 @@ int run_frontier(smdp_graph* g, int32_t src, int sol, bool bfs, smdp_stats* st) 
   a.chunk = tuned ? p.chunk : 0;
   a.rlog = ctx->profiling ? ctx->rlog : nullptr;
   if (a.rlog) SMDP_HIP(hipMemsetAsync(a.rlog, 0, sizeof(unsigned long long) * 2 * kRoundLogCap, ctx->stream));
+  a.progress = tuned ? ctx->progress_dev : nullptr;
   const int grid = grid_size(ctx, p);
   a.small_q = grid * SMDP_WG;
   a.hasout = nullptr;
@@ -877,9 +881,8 @@ int run_frontier(smdp_graph* g, int32_t src, int sol, bool bfs, smdp_stats* st) 
                      g->dist, g->stamp, g->n, inf, src, g->q[0], g->ctl, g->cnt);
   ++launches;
   int r = 0;
-  RoundBatcher rb(p.batch_rounds, 16);
-  for (;;) {
-    for (int j = 0; j < rb.batch; ++j, ++r) {
+  auto enqueue = [&](int k) -> int {
+    for (int j = 0; j < k; ++j, ++r) {
       SMDP_CHECK(prof_mark(ctx, 0));
       if (bfs) launch_round<true>(sol, grid, s, a, r, &launches);
       else launch_round<false>(sol, grid, s, a, r, &launches);
@@ -892,12 +895,51 @@ int run_frontier(smdp_graph* g, int32_t src, int sol, bool bfs, smdp_stats* st) 
       }
     }
     SMDP_HIP(hipGetLastError());
-    SMDP_HIP(hipMemcpyAsync(ctx->pinned, &g->ctl[r & 3].qlen, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    SMDP_HIP(hipStreamSynchronize(s));
-    const int32_t q = ctx->pinned[0];
-    if (q == 0) break;
-    if (r >= p.max_rounds) return SMDP_ERR_OVERFLOW;
-    rb.advance(q, r);
+    return SMDP_OK;
+  };
+  if (tuned) {
+    // The tile kernel reports each round it starts in host memory, so the
+    // host keeps a few rounds queued ahead of the device and stops at the
+    // round that finds the queue empty (rounds enqueued after it return at
+    // once).  A blocking check per batch of rounds left the device idle for
+    // the round trip each time.
+    volatile int32_t* prog = ctx->progress;
+    *prog = 0;
+    SMDP_CHECK(enqueue(kFeedFirst));
+    int seen = 0;
+    double t_seen = now_ms();
+    for (;;) {
+      const int v = *prog;
+      if (v < 0) break;
+      if (v >= r - kFeedSlack) {
+        if (r >= p.max_rounds) {
+          SMDP_HIP(hipStreamSynchronize(s));
+          return SMDP_ERR_OVERFLOW;
+        }
+        SMDP_CHECK(enqueue(kFeed));
+        continue;
+      }
+      if (v != seen) {
+        seen = v;
+        t_seen = now_ms();
+      } else if (now_ms() - t_seen > 100.0) {
+        // no progress report for a while: a fault stops the stream
+        const hipError_t e = hipStreamQuery(s);
+        if (e != hipSuccess && e != hipErrorNotReady) return hip_fail(e, "tuned frontier rounds", __LINE__);
+        t_seen = now_ms();
+      }
+    }
+  } else {
+    RoundBatcher rb(p.batch_rounds, 16);
+    for (;;) {
+      SMDP_CHECK(enqueue(rb.batch));
+      SMDP_HIP(hipMemcpyAsync(ctx->pinned, &g->ctl[r & 3].qlen, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      SMDP_HIP(hipStreamSynchronize(s));
+      const int32_t q = ctx->pinned[0];
+      if (q == 0) break;
+      if (r >= p.max_rounds) return SMDP_ERR_OVERFLOW;
+      rb.advance(q, r);
+    }
   }
   SMDP_HIP(hipEventRecord(ctx->ev_end, s));
   SMDP_HIP(hipEventSynchronize(ctx->ev_end));

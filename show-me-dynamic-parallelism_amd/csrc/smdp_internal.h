@@ -270,6 +270,14 @@ struct RoundBatcher {
   }
 };
 
+// Progress feed of the bucketed phases (delta.hip) and the tuned rounds
+// (frontier.hip): the device reports each step it starts in mapped host
+// memory; the host enqueues kFeedFirst steps before reading it, then kFeed
+// more whenever fewer than kFeedSlack are still queued ahead of the device.
+constexpr int kFeedFirst = 8;
+constexpr int kFeed = 4;
+constexpr int kFeedSlack = 3;
+
 // Runs body(lo, hi) over [0, count) split into contiguous ranges, one per
 // host thread (at most 16).
 template <typename F>
