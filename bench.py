@@ -338,6 +338,7 @@ def batched_section(ctx, args, smdp, orc, np):
     graphs = [smdp.gen_small_graph(i, seed=4) for i in range(256)]
     srcs = np.zeros(256, dtype=np.int32)
     with smdp.Batch(ctx, graphs, srcs) as b:
+        plan = b.autotune() if args.tune else ctx.get_params(smdp.APP_MSSSP)
         b.run()
         sts = [b.run() for _ in range(10)]
         t_batch = sorted(x.ms_kernel for x in sts)[5]
@@ -357,6 +358,8 @@ def batched_section(ctx, args, smdp, orc, np):
                           "edges_relaxed": st.edges_relaxed,
                           "roofline": dict(_roof(12 * st.edges_relaxed + 20 * st.vertices_expanded, t_batch),
                                            kernel="k_mg (hipGraph replay)", traffic=traffic_for("k_mg")),
+                          "plan": {"wg_per_cu": plan.wg_per_cu, "group_share": plan.group_share,
+                                   "autotuned": bool(args.tune)},
                           "ms_256_plain_resident": round(t_plain, 3),
                           "ms_256_plain_with_upload": round(t_plain_upload * 1e3, 3),
                           "speedup": round(t_plain / t_batch, 2)}

@@ -95,6 +95,9 @@ typedef struct {
   int32_t delta;            /* SSSP tuned, bucketed layout: bucket width, a power of two (0 = 64) */
   int32_t persistent;       /* SSSP tuned, bucketed layout: 0 = one launch per pass; 1 = every phase of the solve in
                                one launch whose workgroups are all resident, with a grid barrier between passes */
+  int32_t group_share;      /* small-graph batch (read from the SMDP_APP_MSSSP parameters): a graph with more arcs
+                               than group_share percent of one workgroup's fair share of the batch is solved by a
+                               group of workgroups (0 = 100; 10..1000) */
 } smdp_params;
 
 /* Arc layouts of the tuned SpMV/PageRank solution (smdp_params.layout). */
@@ -257,6 +260,12 @@ int  smdp_batch_create(smdp_ctx* ctx, const smdp_csr* graphs, int32_t ng, const 
 int  smdp_batch_run(smdp_batch* b, smdp_stats* st);
 int  smdp_batch_read(smdp_batch* b, int32_t i, int32_t* dist);
 void smdp_batch_destroy(smdp_batch* b);
+/* Times the batch under candidate plans (workgroups per CU 1 or 2, group_share
+ * 50, 100 or 200; median of five replays after a warm-up), keeps the fastest
+ * for this batch and installs it in the context's SMDP_APP_MSSSP parameters,
+ * so later batches start from it.  No reference counterpart: the reference
+ * solves one graph per call. */
+int  smdp_batch_autotune(smdp_batch* b, smdp_params* chosen);
 
 /* ---- autotuner --------------------------------------------------------------*/
 /* Times candidate parameter sets of SMDP_SOL_TUNED for `app` on the resident

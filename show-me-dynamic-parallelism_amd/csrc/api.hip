@@ -45,6 +45,7 @@ static smdp_params default_params(int app) {
   p.batch_rounds = 4;
   p.max_rounds = 1 << 22;
   p.delta = 64;
+  p.group_share = 100;
   switch (app) {
     case SMDP_APP_SPMV:
     case SMDP_APP_PAGERANK:
@@ -319,6 +320,8 @@ int smdp_set_params(smdp_ctx* ctx, int app, const smdp_params* p) {
   if (q.batch_rounds <= 0) q.batch_rounds = d.batch_rounds;
   if (q.max_rounds <= 0) q.max_rounds = d.max_rounds;
   if (q.delta <= 0) q.delta = d.delta;
+  if (q.group_share <= 0) q.group_share = d.group_share;
+  if (q.group_share < 10 || q.group_share > 1000) return SMDP_ERR_ARG;
   if (q.persistent != 0 && q.persistent != 1) return SMDP_ERR_ARG;
   if (q.wg_per_cu > 64 || q.chunk < 64) return SMDP_ERR_ARG;
   if ((q.delta & (q.delta - 1)) != 0 || q.delta > (1 << 20)) return SMDP_ERR_ARG;  // a power of two

@@ -818,9 +818,106 @@ struct smdp_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int grid = 0;
+  std::vector<int64_t> m;  // arcs of each graph
+  int64_t etot = 0;
 };
 
 using namespace smdp;
+
+// Work plan of a batch for parameters p, and its hipGraph (reset + solve),
+// replacing the previous plan.  A workgroup's fair share is the batch's arcs
+// over the grid (p.wg_per_cu workgroups per CU, at most 2); a graph of more
+// than p.group_share percent of one share is solved by a group of ceil(m /
+// share) workgroups (at most kMgMaxGroup), as long as all grouped workgroups
+// can be resident at once; the rest are claimed one per workgroup, largest
+// first.  On the 256-graph batch (69M arcs): 20.9 ms with no groups, 6.7 ms
+// with groups above two shares, 5.4 ms above one; groups of up to 32 changed
+// nothing.
+static int batch_plan(smdp_batch* b, const smdp_params& p) {
+  smdp_ctx* ctx = b->ctx;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  for (int32_t** q : {&b->d_order, &b->d_plan, &b->d_ggraph, &b->d_gbase, &b->d_gsize, &b->d_gctl}) {
+    if (*q) (void)hipFree(*q);
+    *q = nullptr;
+  }
+  const int32_t ng = b->ng;
+  std::vector<int32_t> order(ng);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->m[x] > b->m[y]; });
+  const int grid = std::max(1, std::min<int>(ng, ctx->cus * std::max(1, std::min(2, p.wg_per_cu))));
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_mg<kMgThreads>),
+                                                   kMgThreads, 0) != hipSuccess)
+    per_cu = 0;
+  const int resident = std::min(grid, per_cu * ctx->cus);
+  const double share = (double)b->etot / grid * (p.group_share > 0 ? p.group_share : 100) / 100.0;
+  std::vector<int32_t> plan, ggraph, gbase, gsize, small;
+  for (int32_t gi : order) {
+    const int k = (int)std::min<int64_t>(kMgMaxGroup, (int64_t)std::ceil(b->m[gi] / std::max(1.0, share)));
+    if (k >= 2 && (int)plan.size() + k <= resident) {
+      ggraph.push_back(gi);
+      gbase.push_back((int32_t)plan.size());
+      gsize.push_back(k);
+      for (int j = 0; j < k; ++j) plan.push_back((int32_t)ggraph.size() - 1);
+    } else {
+      small.push_back(gi);
+    }
+  }
+  int rc = SMDP_OK;
+  auto up32 = [&](int32_t** d, const int32_t* h, int64_t cnt) {
+    if (rc != SMDP_OK) return;
+    if (hipMalloc((void**)d, sizeof(int32_t) * std::max<int64_t>(1, cnt)) != hipSuccess) rc = SMDP_ERR_NOMEM;
+    else if (h && cnt && hipMemcpy(*d, h, sizeof(int32_t) * cnt, hipMemcpyHostToDevice) != hipSuccess)
+      rc = SMDP_ERR_DEVICE;
+  };
+  b->ngroups = (int32_t)ggraph.size();
+  up32(&b->d_order, small.data(), (int64_t)small.size());
+  up32(&b->d_plan, plan.data(), (int64_t)plan.size());
+  up32(&b->d_ggraph, ggraph.data(), (int64_t)ggraph.size());
+  up32(&b->d_gbase, gbase.data(), (int64_t)gbase.size());
+  up32(&b->d_gsize, gsize.data(), (int64_t)gsize.size());
+  up32(&b->d_gctl, nullptr, (int64_t)ggraph.size() * kMgGroupLine);
+  if (rc != SMDP_OK) return rc;
+  GArgs& a = b->args;
+  a.row = b->d_row;
+  a.col = b->d_col;
+  a.w = b->d_w;
+  a.rbase = b->d_rbase;
+  a.vbase = b->d_vbase;
+  a.nv = b->d_nv;
+  a.src = b->d_src;
+  a.order = b->d_order;
+  a.plan = b->d_plan;
+  a.ggraph = b->d_ggraph;
+  a.gbase = b->d_gbase;
+  a.gsize = b->d_gsize;
+  a.gctl = b->d_gctl;
+  a.nplan = (int32_t)plan.size();
+  a.nsmall = (int32_t)small.size();
+  a.ngroups = b->ngroups;
+  a.dist = b->d_dist;
+  a.stamp = b->d_stamp;
+  a.qa = b->d_qa;
+  a.qb = b->d_qb;
+  a.next = b->d_next;
+  a.status = b->d_status;
+  a.cnt = b->d_cnt;
+  a.ng = ng;
+  b->grid = grid;
+  // Capture reset + solve once; every run replays the graph.  No memset nodes:
+  // counters are cleared by a kernel.
+  hipStream_t s = ctx->stream;
+  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return SMDP_ERR_DEVICE;
+  hipLaunchKernelGGL(k_mg_reset, dim3(1), dim3(256), 0, s, b->d_next, b->d_cnt, b->d_gctl, b->ngroups);
+  hipLaunchKernelGGL(k_mg_status_clear, dim3(std::max(1, (ng + 255) / 256)), dim3(256), 0, s, b->d_status, ng);
+  hipLaunchKernelGGL(k_mg<kMgThreads>, dim3(b->grid), dim3(kMgThreads), 0, s, b->args);
+  if (hipStreamEndCapture(s, &b->graph) != hipSuccess) return SMDP_ERR_DEVICE;
+  if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess) return SMDP_ERR_DEVICE;
+  return SMDP_OK;
+}
 
 extern "C" {
 
@@ -886,13 +983,14 @@ int smdp_batch_create(smdp_ctx* ctx, const smdp_csr* graphs, int32_t ng, const i
     std::copy(c.w, c.w + c.m, w.begin() + ebase[i]);
     nv[i] = c.n;
   }
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return graphs[x].m > graphs[y].m; });
   smdp_batch* b = new smdp_batch();
   b->ctx = ctx;
   b->ng = ng;
   b->nv = nv;
   b->vbase = vbase;
+  b->etot = etot;
+  b->m.resize(ng);
+  for (int32_t i = 0; i < ng; ++i) b->m[i] = graphs[i].m;
   int rc = SMDP_OK;
   auto up32 = [&](int32_t** d, const int32_t* h, int64_t cnt) {
     if (rc != SMDP_OK) return;
@@ -912,40 +1010,6 @@ int smdp_batch_create(smdp_ctx* ctx, const smdp_csr* graphs, int32_t ng, const i
   up64(&b->d_vbase, vbase.data(), ng);
   up32(&b->d_nv, nv.data(), ng);
   up32(&b->d_src, srcs, ng);
-  // Work plan.  A workgroup's fair share is the batch's arcs over the grid;
-  // a graph of more than one share is solved by a group of ceil(m / share)
-  // workgroups (at most kMgMaxGroup), as long as all grouped workgroups can be
-  // resident at once; the rest are claimed one per workgroup, largest first.
-  // On the 256-graph batch (69M arcs): 20.9 ms with no groups, 6.7 ms with
-  // groups above two shares, 5.4 ms above one; halving the share (larger and
-  // more groups) gave 10.4 ms, and groups of up to 32 changed nothing.
-  const int grid = std::max(1, std::min<int>(ng, ctx->cus * std::max(1, std::min(2, params_of(ctx, SMDP_APP_MSSSP).wg_per_cu))));
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_mg<kMgThreads>),
-                                                   kMgThreads, 0) != hipSuccess)
-    per_cu = 0;
-  const int resident = std::min(grid, per_cu * ctx->cus);
-  const double share = (double)etot / grid;
-  std::vector<int32_t> plan, ggraph, gbase, gsize, small;
-  for (int32_t gi : order) {
-    const int64_t m = graphs[gi].m;
-    const int k = (int)std::min<int64_t>(kMgMaxGroup, (int64_t)std::ceil(m / std::max(1.0, share)));
-    if (k >= 2 && (int)plan.size() + k <= resident) {
-      ggraph.push_back(gi);
-      gbase.push_back((int32_t)plan.size());
-      gsize.push_back(k);
-      for (int j = 0; j < k; ++j) plan.push_back((int32_t)ggraph.size() - 1);
-    } else {
-      small.push_back(gi);
-    }
-  }
-  b->ngroups = (int32_t)ggraph.size();
-  up32(&b->d_order, small.data(), (int64_t)small.size());
-  up32(&b->d_plan, plan.data(), (int64_t)plan.size());
-  up32(&b->d_ggraph, ggraph.data(), (int64_t)ggraph.size());
-  up32(&b->d_gbase, gbase.data(), (int64_t)gbase.size());
-  up32(&b->d_gsize, gsize.data(), (int64_t)gsize.size());
-  up32(&b->d_gctl, nullptr, (int64_t)ggraph.size() * kMgGroupLine);
   up32(&b->d_dist, nullptr, vtot);
   up32(&b->d_stamp, nullptr, vtot);
   up32(&b->d_qa, nullptr, vtot);
@@ -953,49 +1017,7 @@ int smdp_batch_create(smdp_ctx* ctx, const smdp_csr* graphs, int32_t ng, const i
   up32(&b->d_next, nullptr, 1);
   up32(&b->d_status, nullptr, ng);
   if (rc == SMDP_OK && hipMalloc((void**)&b->d_cnt, kCntParts * sizeof(Counters)) != hipSuccess) rc = SMDP_ERR_NOMEM;
-  if (rc != SMDP_OK) {
-    smdp_batch_destroy(b);
-    return rc;
-  }
-  GArgs& a = b->args;
-  a.row = b->d_row;
-  a.col = b->d_col;
-  a.w = b->d_w;
-  a.rbase = b->d_rbase;
-  a.vbase = b->d_vbase;
-  a.nv = b->d_nv;
-  a.src = b->d_src;
-  a.order = b->d_order;
-  a.plan = b->d_plan;
-  a.ggraph = b->d_ggraph;
-  a.gbase = b->d_gbase;
-  a.gsize = b->d_gsize;
-  a.gctl = b->d_gctl;
-  a.nplan = (int32_t)plan.size();
-  a.nsmall = (int32_t)small.size();
-  a.ngroups = b->ngroups;
-  a.dist = b->d_dist;
-  a.stamp = b->d_stamp;
-  a.qa = b->d_qa;
-  a.qb = b->d_qb;
-  a.next = b->d_next;
-  a.status = b->d_status;
-  a.cnt = b->d_cnt;
-  a.ng = ng;
-  // one 1024-thread workgroup per CU (two when a CU can hold them)
-  b->grid = grid;
-  // Capture reset + solve once; every run replays the graph.  No memset nodes:
-  // counters are cleared by a kernel.
-  hipStream_t s = ctx->stream;
-  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = SMDP_ERR_DEVICE;
-  if (rc == SMDP_OK) {
-    hipLaunchKernelGGL(k_mg_reset, dim3(1), dim3(256), 0, s, b->d_next, b->d_cnt, b->d_gctl, b->ngroups);
-    hipLaunchKernelGGL(k_mg_status_clear, dim3(std::max(1, (ng + 255) / 256)), dim3(256), 0, s, b->d_status, ng);
-    hipLaunchKernelGGL(k_mg<kMgThreads>, dim3(b->grid), dim3(kMgThreads), 0, s, b->args);
-    if (hipStreamEndCapture(s, &b->graph) != hipSuccess) rc = SMDP_ERR_DEVICE;
-  }
-  if (rc == SMDP_OK && hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess)
-    rc = SMDP_ERR_DEVICE;
+  if (rc == SMDP_OK) rc = batch_plan(b, params_of(ctx, SMDP_APP_MSSSP));
   if (rc != SMDP_OK) {
     smdp_batch_destroy(b);
     return rc;
@@ -1037,6 +1059,38 @@ int smdp_batch_run(smdp_batch* b, smdp_stats* st) {
     st->kernel_launches = 3;
   }
   return rc;
+}
+
+int smdp_batch_autotune(smdp_batch* b, smdp_params* chosen) {
+  if (!b) return SMDP_ERR_ARG;
+  smdp_ctx* ctx = b->ctx;
+  SMDP_HIP(hipSetDevice(ctx->device));
+  const smdp_params base = params_of(ctx, SMDP_APP_MSSSP);
+  smdp_params best = base;
+  double best_ms = 1e30;
+  for (int wg : {1, 2})
+    for (int share : {50, 100, 200}) {
+      smdp_params p = base;
+      p.wg_per_cu = wg;
+      p.group_share = share;
+      SMDP_CHECK(batch_plan(b, p));
+      // one warm-up replay, then the median of five
+      double ms[6];
+      for (int i = 0; i < 6; ++i) {
+        smdp_stats st;
+        SMDP_CHECK(smdp_batch_run(b, &st));
+        ms[i] = st.ms_kernel;
+      }
+      std::sort(ms + 1, ms + 6);
+      if (ms[3] < best_ms) {
+        best_ms = ms[3];
+        best = p;
+      }
+    }
+  SMDP_CHECK(smdp_set_params(ctx, SMDP_APP_MSSSP, &best));
+  SMDP_CHECK(batch_plan(b, params_of(ctx, SMDP_APP_MSSSP)));
+  if (chosen) *chosen = params_of(ctx, SMDP_APP_MSSSP);
+  return SMDP_OK;
 }
 
 int smdp_batch_read(smdp_batch* b, int32_t i, int32_t* dist) {

@@ -26,7 +26,10 @@ namespace smdp {
 namespace {
 
 constexpr int kMpNT = 256;                // threads per workgroup
-constexpr int kMpT = 8;                   // merge items per thread
+#ifndef SMDP_MERGE_ITEMS
+#define SMDP_MERGE_ITEMS 16
+#endif
+constexpr int kMpT = SMDP_MERGE_ITEMS;    // merge items per thread
 constexpr int kMpItems = kMpNT * kMpT;    // merge items per tile
 
 struct MergePlan {

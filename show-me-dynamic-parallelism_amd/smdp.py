@@ -41,7 +41,8 @@ class Csr(C.Structure):
 class Params(C.Structure):
     _fields_ = [("heavy_threshold", C.c_int32), ("split_threshold", C.c_int32), ("chunk", C.c_int32),
                 ("wg_per_cu", C.c_int32), ("batch_rounds", C.c_int32), ("max_rounds", C.c_int32),
-                ("layout", C.c_int32), ("delta", C.c_int32), ("persistent", C.c_int32)]
+                ("layout", C.c_int32), ("delta", C.c_int32), ("persistent", C.c_int32),
+                ("group_share", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -127,6 +128,7 @@ def _declare(L):
         "smdp_batch_run": (C.c_int, [vp, P(Stats)]),
         "smdp_batch_read": (C.c_int, [vp, i32, vp]),
         "smdp_batch_destroy": (None, [vp]),
+        "smdp_batch_autotune": (C.c_int, [vp, P(Params)]),
         "smdp_autotune": (C.c_int, [vp, C.c_int, i32, cp, P(Params)]),
         "smdp_read_graph": (C.c_int, [cp, C.c_int, P(P(HGraphStruct))]),
         "smdp_invert": (C.c_int, [P(HGraphStruct), P(P(HGraphStruct))]),
@@ -496,6 +498,11 @@ class Batch:
         out = np.zeros(self.graphs[i].n, dtype=np.int32)
         _check(lib().smdp_batch_read(self.h, i, _ptr(out)), "smdp_batch_read")
         return out
+
+    def autotune(self):
+        p = Params()
+        _check(lib().smdp_batch_autotune(self.h, C.byref(p)), "smdp_batch_autotune")
+        return p
 
     def close(self):
         if self.h:

@@ -165,3 +165,40 @@ def test_batch_group_negative_cycle(ctx):
     with pytest.raises(smdp.SmdpError) as e:
         ctx.sssp_multi_graph(graphs, np.zeros(len(graphs), dtype=np.int32))
     assert e.value.code == smdp.ERR_OVERFLOW
+
+
+def test_batch_autotune_picks_a_candidate_plan_and_stays_exact(ctx):
+    """smdp_batch_autotune re-plans the batch for each candidate (workgroups
+    per CU 1 or 2, group share 50/100/200 percent), keeps the fastest and
+    installs it in the multi-source parameters; the re-captured batch is
+    exact on every replay."""
+    big = smdp.gen_small_graph(0, vlo=40000, vhi=40000, dlo=16.0, dhi=16.0, seed=21)
+    graphs = [big] + [smdp.gen_small_graph(i, vlo=64, vhi=4096, seed=22) for i in range(1, 48)]
+    srcs = np.zeros(len(graphs), dtype=np.int32)
+    refs = [orc.dijkstra(g.row, g.col, g.w, 0) for g in graphs]
+    old = ctx.get_params(smdp.APP_MSSSP)
+    try:
+        with smdp.Batch(ctx, graphs, srcs) as b:
+            chosen = b.autotune()
+            assert chosen.wg_per_cu in (1, 2) and chosen.group_share in (50, 100, 200), chosen.as_dict()
+            now = ctx.get_params(smdp.APP_MSSSP)
+            assert (now.wg_per_cu, now.group_share) == (chosen.wg_per_cu, chosen.group_share)
+            for rep in range(3):
+                b.run()
+                for i in range(len(graphs)):
+                    assert np.array_equal(b.read(i), refs[i]), (rep, i)
+    finally:
+        ctx.set_params(smdp.APP_MSSSP, **old.as_dict())
+
+
+def test_group_share_is_validated(ctx):
+    old = ctx.get_params(smdp.APP_MSSSP)
+    try:
+        for bad in (5, 1001):
+            with pytest.raises(smdp.SmdpError) as e:
+                ctx.set_params(smdp.APP_MSSSP, group_share=bad)
+            assert e.value.code == smdp.ERR_ARG
+        ctx.set_params(smdp.APP_MSSSP, group_share=0)
+        assert ctx.get_params(smdp.APP_MSSSP).group_share == 100
+    finally:
+        ctx.set_params(smdp.APP_MSSSP, **old.as_dict())
