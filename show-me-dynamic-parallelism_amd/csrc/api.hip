@@ -267,6 +267,16 @@ int smdp_create(int device, smdp_ctx** out) {
   c->cus = prop.multiProcessorCount;
   strncpy(c->name, prop.name, sizeof c->name - 1);
   strncpy(c->arch, prop.gcnArchName, sizeof c->arch - 1);
+  // some drivers report no marketing name: ask again, else name the
+  // architecture and compute units
+  if (!c->name[0] && hipDeviceGetName(c->name, (int)sizeof c->name - 1, device) != hipSuccess) c->name[0] = 0;
+  if (!c->name[0]) {
+    char arch[64];
+    strncpy(arch, prop.gcnArchName, sizeof arch - 1);
+    arch[sizeof arch - 1] = 0;
+    if (char* colon = strchr(arch, ':')) *colon = 0;
+    snprintf(c->name, sizeof c->name, "AMD %s, %d CUs", arch, prop.multiProcessorCount);
+  }
   for (int a = 0; a < SMDP_APP_COUNT; ++a) c->params[a] = default_params(a);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev_begin) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess ||
