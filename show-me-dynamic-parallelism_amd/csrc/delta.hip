@@ -939,7 +939,7 @@ int run_delta(smdp_graph* g, Relabel* rl, int32_t src, const smdp_params& p, int
       SMDP_HIP(hipGetLastError());
       return SMDP_OK;
     };
-    SMDP_CHECK(enqueue(kFeedFirst));
+    SMDP_CHECK(enqueue(std::min(kFeedFirst, p.max_rounds)));
     int seen = 0;
     double t_seen = now_ms();
     for (;;) {
@@ -947,10 +947,12 @@ int run_delta(smdp_graph* g, Relabel* rl, int32_t src, const smdp_params& p, int
       if (v < 0) break;  // the kDone phase started: nothing more to enqueue
       if (v >= enq - kFeedSlack) {
         if (enq >= p.max_rounds) {
+          // the cap is enqueued: done only if its last phase found the solve done
           SMDP_HIP(hipStreamSynchronize(s));
+          if (*prog < 0) break;
           return SMDP_ERR_OVERFLOW;
         }
-        SMDP_CHECK(enqueue(kFeed));
+        SMDP_CHECK(enqueue(std::min(kFeed, p.max_rounds - enq)));
         continue;
       }
       if (v != seen) {

@@ -905,7 +905,7 @@ int run_frontier(smdp_graph* g, int32_t src, int sol, bool bfs, smdp_stats* st) 
     // the round trip each time.
     volatile int32_t* prog = ctx->progress;
     *prog = 0;
-    SMDP_CHECK(enqueue(kFeedFirst));
+    SMDP_CHECK(enqueue(std::min(kFeedFirst, p.max_rounds)));
     int seen = 0;
     double t_seen = now_ms();
     for (;;) {
@@ -913,10 +913,12 @@ int run_frontier(smdp_graph* g, int32_t src, int sol, bool bfs, smdp_stats* st) 
       if (v < 0) break;
       if (v >= r - kFeedSlack) {
         if (r >= p.max_rounds) {
+          // the cap is enqueued: done only if its last round found no work
           SMDP_HIP(hipStreamSynchronize(s));
+          if (*prog < 0) break;
           return SMDP_ERR_OVERFLOW;
         }
-        SMDP_CHECK(enqueue(kFeed));
+        SMDP_CHECK(enqueue(std::min(kFeed, p.max_rounds - r)));
         continue;
       }
       if (v != seen) {

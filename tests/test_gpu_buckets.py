@@ -160,3 +160,43 @@ def test_buckets_then_other_results_are_not_readable(bctx):
         with pytest.raises(smdp.SmdpError):
             dg.read(smdp.RES_LEVEL)
         assert np.array_equal(dg.read(smdp.RES_DIST), orc.dijkstra(g.row, g.col, g.w, g.source))
+
+
+def _chain(n, w):
+    """Vertex i -> i + 1 for i < n - 1, every weight w: the solve's rounds and
+    phases do not depend on the order of relaxations."""
+    row = np.minimum(np.arange(n + 1), n - 1).astype(np.int32)
+    col = np.arange(1, n, dtype=np.int32)
+    return smdp.HostGraph(row, col, np.full(n - 1, w, np.int32), 0)
+
+
+@pytest.mark.parametrize("resident", [False, True])
+def test_round_cap_is_exact(bctx, resident):
+    """The host keeps rounds (phases) queued ahead of the device, but never more
+    than max_rounds: a cap of one more than the steps with work (the last one
+    finds none) succeeds, and the steps with work alone are ERR_OVERFLOW."""
+    g = _chain(20, 100)
+    ref = orc.dijkstra(g.row, g.col, g.w, 0)
+
+    def solve(cap):
+        if resident:
+            bctx.set_params(smdp.APP_SSSP, layout=smdp.LAYOUT_BUCKETS, delta=64, max_rounds=cap)
+            with smdp.Graph(bctx, g) as dg:
+                st = dg.sssp(0, smdp.SOL_TUNED)
+                return dg.read(smdp.RES_DIST), st
+        bctx.set_params(smdp.APP_SSSP, max_rounds=cap)
+        return bctx.sssp(g, 0, smdp.SOL_TUNED)
+
+    old = bctx.get_params(smdp.APP_SSSP)
+    try:
+        d, st = solve(1 << 20)
+        assert np.array_equal(d, ref)
+        steps = int(st.rounds)
+        print(f"chain of 20, resident={resident}: {steps} steps with work", flush=True)
+        d, _ = solve(steps + 1)
+        assert np.array_equal(d, ref)
+        with pytest.raises(smdp.SmdpError) as e:
+            solve(steps)
+        assert e.value.code == smdp.ERR_OVERFLOW
+    finally:
+        bctx.set_params(smdp.APP_SSSP, **old.as_dict())

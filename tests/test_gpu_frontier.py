@@ -269,12 +269,12 @@ def test_large_csr_checked_while_staged(ctx):
 
 
 def test_round_batches_follow_queue_decay(ctx):
-    """Rounds are enqueued in batches; once the queue shrinks, the next batch
-    is sized from its decay per round, so a solve enqueues few rounds past
-    the last one with work (doubling batches of 4, 8, 16 would enqueue 28
-    rounds for a 13- to 27-round solve; RMAT-18 takes about 13).  Launches: one initialisation, then
-    the tile kernel and the grid-queue drain per enqueued round.  (A drop-in
-    call runs the unordered rounds.)"""
+    """The tuned rounds report their progress to host memory and the host
+    keeps a few rounds queued ahead, so a solve enqueues few rounds past the
+    last one with work (doubling batches of 4, 8, 16 would enqueue 28 rounds
+    for a 13- to 27-round solve; RMAT-18 takes about 13).  Launches: one
+    initialisation, then the tile kernel and the grid-queue drain per
+    enqueued round.  (A drop-in call runs the unordered rounds.)"""
     g = smdp.gen_rmat(18, 16, wlo=1, whi=255, seed=3)
     d, st = ctx.sssp(g, g.source, smdp.SOL_TUNED)
     assert np.array_equal(d, orc.dijkstra(g.row, g.col, g.w, g.source))
