@@ -35,6 +35,7 @@ struct RArgs {
   float* cnext;                    // PageRank contributions for iteration k+1
   const int32_t* __restrict__ outdeg;
   RState* st;
+  int32_t* progress;               // PageRank: host memory, k + 1 after iteration k, -(k + 1) when it stopped
   Counters* cnt;
   double2* tile_part;              // per tile: (residual, dangling)
   double2* heavy_part;             // grid solution: per heavy-queue entry
@@ -247,7 +248,10 @@ __device__ void finish_pass(const RArgs& a, int k, double2 extra, double2* lds2)
     if (PR) {
       st->residual = tot.x;
       st->dang[(k + 1) & 1] = tot.y;
-      if (tot.x < a.tol_abs || k + 1 >= a.max_iter) st->done = k + 1;
+      const bool stop = tot.x < a.tol_abs || k + 1 >= a.max_iter;
+      if (stop) st->done = k + 1;
+      if (a.progress)
+        __hip_atomic_store(a.progress, stop ? -(k + 1) : k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
     st->ticket = 0;
     st->work_next = 0;

@@ -975,18 +975,19 @@ int run_pagerank(smdp_graph* g, const smdp_pr_opts* o, int sol, smdp_stats* st) 
   hipLaunchKernelGGL(k_pr_init, dim3(std::min<int64_t>((g->n + 255) / 256, 4096)), dim3(256), 0, s, g->rank[0],
                      cbuf[0], (const int32_t*)g->outdeg, g->n, init, b->st, dang0, g->cnt);
   ++launches;
-  // Iterations are enqueued in batches; between batches the host reads the
-  // stop flag and the last L1 residual.  From the second check on, the next
-  // batch is sized by the residual's measured decay per iteration, so that
-  // few launches are left to return at once after the stop rule fires.
-  static_assert(offsetof(RState, done) == offsetof(RState, residual) + sizeof(double), "RState readback layout");
+  // The last workgroup of every iteration applies the stop rule and reports
+  // the iteration to host memory; the host keeps a few iterations queued
+  // ahead of the device and stops enqueueing when one reports that it
+  // stopped (iterations enqueued after it return at once).  A blocking check
+  // per batch of iterations left the device idle for the round trip each
+  // time.
   int k = 0;
-  int batch = std::max(4, p.batch_rounds);
   int32_t done = 0;
-  double r_prev = 0.0;
-  int k_prev = 0;
-  while (k < o->max_iter) {
-    for (int j = 0; j < batch && k < o->max_iter; ++j, ++k) {
+  volatile int32_t* prog = ctx->progress;
+  *prog = 0;
+  a.progress = ctx->progress_dev;
+  auto enqueue = [&](int cnt) {
+    for (int j = 0; j < cnt && k < o->max_iter; ++j, ++k) {
       a.x = cbuf[k & 1];
       a.cnext = cbuf[(k + 1) & 1];
       a.rold = g->rank[k & 1];
@@ -994,25 +995,37 @@ int run_pagerank(smdp_graph* g, const smdp_pr_opts* o, int sol, smdp_stats* st) 
       if (use_bins) binned_pass(ctx, s, a, ba, k, &launches);
       else launch_pass<true>(sol, grid, s, a, k, &launches, ctx);
     }
-    SMDP_HIP(hipGetLastError());
-    SMDP_HIP(hipMemcpyAsync(ctx->pinned, &b->st->residual, sizeof(double) + sizeof(int32_t), hipMemcpyDeviceToHost,
-                            s));
-    SMDP_HIP(hipStreamSynchronize(s));
-    double r = 0.0;
-    std::memcpy(&r, ctx->pinned, sizeof(double));
-    done = ctx->pinned[2];
-    if (done) break;
-    int next = std::min(batch * 2, 32);
-    if (k_prev > 0 && r > a.tol_abs && r < r_prev) {
-      // residual ~ r * rho^j after j more iterations; one extra iteration
-      // is cheaper than another host round trip when the estimate is short
-      const double rho = std::pow(r / r_prev, 1.0 / (k - k_prev));
-      const double need = std::log(a.tol_abs / r) / std::log(rho);
-      if (std::isfinite(need) && need >= 0.0) next = std::max(1, std::min(32, (int)std::ceil(need) + 1));
+  };
+  enqueue(kFeedFirst);
+  SMDP_HIP(hipGetLastError());
+  int seen = 0;
+  double t_seen = now_ms();
+  for (;;) {
+    const int v = *prog;
+    if (v < 0) {
+      done = -v;
+      break;
     }
-    r_prev = r;
-    k_prev = k;
-    batch = next;
+    if (k >= o->max_iter) {
+      // every iteration is enqueued; the last one stops at the latest
+      SMDP_HIP(hipStreamSynchronize(s));
+      done = *prog < 0 ? -*prog : 0;
+      break;
+    }
+    if (v >= k - kFeedSlack) {
+      enqueue(kFeed);
+      SMDP_HIP(hipGetLastError());
+      continue;
+    }
+    if (v != seen) {
+      seen = v;
+      t_seen = now_ms();
+    } else if (now_ms() - t_seen > 100.0) {
+      // no progress report for a while: a fault stops the stream
+      const hipError_t e = hipStreamQuery(s);
+      if (e != hipSuccess && e != hipErrorNotReady) return hip_fail(e, "PageRank iterations", __LINE__);
+      t_seen = now_ms();
+    }
   }
   SMDP_HIP(hipEventRecord(ctx->ev_end, s));
   SMDP_HIP(hipEventSynchronize(ctx->ev_end));

@@ -344,10 +344,11 @@ def test_binned_layout_is_pagerank_only(ctx):
         ctx.set_params(smdp.APP_PAGERANK, layout=3)
 
 
-def test_pagerank_batches_follow_residual_decay(binned_ctx):
-    """Iterations are enqueued in batches sized from the residual's decay, so
-    a converging solve enqueues few iterations past the one where its stop
-    rule fires (doubling batches of 4, 8, 16 would enqueue 28 for an
+def test_pagerank_iterations_are_fed_ahead(binned_ctx):
+    """The last workgroup of each iteration reports it to host memory and the
+    host keeps at most 3 + 4 iterations queued ahead of the one reported, so
+    a converging solve enqueues at most 7 iterations past the one where its
+    stop rule fires (doubling batches of 4, 8, 16 would enqueue 28 for an
     18-iteration solve).  Launches: one initialisation, then a scatter and a
     reduce per enqueued iteration."""
     g = smdp.gen_rmat(16, 16, seed=3)
@@ -357,6 +358,6 @@ def test_pagerank_batches_follow_residual_decay(binned_ctx):
     enqueued = (int(st.kernel_launches) - 1) // 2
     print(f"pagerank rmat16: {it} iterations, {enqueued} enqueued", flush=True)
     assert 12 <= it < 100
-    assert it <= enqueued <= it + 4, (it, enqueued)
+    assert it <= enqueued <= it + 7, (it, enqueued)
     ref, _, _ = orc.pagerank(ig.row, ig.col, od, d=0.85, T=1.0, fixed_iters=it)
     assert np.abs(r.astype(np.float64) - ref).sum() <= 1e-6
