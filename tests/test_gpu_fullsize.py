@@ -49,6 +49,31 @@ def test_pagerank_rmat22(ctx, rmat22):
     assert st.residual < 1e-6 and abs(float(r.sum(dtype=np.float64)) - 1.0) < 1e-5
 
 
+@pytest.mark.parametrize("sol,layout", [(smdp.SOL_FLAT, smdp.LAYOUT_AUTO), (smdp.SOL_GRID, smdp.LAYOUT_AUTO),
+                                        (smdp.SOL_TUNED, smdp.LAYOUT_CSR), (smdp.SOL_TUNED, smdp.LAYOUT_XCD8),
+                                        (smdp.SOL_TUNED, smdp.LAYOUT_BINNED)])
+def test_pagerank_rmat22_solutions_and_layouts(ctx, rmat22, sol, layout):
+    """Every PageRank solution and tuned layout on the full RMAT-22 graph,
+    against the FP64 iterates of the same iteration count (OpenMP oracle,
+    whose reductions may differ from the sequential one in the last bits)."""
+    g = rmat22
+    print(f"pagerank rmat22 sol {sol} layout {layout}", flush=True)
+    ig = smdp.invert(g)
+    od = g.outdegree()
+    old = ctx.get_params(smdp.APP_PAGERANK)
+    try:
+        ctx.set_params(smdp.APP_PAGERANK, layout=layout)
+        with smdp.Graph(ctx, smdp.HostGraph(ig.row, ig.col)) as dg:
+            dg.set_outdeg(od)
+            st = dg.pagerank(smdp.pr_opts(0.85, 1.0, 1e-6, 100), sol)
+            r = dg.read(smdp.RES_RANK, dtype=np.float32)
+    finally:
+        ctx.set_params(smdp.APP_PAGERANK, **old.as_dict())
+    ref, _, _ = orc.pagerank_par(ig.row, ig.col, od, int(st.rounds), d=0.85, T=1.0)
+    assert np.abs(r.astype(np.float64) - ref).sum() <= 1e-6, sol
+    assert st.residual < 1e-6
+
+
 def test_spmv_rmat20(ctx):
     g = smdp.gen_rmat(20, 16, seed=2)
     print("spmv rmat20", flush=True)
@@ -75,6 +100,19 @@ def test_color_rmat23_symmetric(ctx):
     src = np.repeat(np.arange(g.n), np.diff(g.row))
     loop = src == g.col
     assert not np.any(c[src[~loop]] == c[g.col[~loop]])
+
+
+@pytest.mark.parametrize("sol", [smdp.SOL_GRID, smdp.SOL_BLOCK])
+def test_color_rmat23_round_solutions(ctx, sol):
+    """The reference's round rule (graph_color_kernel.cu:28-49) with the
+    grid- and block-level consolidation on the full symmetric RMAT-23 graph:
+    the same colours as the closed form."""
+    g = smdp.gen_rmat(23, 8, symmetrize=True, seed=5)
+    print(f"color rmat23 sol {sol}", flush=True)
+    prio = smdp.gen_prio(g.n, 5)
+    c, _ = ctx.color(g, prio, sol)
+    ref, _ = orc.color_closed(g.row, g.col, prio)
+    assert np.array_equal(c, ref), sol
 
 
 def test_bfs_zipf_tree_4m(ctx):
