@@ -114,8 +114,11 @@ int prof_collect(smdp_ctx* ctx, smdp_stats* st) {
 int read_counters(smdp_ctx* ctx, const Counters* dev, Counters* out) {
   static_assert(kCntParts * sizeof(Counters) <= kPinnedInts * sizeof(int32_t), "pinned buffer holds the counters");
   const Counters* parts = reinterpret_cast<const Counters*>(ctx->pinned);
-  SMDP_HIP(hipMemcpyAsync(ctx->pinned, dev, kCntParts * sizeof(Counters), hipMemcpyDeviceToHost, ctx->stream));
-  SMDP_HIP(hipStreamSynchronize(ctx->stream));
+  if (!ctx->counters_staged) {
+    SMDP_HIP(hipMemcpyAsync(ctx->pinned, dev, kCntParts * sizeof(Counters), hipMemcpyDeviceToHost, ctx->stream));
+    SMDP_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  ctx->counters_staged = false;
   Counters c;
   memset(&c, 0, sizeof c);
   for (int k = 0; k < kCntParts; ++k) {

@@ -27,11 +27,14 @@
 // When a settle pass sees few vertices left beyond its bucket (at most
 // n >> kTailShift, and at most 1/kTailRatio of those settled so far: a small
 // frontier near the source is not a tail), the rest of the solve is one
-// open-ended bucket: its light
-// phases relax whole rows with no upper limit (unordered rounds, as in the
-// reference) until no distance changes.  The tail of a skewed graph spreads
-// a few thousand vertices over many sparse buckets, each of which would cost
-// three phases.
+// open-ended bucket: its light phases relax whole rows with no upper limit
+// (unordered rounds, as in the reference) until no distance changes.  The
+// tail of a skewed graph spreads its last vertices over many sparse buckets,
+// each of which would cost a sweep, a light phase and a settle.  On RMAT-22
+// (n/256, 1/32) started the tail at bucket 5 and took 2.03 ms a solve;
+// (n/16, 1/4) 1.92 ms; (n/8, 1/2), at bucket 2, 1.84 ms with the same arcs
+// relaxed per reachable arc (1.115); a uniform graph of 4M vertices and
+// degree 16 took 3.01, 2.95 and 2.95 ms.
 //
 // A vertex is expanded only while its distance is in the current bucket, and
 // it relaxes each arc in at most one of the light and settle passes with its
@@ -80,8 +83,14 @@ constexpr int kDSmallSplit = SMDP_DELTA_SMALL_SPLIT;  // split degree when the l
 #endif
 constexpr int kDDrainTile = SMDP_DELTA_DRAIN_TILE;  // grid-queue entries per drain tile (a power of two, at most 64)
 constexpr int kRunPhases = 1024;  // phases per persistent launch (the host checks the solve between launches)
-constexpr int kTailShift = 8;     // a settle that leaves at most n >> kTailShift vertices ends in one open bucket
-constexpr int kTailRatio = 32;    // ... if at least kTailRatio times as many are settled
+#ifndef SMDP_DELTA_TAIL_SHIFT
+#define SMDP_DELTA_TAIL_SHIFT 3
+#endif
+#ifndef SMDP_DELTA_TAIL_RATIO
+#define SMDP_DELTA_TAIL_RATIO 2
+#endif
+constexpr int kTailShift = SMDP_DELTA_TAIL_SHIFT;  // a settle that leaves at most n >> kTailShift vertices ends in one open bucket
+constexpr int kTailRatio = SMDP_DELTA_TAIL_RATIO;  // ... if at least kTailRatio times as many are settled
 
 enum : int32_t { kSweepLight = 0, kLight = 1, kSettle = 2, kDone = 3 };
 
@@ -640,11 +649,25 @@ __device__ __forceinline__ void phase_begin(const DArgs& a, const DPhase& d, int
   }
 }
 
-// One phase's pass 1 (launched per phase).
+// Out-arcs of the reached vertices into the work counters (grid-stride).
+__device__ __forceinline__ void count_reached(const DArgs& a) {
+  unsigned long long acc = 0;
+  for (int64_t v = (int64_t)blockIdx.x * SMDP_WG + threadIdx.x; v < a.n; v += (int64_t)gridDim.x * SMDP_WG)
+    if (a.dist[v] != SMDP_SSSP_INF) acc += (unsigned long long)(a.row[v + 1] - a.row[v]);
+  acc = wave_sum(acc);
+  if (lane_id() == 0 && acc) atomicAdd(&cnt_of(a.cnt)->reached, acc);
+}
+
+// One phase's pass 1 (launched per phase).  The first phase past the end of
+// the solve counts the out-arcs of the reached vertices (the solve's work in
+// traversed edges), in a launch the host enqueued anyway.
 __global__ __launch_bounds__(SMDP_WG) void k_delta_step(DArgs a, int p) {
   const DPhase d = phase_of(a, p);
   phase_begin(a, d, p);
-  if (d.mode == kDone) return;
+  if (d.mode == kDone) {
+    if (p > 0 && a.ph[(p - 1) & 3].mode != kDone) count_reached(a);
+    return;
+  }
   if (d.mode == kLight && (int)blockIdx.x * SMDP_WG >= d.nin) return;  // more workgroups than light tiles
   __shared__ DShared sh;
   DTile t;
@@ -823,9 +846,13 @@ static int ensure_buckets(smdp_graph* g, Relabel* rl, int delta) {
 // Bucketed SSSP from src (an id of the renumbered copy); results stay in
 // g->dist in the copy's numbering.  *ms is the device time from the first to
 // the last launch, *launches the kernels launched.  With profiling on, fills
-// the context's per-phase log.  The copy must have no negative weight.
+// the context's per-phase log.  *reached_counted: the done phase added the
+// reached vertices' out-arcs to the counters, which are then already in the
+// context's pinned buffer (counters_staged).  The copy must have no negative
+// weight.
 int run_delta(smdp_graph* g, Relabel* rl, int32_t src, const smdp_params& p, int grid, float* ms_out,
-              int* launches_out) {
+              int* launches_out, bool* reached_counted) {
+  *reached_counted = false;
   smdp_ctx* ctx = g->ctx;
   if (rl->minw < 0 || (g->m > 0 && !g->w)) return SMDP_ERR_ARG;
   const int delta = p.delta > 0 ? p.delta : 64;
@@ -966,6 +993,11 @@ int run_delta(smdp_graph* g, Relabel* rl, int32_t src, const smdp_params& p, int
       }
     }
     ph = enq;
+    // the done phase counted the reached arcs; the counters travel to the
+    // host with the wait for the end event
+    SMDP_HIP(hipMemcpyAsync(ctx->pinned, g->cnt, kCntParts * sizeof(Counters), hipMemcpyDeviceToHost, s));
+    ctx->counters_staged = true;
+    *reached_counted = true;
   }
   SMDP_HIP(hipEventRecord(ctx->ev_end, s));
   SMDP_HIP(hipEventSynchronize(ctx->ev_end));

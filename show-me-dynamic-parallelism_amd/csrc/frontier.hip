@@ -781,17 +781,20 @@ static bool use_relabel(const smdp_params& p, const smdp_graph* g) {
 }
 
 int run_delta(smdp_graph* g, Relabel* rl, int32_t src, const smdp_params& p, int grid, float* ms_out,
-              int* launches_out);
+              int* launches_out, bool* reached_counted);
 
 // Common end of a frontier run: out-arcs of reached vertices, the result kind,
 // the per-round log (log_rounds >= 0: rounds logged by the unordered kernels)
 // and the statistics.
 static int finish_frontier(smdp_graph* g, const int32_t* row, int32_t inf, bool bfs, bool relabel, float ms,
-                           int launches, int log_rounds, bool two_phase, smdp_stats* st) {
+                           int launches, int log_rounds, bool two_phase, smdp_stats* st, bool reached_counted = false) {
   smdp_ctx* ctx = g->ctx;
-  hipLaunchKernelGGL(k_reached, dim3(std::min<int64_t>((g->n + 255) / 256, 2048)), dim3(256), 0, ctx->stream, row,
-                     g->dist, g->n, inf, g->cnt);
-  SMDP_HIP(hipGetLastError());
+  if (!reached_counted) {
+    ctx->counters_staged = false;  // k_reached adds to the counters
+    hipLaunchKernelGGL(k_reached, dim3(std::min<int64_t>((g->n + 255) / 256, 2048)), dim3(256), 0, ctx->stream, row,
+                       g->dist, g->n, inf, g->cnt);
+    SMDP_HIP(hipGetLastError());
+  }
   if (log_rounds >= 0) SMDP_CHECK(collect_round_log(ctx, log_rounds, two_phase));
   if (st) {
     SMDP_CHECK(fill_stats(g, st, ms, launches));
@@ -870,8 +873,10 @@ int run_frontier(smdp_graph* g, int32_t src, int sol, bool bfs, smdp_stats* st) 
   if (buckets) {
     float ms = 0;
     int launches = 0;
-    SMDP_CHECK(run_delta(g, static_cast<Relabel*>(g->relabel), src, p, grid, &ms, &launches));
-    return finish_frontier(g, row, inf, false, relabel, ms, launches, -1, false, st);
+    bool counted = false;
+    SMDP_CHECK(run_delta(g, static_cast<Relabel*>(g->relabel), src, p, grid, &ms, &launches, &counted));
+    if (!st) ctx->counters_staged = false;
+    return finish_frontier(g, row, inf, false, relabel, ms, launches, -1, false, st, counted);
   }
   prof_reset(ctx);
 

@@ -122,6 +122,9 @@ struct smdp_ctx {
   // progress is the host view, progress_dev the device address of the same words.
   int32_t* progress = nullptr;
   int32_t* progress_dev = nullptr;
+  // set when a run enqueued the copy of its counters into `pinned` before its
+  // end event, so the statistics need no copy and wait of their own
+  bool counters_staged = false;
   int32_t* stage[2] = {nullptr, nullptr};  // pinned staging of large uploads (hipHostMalloc, lazily)
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
   unsigned long long* rlog = nullptr;  // per-round (queue length, arcs) when profiling
@@ -270,13 +273,26 @@ struct RoundBatcher {
   }
 };
 
-// Progress feed of the bucketed phases (delta.hip) and the tuned rounds
-// (frontier.hip): the device reports each step it starts in mapped host
-// memory; the host enqueues kFeedFirst steps before reading it, then kFeed
-// more whenever fewer than kFeedSlack are still queued ahead of the device.
-constexpr int kFeedFirst = 8;
-constexpr int kFeed = 4;
-constexpr int kFeedSlack = 3;
+// Progress feed of the bucketed phases (delta.hip), the tuned rounds
+// (frontier.hip) and PageRank's iterations (rowreduce.hip): the device reports
+// each step it starts in mapped host memory; the host enqueues kFeedFirst
+// steps before reading it, then kFeed more whenever at most kFeedSlack are
+// still queued ahead of the device.  Steps enqueued after the last return at
+// once but still cost a launch each: on RMAT-22 SSSP (first, feed, slack) =
+// (8, 4, 3) left 4 empty phase pairs and took 2.083 ms a solve, (4, 2, 1)
+// 2.062 ms.
+#ifndef SMDP_FEED_FIRST
+#define SMDP_FEED_FIRST 4
+#endif
+#ifndef SMDP_FEED
+#define SMDP_FEED 2
+#endif
+#ifndef SMDP_FEED_SLACK
+#define SMDP_FEED_SLACK 1
+#endif
+constexpr int kFeedFirst = SMDP_FEED_FIRST;
+constexpr int kFeed = SMDP_FEED;
+constexpr int kFeedSlack = SMDP_FEED_SLACK;
 
 // Runs body(lo, hi) over [0, count) split into contiguous ranges, one per
 // host thread (at most 16).
