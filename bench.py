@@ -556,7 +556,7 @@ def only_section(args, smdp, orc, np):
                     dg.sssp(g.source, smdp.SOL_TUNED)
                     q, arcs, ms = ctx.round_log()
                     ctx.set_profiling(False)
-                    modes = ("sweep-light", "light", "settle", "done")
+                    modes = ("sweep-light", "light", "settle", "done", "settle-pull")
                     for i in range(len(q)):
                         if q[i] or arcs[i]:
                             # bucketed phases log mode << 60 | bucket << 32 | entries

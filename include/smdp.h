@@ -98,7 +98,19 @@ typedef struct {
   int32_t group_share;      /* small-graph batch (read from the SMDP_APP_MSSSP parameters): a graph with more arcs
                                than group_share percent of one workgroup's fair share of the batch is solved by a
                                group of workgroups (0 = 100; 10..1000) */
+  int32_t settle;           /* SSSP tuned, bucketed layout: how a bucket's settle pass reaches the arcs that leave
+                               the bucket, one of SMDP_SETTLE_* (0 = automatic) */
 } smdp_params;
+
+/* smdp_params.settle.  Push: the bucket's vertices relax their out-arcs that
+ * leave it.  Pull: every vertex beyond the bucket takes the smallest
+ * candidate over its in-arcs from vertices below the bucket's end (needs an
+ * in-arc copy of the graph, 8 B per arc, built once per resident graph).
+ * Automatic: pull when those in-arcs are fewer than the bucket's out-arcs,
+ * from degree counts gathered during the bucket's light passes. */
+#define SMDP_SETTLE_AUTO 0
+#define SMDP_SETTLE_PUSH 1
+#define SMDP_SETTLE_PULL 2
 
 /* Arc layouts of the tuned SpMV/PageRank solution (smdp_params.layout). */
 #define SMDP_LAYOUT_AUTO    0  /* PageRank: BINNED when the graph has at least 2^20 arcs, else CSR; SpMV: CSR */

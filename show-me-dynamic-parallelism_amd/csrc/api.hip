@@ -336,6 +336,8 @@ int smdp_set_params(smdp_ctx* ctx, int app, const smdp_params* p) {
   if (q.group_share <= 0) q.group_share = d.group_share;
   if (q.group_share < 10 || q.group_share > 1000) return SMDP_ERR_ARG;
   if (q.persistent != 0 && q.persistent != 1) return SMDP_ERR_ARG;
+  if (q.settle != SMDP_SETTLE_AUTO && q.settle != SMDP_SETTLE_PUSH && q.settle != SMDP_SETTLE_PULL)
+    return SMDP_ERR_ARG;
   if (q.wg_per_cu > 64 || q.chunk < 64) return SMDP_ERR_ARG;
   if ((q.delta & (q.delta - 1)) != 0 || q.delta > (1 << 20)) return SMDP_ERR_ARG;  // a power of two
   if (q.layout != SMDP_LAYOUT_AUTO && q.layout != SMDP_LAYOUT_CSR && q.layout != SMDP_LAYOUT_BINNED &&

@@ -31,6 +31,11 @@
 namespace smdp {
 
 constexpr int kMsOut = 512;
+#ifndef SMDP_MS_ILP
+#define SMDP_MS_ILP 8
+#endif
+constexpr int kMsIlp = SMDP_MS_ILP;  // target rows of 64 distances a wave loads at once
+static_assert(kMsIlp <= SMDP_WAVE, "lane t queues arc t's target");
 constexpr int kLanes = SMDP_WAVE;  // sources per group
 
 struct MArgs {
@@ -61,31 +66,37 @@ __device__ __forceinline__ void ms_arcs(const MArgs& a, int s, int e, int du, in
     const int cv = a.col[min(k, e - 1)];
     const int cw = a.w[min(k, e - 1)];
     const int cnt = min(SMDP_WAVE, e - b);
-    for (int j0 = 0; j0 < cnt; j0 += 8) {
-      int v[8], dv[8], alt[8];
+    for (int j0 = 0; j0 < cnt; j0 += kMsIlp) {
+      int v[kMsIlp], dv[kMsIlp], alt[kMsIlp];
 #pragma unroll
-      for (int t = 0; t < 8; ++t) {
+      for (int t = 0; t < kMsIlp; ++t) {
         const int j = min(j0 + t, cnt - 1);
         v[t] = __builtin_amdgcn_readlane(cv, j);
         alt[t] = du + __builtin_amdgcn_readlane(cw, j);
       }
 #pragma unroll
-      for (int t = 0; t < 8; ++t) dv[t] = a.mdist[(int64_t)v[t] * kLanes + lane];
+      for (int t = 0; t < kMsIlp; ++t) dv[t] = a.mdist[(int64_t)v[t] * kLanes + lane];
+      // A candidate below the distance just read lowers that distance in this
+      // round, through itself or a lower candidate, so its atomicMin returns
+      // nothing and the wave does not wait for it.  Arc t's target is then
+      // queued by lane t, all of the group's stamp exchanges at once.
+      int myv = 0;
+      bool any = false;
 #pragma unroll
-      for (int t = 0; t < 8; ++t) {
-        bool imp = false;
-        if (j0 + t < cnt && du < SMDP_SSSP_INF && alt[t] < dv[t]) {
-          const int old = atomicMin(&a.mdist[(int64_t)v[t] * kLanes + lane], alt[t]);
-          imp = alt[t] < old;
-        }
+      for (int t = 0; t < kMsIlp; ++t) {
+        const bool imp = j0 + t < cnt && du < SMDP_SSSP_INF && alt[t] < dv[t];
+        if (imp) __hip_atomic_fetch_min(&a.mdist[(int64_t)v[t] * kLanes + lane], alt[t], __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
         const uint64_t m = ballot(imp);
-        bool push = false;
-        if (m) {
-          wins += lane == 0 ? __popcll(m) : 0;
-          if (lane == 0 && a.stamp[v[t]] != label) push = atomicExch(&a.stamp[v[t]], label) != label;
+        wins += lane == 0 ? __popcll(m) : 0;
+        if (lane == t) {
+          myv = v[t];
+          any = m != 0;
         }
-        out.push(push, v[t], qout, qlen);
       }
+      bool push = false;
+      if (any && a.stamp[myv] != label) push = atomicExch(&a.stamp[myv], label) != label;
+      out.push(push, myv, qout, qlen);
     }
     edges += lane == 0 ? cnt : 0;
   }

@@ -82,6 +82,10 @@ constexpr int kDSmallSplit = SMDP_DELTA_SMALL_SPLIT;  // split degree when the l
 #define SMDP_DELTA_DRAIN_TILE 8
 #endif
 constexpr int kDDrainTile = SMDP_DELTA_DRAIN_TILE;  // grid-queue entries per drain tile (a power of two, at most 64)
+#ifndef SMDP_DELTA_WAVE_GROUPS
+#define SMDP_DELTA_WAVE_GROUPS 2
+#endif
+constexpr int kWaveGroups = SMDP_DELTA_WAVE_GROUPS;  // groups of 64 entries a wave reads at once (wave passes)
 constexpr int kRunPhases = 1024;  // phases per persistent launch (the host checks the solve between launches)
 #ifndef SMDP_DELTA_TAIL_SHIFT
 #define SMDP_DELTA_TAIL_SHIFT 3
@@ -93,6 +97,8 @@ constexpr int kTailShift = SMDP_DELTA_TAIL_SHIFT;  // a settle that leaves at mo
 constexpr int kTailRatio = SMDP_DELTA_TAIL_RATIO;  // ... if at least kTailRatio times as many are settled
 
 enum : int32_t { kSweepLight = 0, kLight = 1, kSettle = 2, kDone = 3 };
+// How a settle pass reaches the arcs that leave its bucket (smdp_params.settle).
+enum : int32_t { kSettleAuto = 0, kSettlePush = 1, kSettlePull = 2 };
 
 // Descriptor of one phase; phase p's lives in slot p & 3 (one 128-byte line).
 struct DPhase {
@@ -101,7 +107,12 @@ struct DPhase {
   int32_t nin;   // entries: queue length (light) or vertices (sweeps)
   int32_t tail;  // 1: the bucket has no upper limit (the last one)
   int32_t settled;  // vertices expanded by the settle passes before this phase
-  int32_t pad[27];
+  int32_t pull;     // settle: the vertices beyond the bucket read their in-arcs
+  int32_t resweep;  // a sweep of a bucket already swept once (its vertices are counted)
+  int32_t pad0;
+  int64_t sin;   // census: in-degrees of the vertices found below the bucket's end so far
+  int64_t bout;  // census: out-degrees of the vertices found in the bucket
+  int32_t pad[20];
 };
 static_assert(sizeof(DPhase) == 128, "one line per descriptor");
 constexpr int kPhaseInts = (int)(sizeof(DPhase) / sizeof(int32_t));
@@ -109,18 +120,25 @@ constexpr int kPhaseInts = (int)(sizeof(DPhase) / sizeof(int32_t));
 // Phase after `prev`: qlen = light-queue entries pushed by prev, nb = the
 // largest next-bucket code stored by prev (INT32_MAX - bucket, 0 = none),
 // rem = the vertices prev's settle pass saw beyond its bucket, sel = those it
-// expanded.
+// expanded, jin / jout = the in- and out-degrees of the vertices prev found
+// newly below its bucket's end.  A bucket's settle pulls when the in-arcs
+// of every vertex beyond the bucket (m - sin) are fewer than the out-arcs
+// of the bucket's vertices (bout), or always or never by `settle`.
 __host__ __device__ inline DPhase next_phase(const DPhase& prev, int32_t qlen, int32_t nb, int32_t rem, int32_t sel,
-                                             int32_t n) {
+                                             int64_t jin, int64_t jout, int32_t n, int64_t m, int32_t settle) {
   DPhase d = prev;
+  d.resweep = 0;
   switch (prev.mode) {
     case kSweepLight:
     case kLight:
+      d.sin = prev.sin + jin;
+      d.bout = prev.bout + jout;
       if (qlen > n) {
         // more pushes than the queue holds (a vertex can be queued twice in a
         // race): its entries are incomplete, so the bucket is swept again
         d.mode = kSweepLight;
         d.nin = n;
+        d.resweep = 1;
       } else if (qlen > 0) {
         d.mode = kLight;
         d.nin = qlen;
@@ -130,10 +148,13 @@ __host__ __device__ inline DPhase next_phase(const DPhase& prev, int32_t qlen, i
       } else {
         d.mode = kSettle;
         d.nin = n;
+        d.pull = settle == kSettlePull || (settle == kSettleAuto && m - d.sin < d.bout);
       }
       break;
     case kSettle:
       d.settled = prev.settled + sel;
+      d.pull = 0;
+      d.bout = 0;
       if (nb > 0) {
         d.mode = kSweepLight;
         d.bucket = INT32_MAX - nb;
@@ -166,6 +187,16 @@ struct DArgs {
   Ctl* ctl;
   DPhase* ph;
   Counters* cnt;
+  // pull settle: in-arcs (sources rcol, weights rw) of row rrow[v] ..
+  // rrow[v + 1], per vertex {out-degree, in-degree}, bit v set when v has
+  // in-arcs; null when settle is kSettlePush
+  const int32_t* __restrict__ rrow;
+  const int32_t* __restrict__ rcol;
+  const int32_t* __restrict__ rw;
+  const int2* __restrict__ deg2;
+  const uint32_t* __restrict__ hasin;
+  int64_t m;
+  int32_t settle;   // kSettleAuto, kSettlePush or kSettlePull
   int32_t small_q;  // a light queue shorter than this cuts entries of >= kDSmallSplit arcs into chunks
   unsigned long long* rlog;  // profiling: per phase (entries, arcs relaxed), or nullptr
   int32_t* progress;         // host memory: p + 1 when phase p starts, -(p + 1) when it is the kDone phase
@@ -177,6 +208,7 @@ struct DArgs {
 };
 
 __device__ __forceinline__ bool d_has_arcs(const DArgs& a, int v) { return (a.hasout[v >> 5] >> (v & 31)) & 1u; }
+__device__ __forceinline__ bool d_has_in(const DArgs& a, int v) { return (a.hasin[v >> 5] >> (v & 31)) & 1u; }
 
 // First arc of band j (0..kBands) of row u; band kBands is the heavy part.
 __device__ __forceinline__ int band_start(const DArgs& a, int u, int j) {
@@ -209,18 +241,29 @@ __device__ __forceinline__ DPhase phase_of(const DArgs& a, int p) {
     d.nin = a.n;
     d.tail = 0;
     d.settled = 0;
+    d.pull = 0;
+    d.resweep = 0;
+    d.sin = 0;
+    d.bout = 0;
     return d;
   }
   const DPhase prev = a.ph[(p - 1) & 3];
   const Ctl* c = a.ctl + (p & 3);
   int nb = 0, rem = 0, sel = 0;
-  if (prev.mode == kSettle)
+  int64_t jin = 0, jout = 0;
+  if (prev.mode == kSettle) {
     for (int k = 0; k < kClaimParts; ++k) {
       nb = max(nb, c->nbk[k * kClaimStride]);
       rem += c->rem[k * kClaimStride];
       sel += c->sel[k * kClaimStride];
     }
-  return next_phase(prev, c->qlen, nb, rem, sel, a.n);
+  } else if (a.settle == kSettleAuto) {
+    for (int k = 0; k < kClaimParts; ++k) {
+      jin += c->jin[k * kClaimStride];
+      jout += c->jout[k * kClaimStride];
+    }
+  }
+  return next_phase(prev, c->qlen, nb, rem, sel, jin, jout, a.n, a.m, a.settle);
 }
 
 // Candidate relaxations of one step, packed to the low lanes through the
@@ -230,8 +273,8 @@ __device__ __forceinline__ DPhase phase_of(const DArgs& a, int p) {
 // lower one: the atomicMin returns nothing (the wave does not wait for it),
 // and every candidate counts as a winner.  Light phases queue the vertex for
 // the next light phase (once per phase, by a device-coherent stamp check; a
-// rare race queues it twice, which costs one more expansion); the settle phase keeps
-// the smallest candidate distance of a vertex with arcs.
+// rare race queues it twice, which costs one more expansion); the settle
+// phase keeps the smallest candidate distance of a vertex with arcs.
 template <int K>
 __device__ __forceinline__ void relax_packed_d(const DArgs& a, const int* v, const int* alt, const bool* want,
                                                bool settle, int label, int32_t* s_pair, Sink& out, int& nbmin,
@@ -283,14 +326,17 @@ __device__ __forceinline__ void relax_packed_d(const DArgs& a, const int* v, con
 
 // The settle pass's smallest distance beyond the bucket and its count of
 // vertices seen beyond it, over the workgroup, into the kClaimParts copies of
-// slot `c` (copy blockIdx % kClaimParts).  All threads; s_red needs
-// 2 * SMDP_WAVES_PER_WG ints.
-__device__ __forceinline__ void store_next_bucket(Ctl* c, int nbmin, int nrem, int nsel, int shift, int* s_red) {
+// slot `c` (copy blockIdx % kClaimParts); clears the thread's values (a
+// persistent launch stores after each of a phase's two passes).  All
+// threads; s_red needs 3 * SMDP_WAVES_PER_WG ints.
+__device__ __forceinline__ void store_next_bucket(Ctl* c, int& nbmin, int& nrem, int& nsel, int shift, int* s_red) {
   constexpr int W = SMDP_WAVES_PER_WG;
   int m = nbmin;
 #pragma unroll
   for (int off = SMDP_WAVE / 2; off > 0; off >>= 1) m = min(m, __shfl_xor(m, off));
   const int r = wave_sum(nrem), q = wave_sum(nsel);
+  nbmin = INT32_MAX;
+  nrem = nsel = 0;
   if (lane_id() == 0) {
     s_red[threadIdx.x / SMDP_WAVE] = m;
     s_red[W + threadIdx.x / SMDP_WAVE] = r;
@@ -312,13 +358,41 @@ __device__ __forceinline__ void store_next_bucket(Ctl* c, int nbmin, int nrem, i
   __syncthreads();
 }
 
+// The census of one sweep or light pass over the workgroup (the in- and
+// out-degrees of the vertices it found newly below the bucket's end), added
+// to copy blockIdx % kClaimParts of slot `c`; clears the thread's sums.  All
+// threads; s_red needs 2 * SMDP_WAVES_PER_WG words.
+__device__ __forceinline__ void store_joins(Ctl* c, unsigned long long& jin, unsigned long long& jout,
+                                            unsigned long long* s_red) {
+  constexpr int W = SMDP_WAVES_PER_WG;
+  const unsigned long long i = wave_sum(jin), o = wave_sum(jout);
+  jin = jout = 0;
+  if (lane_id() == 0) {
+    s_red[threadIdx.x / SMDP_WAVE] = i;
+    s_red[W + threadIdx.x / SMDP_WAVE] = o;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long si = 0, so = 0;
+    for (int k = 0; k < W; ++k) {
+      si += s_red[k];
+      so += s_red[W + k];
+    }
+    const int part = (blockIdx.x % kClaimParts) * kClaimStride;
+    if (si) atomicAdd(&c->jin[part], (uint32_t)si);
+    if (so) atomicAdd(&c->jout[part], (uint32_t)so);
+  }
+  __syncthreads();
+}
+
 // Profiling log of phase p: word 0 = mode << 60 | bucket << 32 | entries
-// (written by the step kernel, d != nullptr), word 1 += arcs relaxed.
+// (written by the step kernel, d != nullptr; a pull settle logs mode 4),
+// word 1 += arcs relaxed.
 __device__ __forceinline__ void log_phase(const DArgs& a, int p, const DPhase* d, unsigned long long wave_arcs) {
   if (!a.rlog || p >= kRoundLogCap) return;
   if (blockIdx.x == 0 && threadIdx.x == 0 && d)
-    a.rlog[2 * p] = (unsigned long long)d->mode << 60 | (unsigned long long)(uint32_t)d->bucket << 32 |
-                    (uint32_t)d->nin;
+    a.rlog[2 * p] = (unsigned long long)(d->mode == kSettle && d->pull ? 4 : d->mode) << 60 |
+                    (unsigned long long)(uint32_t)d->bucket << 32 | (uint32_t)d->nin;
   if (lane_id() == 0 && wave_arcs) atomicAdd(&a.rlog[2 * p + 1], wave_arcs);
 }
 
@@ -345,21 +419,17 @@ __global__ void k_delta_bar_reset(uint32_t* bar) {
   for (int i = threadIdx.x; i < kBarLines * 32; i += blockDim.x) bar[i] = 0;
 }
 
-constexpr int kSweepPer = 16;                    // vertices per thread per sweep batch
-constexpr int kSweepGroup = 2;                   // of which one compaction takes this many
-constexpr int kSweepList = kSweepGroup * SMDP_WG;
-
 // LDS of the phase kernels.
 struct DShared {
   int tile, tend, tot, ok;
-  int32_t start[SMDP_WG];  // first arc of each entry of the tile
-  int32_t du[SMDP_WG];     // its distance
-  int32_t off[SMDP_WG];    // exclusive scan of the arcs expanded in the tile
+  int32_t vid[kDDrainTile];    // drain: vertex of each grid-queue entry of the tile
+  int32_t start[kDDrainTile];  // its first arc
+  int32_t du[kDDrainTile];     // its distance
+  int32_t off[kDDrainTile];    // exclusive scan of the entries' arcs
   int32_t scan[3 * SMDP_WAVES_PER_WG];
-  int32_t list[kSweepList];  // sweep: the selected vertices of a group
-  int32_t ldu[kSweepList];   // and their distances
   int32_t out[SMDP_WAVES_PER_WG * kOutCap];
   int32_t pair[SMDP_WAVES_PER_WG * 2 * kDIlp * SMDP_WAVE];
+  int32_t wl[SMDP_WAVES_PER_WG * 3 * kWaveGroups * SMDP_WAVE];  // wave passes: per wave, counts, bases, distances
   int32_t flush[SMDP_WAVES_PER_WG + 1];
   unsigned long long red[SMDP_WAVES_PER_WG * 8];
 };
@@ -367,14 +437,19 @@ struct DShared {
 // Per-workgroup state of one phase's expansion.
 struct DTile {
   int32_t* s_pair;   // this wave's 2 * kDIlp * SMDP_WAVE ints
+  int32_t* wl;       // this wave's 3 * kWaveGroups * SMDP_WAVE ints
   Ctl* cin;          // the phase's input slot (grid queue, claims)
   Ctl* cout;         // the phase's output slot (light queue, next bucket)
   int split;
   int lim;
   int label;
   bool settle;
-  bool tail;  // the open last bucket: light phases scan whole rows
+  bool pull;     // settle: the vertices beyond the bucket read their in-arcs
+  bool tail;     // the open last bucket: light phases scan whole rows
+  bool census;   // sweep or light pass of automatic settles: count the vertices found newly below lim
+  bool members;  // ... and, in a bucket's first sweep, the vertices found in the bucket
   unsigned long long edges, skipped, pg, verts;
+  unsigned long long jin, jout;  // census sums of this thread
   int nbmin, nrem, nsel;
 };
 
@@ -382,6 +457,7 @@ struct DTile {
 // phase's output queue.
 __device__ __forceinline__ void tile_init(const DArgs& a, const DPhase& d, int p, DShared& sh, DTile& t) {
   t.s_pair = sh.pair + (threadIdx.x / SMDP_WAVE) * 2 * kDIlp * SMDP_WAVE;
+  t.wl = sh.wl + (threadIdx.x / SMDP_WAVE) * 3 * kWaveGroups * SMDP_WAVE;
   t.cin = a.ctl + (p & 3);
   t.cout = a.ctl + ((p + 1) & 3);
   // a short light queue has fewer tiles than workgroups: its entries of a
@@ -391,6 +467,10 @@ __device__ __forceinline__ void tile_init(const DArgs& a, const DPhase& d, int p
   t.lim = t.tail ? INT32_MAX : (d.bucket + 1) << a.shift;  // first distance beyond the bucket
   t.label = p + 1;
   t.settle = d.mode == kSettle;
+  t.pull = t.settle && d.pull;
+  t.census = a.settle == kSettleAuto && !t.settle && !t.tail;
+  t.members = t.census && d.mode == kSweepLight && !d.resweep;
+  t.jin = t.jout = 0;
   t.nbmin = INT32_MAX;
   t.nrem = 0;
   t.nsel = 0;
@@ -399,9 +479,18 @@ __device__ __forceinline__ void tile_init(const DArgs& a, const DPhase& d, int p
 // Relaxes the arcs [s_start + j - s_off] of the NE staged entries,
 // edge-balanced: kDIlp arcs per thread per step, all loads of a step issued
 // before its atomics.  Every thread of the workgroup calls it.
-template <int NE>
-__device__ __forceinline__ void relax_staged(const DArgs& a, DTile& t, const int32_t* s_start, const int32_t* s_du,
-                                             const int32_t* s_off, int j0, int tot, Sink& out) {
+//
+// PULL (a pull settle): the entries are vertices beyond the bucket and the
+// arcs their in-arcs.  An in-arc from a source already below the bucket's
+// end (settled) gives the entry the candidate dist[source] + w; no such
+// candidate is below lim (those were relaxed by the light passes), so the
+// sources read are never written in this pass and the result does not depend
+// on the order of the candidates.  A vertex's in-arcs may be spread over
+// several threads: each candidate below the distance the sweep read goes
+// through the same non-returning atomicMin as a push.
+template <int NE, bool PULL>
+__device__ __forceinline__ void relax_staged(const DArgs& a, DTile& t, const int32_t* s_vid, const int32_t* s_start,
+                                             const int32_t* s_du, const int32_t* s_off, int j0, int tot, Sink& out) {
   for (int jb = j0; jb < tot; jb += kDIlp * SMDP_WG) {
     int j[kDIlp], own[kDIlp], k[kDIlp], v[kDIlp], alt[kDIlp], seen[kDIlp];
     bool rel[kDIlp], want[kDIlp];
@@ -410,54 +499,61 @@ __device__ __forceinline__ void relax_staged(const DArgs& a, DTile& t, const int
     tile_owners<kDIlp, NE>(s_off, j, own);
 #pragma unroll
     for (int q = 0; q < kDIlp; ++q) k[q] = s_start[own[q]] + (j[q] - s_off[own[q]]);
+    if constexpr (PULL) {
+      int wk[kDIlp];
 #pragma unroll
-    for (int q = 0; q < kDIlp; ++q) v[q] = stream_load(a.col + k[q]);
+      for (int q = 0; q < kDIlp; ++q) v[q] = stream_load(a.rcol + k[q]);  // the arc's source
 #pragma unroll
-    for (int q = 0; q < kDIlp; ++q) alt[q] = s_du[own[q]] + stream_load(a.w + k[q]);
+      for (int q = 0; q < kDIlp; ++q) wk[q] = stream_load(a.rw + k[q]);
 #pragma unroll
-    for (int q = 0; q < kDIlp; ++q) {
-      const bool ok = jb + (int)threadIdx.x + q * SMDP_WG < tot;
-      rel[q] = ok && (t.settle ? alt[q] >= t.lim : alt[q] < t.lim);
-      t.skipped += ok && !rel[q];
-      t.edges += rel[q];
+      for (int q = 0; q < kDIlp; ++q) seen[q] = a.dist[v[q]];
+#pragma unroll
+      for (int q = 0; q < kDIlp; ++q) {
+        const bool ok = jb + (int)threadIdx.x + q * SMDP_WG < tot;
+        rel[q] = ok && seen[q] < t.lim;
+        t.skipped += ok && !rel[q];
+        t.edges += rel[q];
+        alt[q] = rel[q] ? seen[q] + wk[q] : INT32_MAX;
+        want[q] = rel[q] && alt[q] < s_du[own[q]];
+        v[q] = s_vid[own[q]];  // the candidate is for the entry itself
+      }
+      relax_packed_d<kDIlp>(a, v, alt, want, true, t.label, t.s_pair, out, t.nbmin, t.nrem);
+    } else {
+#pragma unroll
+      for (int q = 0; q < kDIlp; ++q) v[q] = stream_load(a.col + k[q]);
+#pragma unroll
+      for (int q = 0; q < kDIlp; ++q) alt[q] = s_du[own[q]] + stream_load(a.w + k[q]);
+#pragma unroll
+      for (int q = 0; q < kDIlp; ++q) {
+        const bool ok = jb + (int)threadIdx.x + q * SMDP_WG < tot;
+        rel[q] = ok && (t.settle ? alt[q] >= t.lim : alt[q] < t.lim);
+        t.skipped += ok && !rel[q];
+        t.edges += rel[q];
+      }
+      // only arcs this pass relaxes gather their distance
+#pragma unroll
+      for (int q = 0; q < kDIlp; ++q) seen[q] = rel[q] ? a.dist[v[q]] : 0;
+#pragma unroll
+      for (int q = 0; q < kDIlp; ++q) want[q] = rel[q] && alt[q] < seen[q];
+      if (t.census) {
+        // a candidate to a vertex read at or beyond lim brings that vertex
+        // below lim: its degrees join the census (the other lanes read the
+        // line of vertex 0, so the loads stay unconditional and in flight)
+        int2 dg[kDIlp];
+#pragma unroll
+        for (int q = 0; q < kDIlp; ++q) dg[q] = a.deg2[want[q] && seen[q] >= t.lim ? v[q] : 0];
+        relax_packed_d<kDIlp>(a, v, alt, want, false, t.label, t.s_pair, out, t.nbmin, t.nrem);
+#pragma unroll
+        for (int q = 0; q < kDIlp; ++q)
+          if (want[q] && seen[q] >= t.lim) {
+            t.jout += (unsigned)dg[q].x;
+            t.jin += (unsigned)dg[q].y;
+          }
+      } else {
+        relax_packed_d<kDIlp>(a, v, alt, want, t.settle, t.label, t.s_pair, out, t.nbmin, t.nrem);
+      }
     }
-    // only arcs this pass relaxes gather their distance
-#pragma unroll
-    for (int q = 0; q < kDIlp; ++q) seen[q] = rel[q] ? a.dist[v[q]] : 0;
-#pragma unroll
-    for (int q = 0; q < kDIlp; ++q) want[q] = rel[q] && alt[q] < seen[q];
-    relax_packed_d<kDIlp>(a, v, alt, want, t.settle, t.label, t.s_pair, out, t.nbmin, t.nrem);
   }
-}
-
-// Expands part `part` of `parts` of one tile: this thread's entry is (u, s,
-// deg, du) with deg = 0 for none.  Entries of >= split arcs go to the grid
-// queue as chunks (by part 0); the others are relaxed edge-balanced, each
-// part taking an equal share of the tile's arcs.  Every thread of the
-// workgroup calls it.
-__device__ __forceinline__ void expand_tile(const DArgs& a, DShared& sh, DTile& t, int u, int s, int deg, int du,
-                                            Sink& out, int part = 0, int parts = 1) {
-  const int lane = lane_id();
-  const int nch = (deg >= t.split && part == 0) ? (deg + a.chunk - 1) / a.chunk : 0;
-  const int incl = wave_inclusive_scan(nch);
-  const int wtot = __shfl(incl, SMDP_WAVE - 1);
-  if (wtot > 0) {
-    int b0 = 0;
-    if (lane == 0) b0 = atomicAdd(&t.cin->hlen, wtot);
-    b0 = __shfl(b0, 0);
-    const int off = b0 + incl - nch;
-    for (int c = 0; c < nch; ++c) a.hq[off + c] = Chunk{u, s + c * a.chunk};
-    t.pg += nch;
-  }
-  sh.start[threadIdx.x] = s;
-  sh.du[threadIdx.x] = du;
-  int tot;
-  const int ex = wg_exclusive_scan(deg >= t.split ? 0 : deg, sh.scan, &tot);
-  sh.off[threadIdx.x] = ex;
-  __syncthreads();
-  relax_staged<SMDP_WG>(a, t, sh.start, sh.du, sh.off, (int)((int64_t)tot * part / parts),
-                        (int)((int64_t)tot * (part + 1) / parts), out);
-  __syncthreads();
 }
 
 // A vertex at distance dv is expanded by a pass over `bucket`: it lies in the
@@ -467,111 +563,309 @@ __device__ __forceinline__ bool selected(int dv, int bucket, bool tail, int shif
   return b == bucket || (tail && b > bucket);
 }
 
-// Expands the `count` vertices of sh.list / sh.ldu in tiles of SMDP_WG: whole
-// rows in the open last bucket, else the settle pass's or the light pass's
-// bands.  All threads.
-__device__ __forceinline__ void expand_list(const DArgs& a, DShared& sh, DTile& t, int count, bool settle,
-                                            Sink& out) {
-  for (int e0 = 0; e0 < count; e0 += SMDP_WG) {
-    const int i = e0 + (int)threadIdx.x;
-    int u = 0, s = 0, deg = 0, du = 0;
-    if (i < count) {
-      u = sh.list[i];
-      du = sh.ldu[i];
-      if (t.tail) {
-        s = a.row[u];
-        deg = a.row[u + 1] - s;
-      } else if (settle) {
-        s = settle_start(a, u, du, t.lim);
-        deg = a.row[u + 1] - s;
-      } else {
-        s = a.row[u];
-        deg = light_end(a, u, du, t.lim) - s;
+// Pull settle pass (each wave on its own; no workgroup barrier).  A wave
+// reads kWaveGroups groups of 64 consecutive vertices at once (groups dealt
+// grid-stride over the waves): a vertex of the bucket with out-arcs counts as
+// settled, one beyond it as seen, and one with in-arcs at or beyond the
+// bucket's end (unreached ones included) is pulled.  The pulled vertices'
+// in-arcs are laid end to end (a wave scan of their counts, kept in the
+// wave's LDS area with each vertex's first in-arc and distance) and read 64 *
+// kDIlp at a time, each lane finding its arc's vertex by a binary search of
+// the counts; a segmented minimum over the lanes of each vertex leaves one
+// candidate per vertex per step, which goes through a non-returning
+// atomicMin when it is below the distance the vertex had (a vertex's arcs can
+// span steps).  A vertex with split or more in-arcs goes to the grid queue
+// as chunks instead.
+__device__ __forceinline__ void pull_pass(const DArgs& a, const DPhase& d, DTile& t, Sink& out) {
+  constexpr int G = kWaveGroups;
+  constexpr int GW = G * SMDP_WAVE;
+  static_assert((GW & (GW - 1)) == 0, "binary search over a power of two");
+  const int lane = lane_id();
+  const int gw = (int)blockIdx.x * SMDP_WAVES_PER_WG + (int)threadIdx.x / SMDP_WAVE;
+  const int nw = (int)gridDim.x * SMDP_WAVES_PER_WG;
+  const int ngroups = (a.n + SMDP_WAVE - 1) / SMDP_WAVE;
+  int32_t* s_pre = t.wl;         // inclusive in-arc counts of the wave's G * 64 vertices
+  int32_t* s_base = s_pre + GW;  // first in-arc minus the vertex's first position in the wave's arc list
+  int32_t* s_dv = s_base + GW;   // distance read
+  for (int g0 = gw; g0 < ngroups; g0 += G * nw) {
+    int v[G], dv[G], st[G], deg[G];
+    bool hi[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int grp = g0 + g * nw;
+      v[g] = grp < ngroups ? grp * SMDP_WAVE + lane : a.n;
+      dv[g] = -1;  // not read
+      hi[g] = false;
+      if (v[g] < a.n) {
+        const bool ho = d_has_arcs(a, v[g]);
+        hi[g] = d_has_in(a, v[g]);
+        if (ho || hi[g]) dv[g] = a.dist[v[g]];
+        if (ho && dv[g] != SMDP_SSSP_INF) {
+          const int b = dv[g] >> a.shift;
+          if (b == d.bucket) {
+            ++t.nsel;
+          } else if (b > d.bucket) {
+            t.nbmin = min(t.nbmin, dv[g]);
+            ++t.nrem;
+          }
+        }
       }
-      ++t.verts;
-      ++t.nsel;
     }
-    expand_tile(a, sh, t, u, s, deg, du, out);
+    // in-rows of the pulled vertices (loads of vertex 0's for the others)
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const bool pk = hi[g] && dv[g] >= t.lim;
+      const int x = pk ? v[g] : 0;
+      st[g] = a.rrow[x];
+      deg[g] = pk ? a.rrow[x + 1] - st[g] : 0;
+      t.verts += pk;
+    }
+    // long in-rows go to the grid queue
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int nch = deg[g] >= t.split ? (deg[g] + a.chunk - 1) / a.chunk : 0;
+      const int incl = wave_inclusive_scan(nch);
+      const int wtot = __shfl(incl, SMDP_WAVE - 1);
+      if (wtot > 0) {
+        int b0 = 0;
+        if (lane == 0) b0 = atomicAdd(&t.cin->hlen, wtot);
+        b0 = __shfl(b0, 0);
+        for (int c = 0; c < nch; ++c) a.hq[b0 + incl - nch + c] = Chunk{v[g], st[g] + c * a.chunk};
+        t.pg += nch;
+        if (nch) deg[g] = 0;
+      }
+    }
+    int tot = 0;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int incl = wave_inclusive_scan(deg[g]);
+      s_pre[g * SMDP_WAVE + lane] = tot + incl;
+      s_base[g * SMDP_WAVE + lane] = st[g] - (tot + incl - deg[g]);
+      s_dv[g * SMDP_WAVE + lane] = dv[g];
+      tot += __shfl(incl, SMDP_WAVE - 1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int f0 = 0; f0 < tot; f0 += kDIlp * SMDP_WAVE) {
+      int f[kDIlp], o[kDIlp], src[kDIlp], wk[kDIlp], du[kDIlp];
+#pragma unroll
+      for (int q = 0; q < kDIlp; ++q) f[q] = min(f0 + q * SMDP_WAVE + lane, tot - 1);
+      // o = the number of vertices whose arcs all come before arc f
+#pragma unroll
+      for (int q = 0; q < kDIlp; ++q) {
+        int x = 0;
+#pragma unroll
+        for (int step = GW / 2; step > 0; step >>= 1)
+          if (s_pre[x + step - 1] <= f[q]) x += step;
+        o[q] = x;
+      }
+#pragma unroll
+      for (int q = 0; q < kDIlp; ++q) {
+        const int k = s_base[o[q]] + f[q];
+        src[q] = stream_load(a.rcol + k);
+        wk[q] = stream_load(a.rw + k);
+      }
+#pragma unroll
+      for (int q = 0; q < kDIlp; ++q) du[q] = a.dist[src[q]];
+#pragma unroll
+      for (int q = 0; q < kDIlp; ++q) {
+        const bool ok = f0 + q * SMDP_WAVE + lane < tot;
+        const bool r = ok && du[q] < t.lim;
+        t.edges += r;
+        t.skipped += ok && !r;
+        // the smallest candidate over this step's lanes of vertex o[q] (the
+        // lanes' vertices do not decrease with the lane; clamped lanes repeat
+        // the last arc's vertex with no candidate)
+        int c = r ? du[q] + wk[q] : INT32_MAX;
+#pragma unroll
+        for (int dd = 1; dd < SMDP_WAVE; dd <<= 1) {
+          const int oo = __shfl_up(o[q], dd);
+          const int cc = __shfl_up(c, dd);
+          if (lane >= dd && oo == o[q]) c = min(c, cc);
+        }
+        const int onext = __shfl_down(o[q], 1);
+        if ((lane == SMDP_WAVE - 1 || onext != o[q]) && c < s_dv[o[q]]) {
+          const int x = (g0 + (o[q] / SMDP_WAVE) * nw) * SMDP_WAVE + o[q] % SMDP_WAVE;
+          __hip_atomic_fetch_min(&a.dist[x], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ++out.tries;
+          ++out.wins;
+          if (d_has_arcs(a, x)) {
+            t.nbmin = min(t.nbmin, c);
+            ++t.nrem;
+          }
+        }
+      }
+    }
+    // the LDS area is written again by the next groups
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }
 
-// Pass 1 of phase d (all threads of the workgroup).  Its entries are the
-// vertices 0 .. n-1 (sweeps) or the light queue's entries, dealt in tiles of
-// SMDP_WG consecutive entries grid-stride (the renumbering puts the rows with
-// most arcs first, so contiguous ranges per workgroup would give the first
-// workgroups most of the work).  kSweepPer tiles are read at once, so a
-// thread has that many entries' loads in flight (most sweep tiles hold no
-// vertex of the bucket and cost one round of loads); the selected vertices
-// are compacted in LDS and expanded in tiles.
-__device__ __forceinline__ void phase_tiles(const DArgs& a, const DPhase& d, int p, DShared& sh, DTile& t,
-                                            Sink& out) {
+// Pass 1 of a sweep, light or push-settle phase, each wave on its own (no
+// workgroup barrier, so a compute unit's waves wait on their memory steps
+// independently).  Entries are the vertices 0 .. n-1 (sweeps) or the light
+// queue's entries, in groups of 64 dealt grid-stride over the waves, a wave
+// reading kWaveGroups groups at once.  The selected entries' arc ranges (the
+// pass's bands of their rows) are laid end to end by a wave scan, kept in the
+// wave's LDS area with each entry's first arc and distance, and read 64 *
+// kDIlp at a time, each lane finding its arc's entry by a binary search of
+// the counts; candidates go through relax_packed_d.  An entry with split or
+// more arcs goes to the grid queue as chunks instead.
+__device__ __forceinline__ void wave_pass(const DArgs& a, const DPhase& d, int p, DTile& t, Sink& out) {
+  constexpr int G = kWaveGroups;
+  constexpr int GW = G * SMDP_WAVE;
+  static_assert((GW & (GW - 1)) == 0, "binary search over a power of two");
+  const int lane = lane_id();
+  const int gw = (int)blockIdx.x * SMDP_WAVES_PER_WG + (int)threadIdx.x / SMDP_WAVE;
+  const int nw = (int)gridDim.x * SMDP_WAVES_PER_WG;
   const int L = d.nin;
   const int bucket = d.bucket;
-  const bool settle = d.mode == kSettle;
   const int32_t* qin = d.mode == kLight ? ((p & 1) ? a.q1 : a.q0) : nullptr;  // queued vertices have arcs
-  const int ntiles = (L + SMDP_WG - 1) / SMDP_WG;
-  for (int t0 = blockIdx.x; t0 < ntiles; t0 += kSweepPer * gridDim.x) {
-    int vv[kSweepPer], dv[kSweepPer];
-    int cnt = 0;
+  const int ngroups = (L + SMDP_WAVE - 1) / SMDP_WAVE;
+  int32_t* s_pre = t.wl;         // inclusive arc counts of the wave's G * 64 entries
+  int32_t* s_base = s_pre + GW;  // first arc minus the entry's first position in the wave's arc list
+  int32_t* s_du = s_base + GW;   // the entry's distance
+  for (int g0 = gw; g0 < ngroups; g0 += G * nw) {
+    int v[G], dv[G], st[G], deg[G];
+    bool sel[G];
 #pragma unroll
-    for (int k = 0; k < kSweepPer; ++k) {
-      const int i = (t0 + k * (int)gridDim.x) * SMDP_WG + (int)threadIdx.x;
-      const bool in = t0 + k * (int)gridDim.x < ntiles && i < L;
-      vv[k] = qin ? (in ? qin[i] : -1) : (in ? i : -1);
+    for (int g = 0; g < G; ++g) {
+      const int i = (g0 + g * nw) * SMDP_WAVE + lane;
+      const bool in = g0 + g * nw < ngroups && i < L;
+      v[g] = in ? (qin ? qin[i] : i) : -1;
     }
 #pragma unroll
-    for (int k = 0; k < kSweepPer; ++k) {
-      dv[k] = SMDP_SSSP_INF;
-      if (vv[k] >= 0 && (qin || d_has_arcs(a, vv[k]))) dv[k] = a.dist[vv[k]];
+    for (int g = 0; g < G; ++g) {
+      dv[g] = SMDP_SSSP_INF;
+      if (v[g] >= 0 && (qin || d_has_arcs(a, v[g]))) dv[g] = a.dist[v[g]];
     }
-    // selected: in the bucket (in the open last bucket, at or beyond it)
 #pragma unroll
-    for (int k = 0; k < kSweepPer; ++k) {
-      if (dv[k] == SMDP_SSSP_INF) continue;
-      if (selected(dv[k], bucket, t.tail, a.shift)) {
-        ++cnt;
-      } else if (settle && (dv[k] >> a.shift) > bucket) {
-        t.nbmin = min(t.nbmin, dv[k]);
+    for (int g = 0; g < G; ++g) {
+      sel[g] = dv[g] != SMDP_SSSP_INF && selected(dv[g], bucket, t.tail, a.shift);
+      if (t.settle && !sel[g] && dv[g] != SMDP_SSSP_INF && (dv[g] >> a.shift) > bucket) {
+        t.nbmin = min(t.nbmin, dv[g]);
         ++t.nrem;
       }
-    }
-    int total;
-    int pos = wg_exclusive_scan(cnt, sh.scan, &total);
-    if (total == 0) continue;
-    if (total <= kSweepList) {
-      // the whole batch's selection fits one list (sparse buckets)
-#pragma unroll
-      for (int k = 0; k < kSweepPer; ++k) {
-        if (dv[k] != SMDP_SSSP_INF && selected(dv[k], bucket, t.tail, a.shift)) {
-          sh.list[pos] = vv[k];
-          sh.ldu[pos] = dv[k];
-          ++pos;
-        }
+      if (t.members && dv[g] != SMDP_SSSP_INF && (dv[g] >> a.shift) == bucket) {
+        // the bucket's first sweep: its vertices with out-arcs join the census
+        // (leaving out those without makes a pull look a little dearer)
+        const int2 dg = a.deg2[v[g]];
+        t.jout += (unsigned)dg.x;
+        t.jin += (unsigned)dg.y;
       }
-      __syncthreads();
-      expand_list(a, sh, t, total, settle, out);
-      continue;
     }
-    for (int g = 0; g < kSweepPer; g += kSweepGroup) {
-      int c = 0;
+    // the pass's arc range of each selected entry (vertex 0's for the others)
 #pragma unroll
-      for (int k = 0; k < kSweepGroup; ++k)
-        c += dv[g + k] != SMDP_SSSP_INF && selected(dv[g + k], bucket, t.tail, a.shift);
-      int gtot;
-      int gpos = wg_exclusive_scan(c, sh.scan, &gtot);
-      if (gtot == 0) continue;
-#pragma unroll
-      for (int k = 0; k < kSweepGroup; ++k) {
-        if (dv[g + k] != SMDP_SSSP_INF && selected(dv[g + k], bucket, t.tail, a.shift)) {
-          sh.list[gpos] = vv[g + k];
-          sh.ldu[gpos] = dv[g + k];
-          ++gpos;
-        }
+    for (int g = 0; g < G; ++g) {
+      const int x = sel[g] ? v[g] : 0;
+      const int du = sel[g] ? dv[g] : 0;
+      int e;
+      if (t.tail) {
+        st[g] = a.row[x];
+        e = a.row[x + 1];
+      } else if (t.settle) {
+        st[g] = settle_start(a, x, du, t.lim);
+        e = a.row[x + 1];
+      } else {
+        st[g] = a.row[x];
+        e = light_end(a, x, du, t.lim);
       }
-      __syncthreads();
-      expand_list(a, sh, t, gtot, settle, out);
+      deg[g] = sel[g] ? e - st[g] : 0;
+      t.verts += sel[g];
+      t.nsel += sel[g];
     }
+    // long ranges go to the grid queue
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int nch = deg[g] >= t.split ? (deg[g] + a.chunk - 1) / a.chunk : 0;
+      const int incl = wave_inclusive_scan(nch);
+      const int wtot = __shfl(incl, SMDP_WAVE - 1);
+      if (wtot > 0) {
+        int b0 = 0;
+        if (lane == 0) b0 = atomicAdd(&t.cin->hlen, wtot);
+        b0 = __shfl(b0, 0);
+        for (int c = 0; c < nch; ++c) a.hq[b0 + incl - nch + c] = Chunk{v[g], st[g] + c * a.chunk};
+        t.pg += nch;
+        if (nch) deg[g] = 0;
+      }
+    }
+    int tot = 0;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int incl = wave_inclusive_scan(deg[g]);
+      s_pre[g * SMDP_WAVE + lane] = tot + incl;
+      s_base[g * SMDP_WAVE + lane] = st[g] - (tot + incl - deg[g]);
+      s_du[g * SMDP_WAVE + lane] = dv[g];
+      tot += __shfl(incl, SMDP_WAVE - 1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int f0 = 0; f0 < tot; f0 += kDIlp * SMDP_WAVE) {
+      int f[kDIlp], o[kDIlp], vt[kDIlp], alt[kDIlp], seen[kDIlp];
+      bool rel[kDIlp], want[kDIlp];
+#pragma unroll
+      for (int q = 0; q < kDIlp; ++q) f[q] = min(f0 + q * SMDP_WAVE + lane, tot - 1);
+      // o = the number of entries whose arcs all come before arc f
+#pragma unroll
+      for (int q = 0; q < kDIlp; ++q) {
+        int x = 0;
+#pragma unroll
+        for (int step = GW / 2; step > 0; step >>= 1)
+          if (s_pre[x + step - 1] <= f[q]) x += step;
+        o[q] = x;
+      }
+#pragma unroll
+      for (int q = 0; q < kDIlp; ++q) {
+        const int k = s_base[o[q]] + f[q];
+        vt[q] = stream_load(a.col + k);
+        alt[q] = s_du[o[q]] + stream_load(a.w + k);
+      }
+#pragma unroll
+      for (int q = 0; q < kDIlp; ++q) {
+        const bool ok = f0 + q * SMDP_WAVE + lane < tot;
+        rel[q] = ok && (t.settle ? alt[q] >= t.lim : alt[q] < t.lim);
+        t.skipped += ok && !rel[q];
+        t.edges += rel[q];
+      }
+      // only arcs this pass relaxes gather their distance
+#pragma unroll
+      for (int q = 0; q < kDIlp; ++q) seen[q] = rel[q] ? a.dist[vt[q]] : 0;
+#pragma unroll
+      for (int q = 0; q < kDIlp; ++q) want[q] = rel[q] && alt[q] < seen[q];
+      if (t.census) {
+        // as in relax_staged: a candidate to a vertex read at or beyond lim
+        // brings its degrees into the census
+        int2 dg[kDIlp];
+#pragma unroll
+        for (int q = 0; q < kDIlp; ++q) dg[q] = a.deg2[want[q] && seen[q] >= t.lim ? vt[q] : 0];
+        relax_packed_d<kDIlp>(a, vt, alt, want, false, t.label, t.s_pair, out, t.nbmin, t.nrem);
+#pragma unroll
+        for (int q = 0; q < kDIlp; ++q)
+          if (want[q] && seen[q] >= t.lim) {
+            t.jout += (unsigned)dg[q].x;
+            t.jin += (unsigned)dg[q].y;
+          }
+      } else {
+        relax_packed_d<kDIlp>(a, vt, alt, want, t.settle, t.label, t.s_pair, out, t.nbmin, t.nrem);
+      }
+    }
+    // the LDS area is written again by the next groups
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
+}
+
+// Pass 1 of phase d (all threads of the workgroup, each wave on its own).
+__device__ __forceinline__ void phase_tiles(const DArgs& a, const DPhase& d, int p, DShared& sh, DTile& t,
+                                            Sink& out) {
+  (void)sh;
+  if (t.pull) pull_pass(a, d, t, out);
+  else wave_pass(a, d, p, t, out);
 }
 
 // Pass 2 of phase d (all threads): the H chunks of the phase's grid queue,
@@ -595,13 +889,17 @@ __device__ __forceinline__ unsigned long long phase_chunks(const DArgs& a, const
     if (tile < 0) break;
     if (threadIdx.x < SMDP_WAVE) {
       const int idx = tile * ept + (int)threadIdx.x;
-      int len = 0, st = 0, du = 0;
+      int len = 0, st = 0, du = 0, v = 0;
       if ((int)threadIdx.x < ept && idx < H) {
         const Chunk c = a.hq[idx];
+        v = c.v;
         du = a.dist[c.v];
         // a light entry's distance may have fallen since it was cut: its
-        // range only grows, and the vertex is queued again for the rest
-        const int vend = t.settle || t.tail ? a.row[c.v + 1] : light_end(a, c.v, du, t.lim);
+        // range only grows, and the vertex is queued again for the rest (a
+        // pulling vertex is written only by its own chunks, so du stays at
+        // or beyond lim)
+        const int vend = t.pull ? a.rrow[c.v + 1]
+                                : (t.settle || t.tail ? a.row[c.v + 1] : light_end(a, c.v, du, t.lim));
         const int end = vend - c.first > a.chunk ? c.first + a.chunk : vend;
         len = end - c.first;
         st = c.first;
@@ -609,23 +907,27 @@ __device__ __forceinline__ unsigned long long phase_chunks(const DArgs& a, const
       const int incl = wave_inclusive_scan(len);
       if ((int)threadIdx.x < kDDrainTile) {
         sh.off[threadIdx.x] = incl - len;
+        sh.vid[threadIdx.x] = v;
         sh.start[threadIdx.x] = st;
         sh.du[threadIdx.x] = du;
       }
       if (threadIdx.x == kDDrainTile - 1) sh.tot = incl;
     }
     __syncthreads();
-    relax_staged<kDDrainTile>(a, t, sh.start, sh.du, sh.off, 0, sh.tot, out);
+    if (t.pull) relax_staged<kDDrainTile, true>(a, t, sh.vid, sh.start, sh.du, sh.off, 0, sh.tot, out);
+    else relax_staged<kDDrainTile, false>(a, t, sh.vid, sh.start, sh.du, sh.off, 0, sh.tot, out);
     __syncthreads();
   }
   return t.edges - e0;
 }
 
-// End of a pass: the workgroup's staged queue entries go out, and a settle
-// pass records its next-bucket minimum.  All threads.
+// End of a pass: the workgroup's staged queue entries go out, a settle pass
+// records its next-bucket minimum and a counting pass its census.  All
+// threads.
 __device__ __forceinline__ void end_pass(const DArgs& a, DShared& sh, DTile& t, Sink& out) {
   wg_flush(out, sh.flush);
   if (t.settle) store_next_bucket(t.cout, t.nbmin, t.nrem, t.nsel, a.shift, sh.scan);
+  else if (t.census) store_joins(t.cout, t.jin, t.jout, sh.red);
 }
 
 // The workgroup's work counters, one atomic per counter.  All threads.
@@ -799,6 +1101,84 @@ __global__ __launch_bounds__(SMDP_WG) void k_band_rows(const int32_t* __restrict
   }
 }
 
+// In-arcs of the copy: row u's arcs scattered to their targets' in-rows
+// (cur = the in-row starts, advanced by one atomic per arc), one wave per
+// row.  The order inside an in-row follows the atomics; a pull settle takes
+// the minimum over it, which does not depend on that order.
+__global__ __launch_bounds__(SMDP_WG) void k_in_arcs(const int32_t* __restrict__ row, const int32_t* __restrict__ col,
+                                                     const int32_t* __restrict__ w, int32_t n, int32_t* cur,
+                                                     int32_t* rcol, int32_t* rw) {
+  const int lane = lane_id();
+  const int64_t nw = (int64_t)gridDim.x * SMDP_WAVES_PER_WG;
+  for (int64_t u = (int64_t)blockIdx.x * SMDP_WAVES_PER_WG + threadIdx.x / SMDP_WAVE; u < n; u += nw) {
+    const int e = row[u + 1];
+    for (int k = row[u] + lane; k < e; k += SMDP_WAVE) {
+      const int pos = atomicAdd(&cur[col[k]], 1);
+      rcol[pos] = (int32_t)u;
+      rw[pos] = w[k];
+    }
+  }
+}
+
+// {out-degree, in-degree} of every vertex of the copy.
+__global__ void k_deg2(const int32_t* __restrict__ row, const int32_t* __restrict__ rrow, int32_t n, int2* deg2) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x)
+    deg2[v] = make_int2(row[v + 1] - row[v], rrow[v + 1] - rrow[v]);
+}
+
+// The copy's in-arcs, degree pairs and in-arc bits for the pull settle, built
+// once per resident graph from the in-row offsets the renumbering computed.
+static int ensure_pull(smdp_graph* g, Relabel* rl) {
+  if (rl->rrow) return SMDP_OK;
+  const int64_t n = g->n, m = std::max<int64_t>(1, g->m);
+  if ((int64_t)rl->h_rrow.size() != n + 1 || !rl->w) return SMDP_ERR_ARG;
+  hipStream_t s = g->ctx->stream;
+  int32_t *rrow = nullptr, *rcol = nullptr, *rw = nullptr, *deg2 = nullptr, *cur = nullptr;
+  auto release = [&]() {
+    for (int32_t* q : {rrow, rcol, rw, deg2, cur})
+      if (q) (void)hipFree(q);
+  };
+  if (hipMalloc((void**)&rrow, sizeof(int32_t) * (n + 1)) != hipSuccess ||
+      hipMalloc((void**)&rcol, sizeof(int32_t) * m) != hipSuccess ||
+      hipMalloc((void**)&rw, sizeof(int32_t) * m) != hipSuccess ||
+      hipMalloc((void**)&deg2, sizeof(int2) * std::max<int64_t>(1, n)) != hipSuccess ||
+      hipMalloc((void**)&cur, sizeof(int32_t) * (n + 1)) != hipSuccess) {
+    release();
+    return SMDP_ERR_NOMEM;
+  }
+  bool ok = hipMemcpyAsync(rrow, rl->h_rrow.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, s) ==
+                hipSuccess &&
+            hipMemcpyAsync(cur, rrow, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToDevice, s) == hipSuccess;
+  if (ok && g->m > 0) {
+    const int64_t blocks = std::min<int64_t>((n + SMDP_WAVES_PER_WG - 1) / SMDP_WAVES_PER_WG, 16384);
+    hipLaunchKernelGGL(k_in_arcs, dim3((unsigned)std::max<int64_t>(1, blocks)), dim3(SMDP_WG), 0, s, rl->row,
+                       rl->col, rl->w, g->n, cur, rcol, rw);
+    ok = hipGetLastError() == hipSuccess;
+  }
+  if (ok && n > 0) {
+    hipLaunchKernelGGL(k_deg2, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 4096)), dim3(256), 0, s, rl->row,
+                       rrow, g->n, reinterpret_cast<int2*>(deg2));
+    ok = hipGetLastError() == hipSuccess;
+  }
+  if (!ok || hipStreamSynchronize(s) != hipSuccess) {
+    release();
+    return SMDP_ERR_DEVICE;
+  }
+  (void)hipFree(cur);
+  cur = nullptr;
+  const int rc = build_hasout(g, rrow, &rl->hasin);
+  if (rc != SMDP_OK) {
+    release();
+    return rc;
+  }
+  rl->rrow = rrow;
+  rl->rcol = rcol;
+  rl->rw = rw;
+  rl->deg2 = deg2;
+  std::vector<int32_t>().swap(rl->h_rrow);
+  return SMDP_OK;
+}
+
 // log2 of the band width for bucket width delta (a power of two).
 static int band_shift(int delta) {
   int shift = 0;
@@ -858,6 +1238,14 @@ int run_delta(smdp_graph* g, Relabel* rl, int32_t src, const smdp_params& p, int
   const int delta = p.delta > 0 ? p.delta : 64;
   SMDP_CHECK(ensure_buckets(g, rl, delta));
   SMDP_CHECK(build_hasout(g, rl->row, &rl->hasout));
+  // the pull settle's in-arcs; an automatic choice that finds no room for
+  // them settles by push
+  int settle = p.settle;
+  if (settle != kSettlePush) {
+    const int rc = ensure_pull(g, rl);
+    if (rc == SMDP_ERR_NOMEM && settle == kSettleAuto) settle = kSettlePush;
+    else if (rc != SMDP_OK) return rc;
+  }
   int shift = 0;
   while ((1 << shift) < delta) ++shift;
 
@@ -875,6 +1263,14 @@ int run_delta(smdp_graph* g, Relabel* rl, int32_t src, const smdp_params& p, int
   a.ctl = g->ctl;
   a.ph = reinterpret_cast<DPhase*>(rl->phase);
   a.cnt = g->cnt;
+  const bool pull = settle != kSettlePush;
+  a.rrow = pull ? rl->rrow : nullptr;
+  a.rcol = pull ? rl->rcol : nullptr;
+  a.rw = pull ? rl->rw : nullptr;
+  a.deg2 = pull ? reinterpret_cast<const int2*>(rl->deg2) : nullptr;
+  a.hasin = pull ? rl->hasin : nullptr;
+  a.m = g->m;
+  a.settle = settle;
   a.rlog = ctx->profiling ? ctx->rlog : nullptr;
   a.progress = nullptr;
   a.small_q = grid * SMDP_WG;
@@ -949,7 +1345,7 @@ int run_delta(smdp_graph* g, Relabel* rl, int32_t src, const smdp_params& p, int
     // enqueueing at the phase that finds the solve done; a blocking check per
     // batch left the device idle ~35 us each time, and batches sized blindly
     // ran up to 25 phases past the end.
-      a.progress = ctx->progress_dev;
+    a.progress = ctx->progress_dev;
     volatile int32_t* prog = ctx->progress;
     *prog = 0;
     int enq = 0;

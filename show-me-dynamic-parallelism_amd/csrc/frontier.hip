@@ -638,7 +638,8 @@ __global__ void k_unpermute(const int32_t* __restrict__ d, const int32_t* __rest
 void relabel_free(smdp_graph* g) {
   Relabel* rl = static_cast<Relabel*>(g->relabel);
   if (!rl) return;
-  void* ptrs[] = {rl->row, rl->col, rl->w, rl->newid, rl->hasout, rl->band, rl->phase};
+  void* ptrs[] = {rl->row, rl->col, rl->w, rl->newid, rl->hasout, rl->band, rl->phase,
+                  rl->rrow, rl->rcol, rl->rw, rl->deg2, rl->hasin};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (rl->hctl) (void)hipHostFree(rl->hctl);
@@ -702,6 +703,10 @@ static int ensure_relabel(smdp_graph* g, Relabel** out) {
   Relabel* rl = static_cast<Relabel*>(g->relabel);
   rl->h_newid = std::move(newid);
   rl->minw = minw;
+  // in-row offsets of the copy, for the bucketed solve's pull settle
+  rl->h_rrow.assign((size_t)n + 1, 0);
+  for (int64_t i = 0; i < n; ++i)
+    rl->h_rrow[(size_t)i + 1] = rl->h_rrow[(size_t)i] + indeg[(size_t)order[(size_t)i]];
   int rc = alloc_once(&rl->row, n + 1);
   if (rc == SMDP_OK) rc = alloc_once(&rl->col, std::max<int64_t>(1, m));
   if (rc == SMDP_OK) rc = alloc_once(&rl->newid, std::max<int64_t>(1, n));

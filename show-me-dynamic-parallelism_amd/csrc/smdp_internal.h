@@ -37,6 +37,11 @@ struct Ctl {
   // bound: a vertex lowered twice counts twice), in kClaimParts copies
   int32_t rem[kClaimParts * kClaimStride];
   int32_t sel[kClaimParts * kClaimStride];  // bucketed SSSP: vertices the settle pass expanded
+  // bucketed SSSP: in- and out-degrees of the vertices a sweep or light pass
+  // found newly below the bucket's end (the census that chooses between a
+  // push and a pull settle), in kClaimParts copies
+  uint32_t jin[kClaimParts * kClaimStride];
+  uint32_t jout[kClaimParts * kClaimStride];
 };
 static_assert(sizeof(Ctl) % 128 == 0, "Ctl is made of 128-byte lines");
 constexpr int kCtlInts = (int)(sizeof(Ctl) / sizeof(int32_t));
@@ -325,6 +330,15 @@ struct Relabel {
   int32_t* band = nullptr;           // bucketed SSSP: band starts, kBands per row
   int32_t band_delta = 0;            // the bucket width band was built for (0: not built)
   int32_t* phase = nullptr;          // bucketed SSSP: 4 phase descriptors (delta.hip)
+  // bucketed SSSP, pull settle: the copy's in-arcs (rrow[v] .. rrow[v + 1]:
+  // sources in rcol, weights in rw), per vertex {out-degree, in-degree}, and
+  // bit v set when v has in-arcs; built once per resident graph
+  int32_t* rrow = nullptr;
+  int32_t* rcol = nullptr;
+  int32_t* rw = nullptr;
+  int32_t* deg2 = nullptr;
+  uint32_t* hasin = nullptr;
+  std::vector<int32_t> h_rrow;       // rrow on the host, kept from the renumbering until uploaded
   void* hctl = nullptr;              // bucketed SSSP: pinned host copy of one control slot
 };
 

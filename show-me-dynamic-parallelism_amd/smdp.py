@@ -20,6 +20,7 @@ SOL_FLAT, SOL_NAIVE, SOL_WAVE, SOL_BLOCK, SOL_GRID, SOL_HIER, SOL_TUNED = range(
 SOLUTIONS = {"flat": 0, "naive": 1, "wave": 2, "block": 3, "grid": 4, "hier": 5, "tuned": 6}
 APP_SSSP, APP_BFS, APP_SPMV, APP_PAGERANK, APP_COLOR, APP_MSSSP = range(6)
 LAYOUT_AUTO, LAYOUT_CSR, LAYOUT_BINNED, LAYOUT_HOT, LAYOUT_BUCKETS, LAYOUT_XCD8 = 0, 1, 2, 3, 4, 8
+SETTLE_AUTO, SETTLE_PUSH, SETTLE_PULL = 0, 1, 2
 RES_DIST, RES_LEVEL, RES_Y, RES_RANK, RES_COLOR, RES_MDIST, RES_HEIGHT, RES_DESC = range(8)
 FMT_DIMACS9, FMT_DIMACS10, FMT_SLNDC = range(3)
 
@@ -42,7 +43,7 @@ class Params(C.Structure):
     _fields_ = [("heavy_threshold", C.c_int32), ("split_threshold", C.c_int32), ("chunk", C.c_int32),
                 ("wg_per_cu", C.c_int32), ("batch_rounds", C.c_int32), ("max_rounds", C.c_int32),
                 ("layout", C.c_int32), ("delta", C.c_int32), ("persistent", C.c_int32),
-                ("group_share", C.c_int32)]
+                ("group_share", C.c_int32), ("settle", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
