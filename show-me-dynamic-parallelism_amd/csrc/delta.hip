@@ -23,7 +23,18 @@
 //                candidate lies beyond bucket b.  The pass also records the
 //                smallest distance beyond bucket b (distances it reads and
 //                candidates that won their atomicMin), which names the next
-//                bucket, so empty buckets cost nothing.
+//                bucket, so empty buckets cost nothing.  A settle can instead
+//                pull: every vertex beyond bucket b reads its in-arcs (from an
+//                in-arc copy of the graph) and takes the smallest candidate
+//                from a source below the bucket's end.  No such candidate is
+//                below the end (the light phases relaxed those), so the
+//                result is the push settle's.  The light passes count the in-
+//                and out-degrees of the vertices they bring below the bucket's
+//                end; the settle pulls when the in-arcs beyond the bucket are
+//                fewer than the bucket's out-arcs.  On RMAT-22 bucket 0's
+//                settle reads 4.8M in-arcs instead of relaxing 57.5M out-arcs
+//                (0.65 -> 0.13 ms); on a grid the bucket's out-arcs stay fewer
+//                and every settle pushes.
 // When a settle pass sees few vertices left beyond its bucket (at most
 // n >> kTailShift, and at most 1/kTailRatio of those settled so far: a small
 // frontier near the source is not a tail), the rest of the solve is one
@@ -54,9 +65,12 @@
 // Phases are launched back to back; each kernel computes its own phase from
 // the previous phase's descriptor and counters (values no kernel of the
 // current phase writes), so the host enqueues phases blindly and reads the
-// state back once per batch.  Vertices with many arcs are cut into chunks on
-// the grid queue and drained by a second kernel per phase, as in the
-// unordered tuned solution.
+// state back once per batch.  A phase's first pass runs per wave, with no
+// workgroup barrier (a compute unit's waves wait on their memory steps
+// independently): a wave reads 128 entries, lays their arc ranges end to end
+// and relaxes them 256 at a time.  Vertices with many arcs are cut into
+// chunks on the grid queue and drained by a second kernel per phase, as in
+// the unordered tuned solution.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -97,6 +111,20 @@ constexpr int kTailShift = SMDP_DELTA_TAIL_SHIFT;  // a settle that leaves at mo
 constexpr int kTailRatio = SMDP_DELTA_TAIL_RATIO;  // ... if at least kTailRatio times as many are settled
 
 enum : int32_t { kSweepLight = 0, kLight = 1, kSettle = 2, kDone = 3 };
+
+// Per-row record of the bucketed layout, one 64-byte line: the fields a pass
+// reads for one vertex come in one line fill.
+constexpr int kRec = 16;
+constexpr int kRecEnd = kBands + 1;    // [0] row start, [1 .. kBands] band starts (kBands: the heavy part), then row end
+constexpr int kRecIn = kBands + 2;     // in-row start (pull settles)
+constexpr int kRecInEnd = kBands + 3;  // in-row end
+constexpr int kRecDeg = kBands + 4;    // out-degree, then in-degree (the census)
+static_assert(kRecDeg + 2 <= kRec && kRecDeg % 2 == 0, "record fields");
+typedef int int2v __attribute__((ext_vector_type(2)));  // {vertex, weight} of an arc
+// A light-queue entry with this bit set is a vertex its candidate brought from
+// at or beyond the bucket's end to below it: its expansion adds its degrees
+// to the census (from the record line it loads anyway).
+constexpr int32_t kJoinFlag = INT32_MIN;
 // How a settle pass reaches the arcs that leave its bucket (smdp_params.settle).
 enum : int32_t { kSettleAuto = 0, kSettlePush = 1, kSettlePull = 2 };
 
@@ -175,9 +203,8 @@ __host__ __device__ inline DPhase next_phase(const DPhase& prev, int32_t qlen, i
 
 struct DArgs {
   const int32_t* __restrict__ row;
-  const int32_t* __restrict__ col;
-  const int32_t* __restrict__ w;
-  const int32_t* __restrict__ band;    // kBands band starts per row (light arcs by weight band, heavy last)
+  const int32_t* __restrict__ rec;     // kRec ints per row: start, band starts, end, in-row, degrees
+  const int2v* __restrict__ cw;        // arcs in band order: {target, weight}
   const uint32_t* __restrict__ hasout; // bit v set when row v has arcs
   int32_t* dist;
   int32_t* stamp;  // phase label of the last light-queue push
@@ -187,13 +214,9 @@ struct DArgs {
   Ctl* ctl;
   DPhase* ph;
   Counters* cnt;
-  // pull settle: in-arcs (sources rcol, weights rw) of row rrow[v] ..
-  // rrow[v + 1], per vertex {out-degree, in-degree}, bit v set when v has
-  // in-arcs; null when settle is kSettlePush
-  const int32_t* __restrict__ rrow;
-  const int32_t* __restrict__ rcol;
-  const int32_t* __restrict__ rw;
-  const int2* __restrict__ deg2;
+  // pull settle: in-arcs {source, weight} of the in-rows the records name,
+  // bit v set when v has in-arcs; null when settle is kSettlePush
+  const int2v* __restrict__ rcw;
   const uint32_t* __restrict__ hasin;
   int64_t m;
   int32_t settle;   // kSettleAuto, kSettlePush or kSettlePull
@@ -211,8 +234,11 @@ __device__ __forceinline__ bool d_has_arcs(const DArgs& a, int v) { return (a.ha
 __device__ __forceinline__ bool d_has_in(const DArgs& a, int v) { return (a.hasin[v >> 5] >> (v & 31)) & 1u; }
 
 // First arc of band j (0..kBands) of row u; band kBands is the heavy part.
-__device__ __forceinline__ int band_start(const DArgs& a, int u, int j) {
-  return j == 0 ? a.row[u] : a.band[(int64_t)u * kBands + j - 1];
+__device__ __forceinline__ int band_start(const DArgs& a, int u, int j) { return a.rec[(int64_t)u * kRec + j]; }
+__device__ __forceinline__ int rec_field(const DArgs& a, int u, int f) { return a.rec[(int64_t)u * kRec + f]; }
+// {out-degree, in-degree} of v
+__device__ __forceinline__ int2v rec_degrees(const DArgs& a, int v) {
+  return *reinterpret_cast<const int2v*>(a.rec + (int64_t)v * kRec + kRecDeg);
 }
 
 // A vertex at distance du in a bucket ending at lim relaxes, in a light pass,
@@ -299,19 +325,20 @@ __device__ __forceinline__ void relax_packed_d(const DArgs& a, const int* v, con
     int vv = 0;
     bool push = false;
     if (i < total) {
-      vv = s_pair[2 * i];
+      vv = s_pair[2 * i];  // may carry kJoinFlag, which goes into the queue with it
+      const int x = vv & ~kJoinFlag;
       const int aa = s_pair[2 * i + 1];
       ++out.tries;
       ++out.wins;
-      __hip_atomic_fetch_min(&a.dist[vv], aa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (d_has_arcs(a, vv)) {
+      __hip_atomic_fetch_min(&a.dist[x], aa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (d_has_arcs(a, x)) {
         if (settle) {
           nbmin = min(nbmin, aa);
           ++nrem;
-        } else if (__hip_atomic_load(&a.stamp[vv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != label) {
+        } else if (__hip_atomic_load(&a.stamp[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != label) {
           // device-coherent read and write: a plain read could see another
           // compute unit's stale cached stamp and queue the vertex again
-          __hip_atomic_store(&a.stamp[vv], label, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(&a.stamp[x], label, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           push = true;
         }
       }
@@ -502,9 +529,11 @@ __device__ __forceinline__ void relax_staged(const DArgs& a, DTile& t, const int
     if constexpr (PULL) {
       int wk[kDIlp];
 #pragma unroll
-      for (int q = 0; q < kDIlp; ++q) v[q] = stream_load(a.rcol + k[q]);  // the arc's source
-#pragma unroll
-      for (int q = 0; q < kDIlp; ++q) wk[q] = stream_load(a.rw + k[q]);
+      for (int q = 0; q < kDIlp; ++q) {
+        const int2v e = stream_load(a.rcw + k[q]);
+        v[q] = e.x;  // the arc's source
+        wk[q] = e.y;
+      }
 #pragma unroll
       for (int q = 0; q < kDIlp; ++q) seen[q] = a.dist[v[q]];
 #pragma unroll
@@ -520,9 +549,11 @@ __device__ __forceinline__ void relax_staged(const DArgs& a, DTile& t, const int
       relax_packed_d<kDIlp>(a, v, alt, want, true, t.label, t.s_pair, out, t.nbmin, t.nrem);
     } else {
 #pragma unroll
-      for (int q = 0; q < kDIlp; ++q) v[q] = stream_load(a.col + k[q]);
-#pragma unroll
-      for (int q = 0; q < kDIlp; ++q) alt[q] = s_du[own[q]] + stream_load(a.w + k[q]);
+      for (int q = 0; q < kDIlp; ++q) {
+        const int2v e = stream_load(a.cw + k[q]);
+        v[q] = e.x;
+        alt[q] = s_du[own[q]] + e.y;
+      }
 #pragma unroll
       for (int q = 0; q < kDIlp; ++q) {
         const bool ok = jb + (int)threadIdx.x + q * SMDP_WG < tot;
@@ -535,23 +566,12 @@ __device__ __forceinline__ void relax_staged(const DArgs& a, DTile& t, const int
       for (int q = 0; q < kDIlp; ++q) seen[q] = rel[q] ? a.dist[v[q]] : 0;
 #pragma unroll
       for (int q = 0; q < kDIlp; ++q) want[q] = rel[q] && alt[q] < seen[q];
-      if (t.census) {
-        // a candidate to a vertex read at or beyond lim brings that vertex
-        // below lim: its degrees join the census (the other lanes read the
-        // line of vertex 0, so the loads stay unconditional and in flight)
-        int2 dg[kDIlp];
+      // a candidate to a vertex read at or beyond lim brings that vertex
+      // below lim: its queue entry is flagged for the census
+      if (t.census)
 #pragma unroll
-        for (int q = 0; q < kDIlp; ++q) dg[q] = a.deg2[want[q] && seen[q] >= t.lim ? v[q] : 0];
-        relax_packed_d<kDIlp>(a, v, alt, want, false, t.label, t.s_pair, out, t.nbmin, t.nrem);
-#pragma unroll
-        for (int q = 0; q < kDIlp; ++q)
-          if (want[q] && seen[q] >= t.lim) {
-            t.jout += (unsigned)dg[q].x;
-            t.jin += (unsigned)dg[q].y;
-          }
-      } else {
-        relax_packed_d<kDIlp>(a, v, alt, want, t.settle, t.label, t.s_pair, out, t.nbmin, t.nrem);
-      }
+        for (int q = 0; q < kDIlp; ++q) v[q] |= want[q] && seen[q] >= t.lim ? kJoinFlag : 0;
+      relax_packed_d<kDIlp>(a, v, alt, want, t.settle, t.label, t.s_pair, out, t.nbmin, t.nrem);
     }
   }
 }
@@ -616,8 +636,8 @@ __device__ __forceinline__ void pull_pass(const DArgs& a, const DPhase& d, DTile
     for (int g = 0; g < G; ++g) {
       const bool pk = hi[g] && dv[g] >= t.lim;
       const int x = pk ? v[g] : 0;
-      st[g] = a.rrow[x];
-      deg[g] = pk ? a.rrow[x + 1] - st[g] : 0;
+      st[g] = rec_field(a, x, kRecIn);
+      deg[g] = pk ? rec_field(a, x, kRecInEnd) - st[g] : 0;
       t.verts += pk;
     }
     // long in-rows go to the grid queue
@@ -662,9 +682,9 @@ __device__ __forceinline__ void pull_pass(const DArgs& a, const DPhase& d, DTile
       }
 #pragma unroll
       for (int q = 0; q < kDIlp; ++q) {
-        const int k = s_base[o[q]] + f[q];
-        src[q] = stream_load(a.rcol + k);
-        wk[q] = stream_load(a.rw + k);
+        const int2v e = stream_load(a.rcw + s_base[o[q]] + f[q]);
+        src[q] = e.x;
+        wk[q] = e.y;
       }
 #pragma unroll
       for (int q = 0; q < kDIlp; ++q) du[q] = a.dist[src[q]];
@@ -730,12 +750,14 @@ __device__ __forceinline__ void wave_pass(const DArgs& a, const DPhase& d, int p
   int32_t* s_du = s_base + GW;   // the entry's distance
   for (int g0 = gw; g0 < ngroups; g0 += G * nw) {
     int v[G], dv[G], st[G], deg[G];
-    bool sel[G];
+    bool sel[G], join[G];
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       const int i = (g0 + g * nw) * SMDP_WAVE + lane;
       const bool in = g0 + g * nw < ngroups && i < L;
-      v[g] = in ? (qin ? qin[i] : i) : -1;
+      const int raw = in ? (qin ? qin[i] : i) : -1;
+      join[g] = in && qin && raw < 0;
+      v[g] = in ? raw & ~kJoinFlag : -1;
     }
 #pragma unroll
     for (int g = 0; g < G; ++g) {
@@ -749,13 +771,6 @@ __device__ __forceinline__ void wave_pass(const DArgs& a, const DPhase& d, int p
         t.nbmin = min(t.nbmin, dv[g]);
         ++t.nrem;
       }
-      if (t.members && dv[g] != SMDP_SSSP_INF && (dv[g] >> a.shift) == bucket) {
-        // the bucket's first sweep: its vertices with out-arcs join the census
-        // (leaving out those without makes a pull look a little dearer)
-        const int2 dg = a.deg2[v[g]];
-        t.jout += (unsigned)dg.x;
-        t.jin += (unsigned)dg.y;
-      }
     }
     // the pass's arc range of each selected entry (vertex 0's for the others)
 #pragma unroll
@@ -764,18 +779,28 @@ __device__ __forceinline__ void wave_pass(const DArgs& a, const DPhase& d, int p
       const int du = sel[g] ? dv[g] : 0;
       int e;
       if (t.tail) {
-        st[g] = a.row[x];
-        e = a.row[x + 1];
+        st[g] = rec_field(a, x, 0);
+        e = rec_field(a, x, kRecEnd);
       } else if (t.settle) {
         st[g] = settle_start(a, x, du, t.lim);
-        e = a.row[x + 1];
+        e = rec_field(a, x, kRecEnd);
       } else {
-        st[g] = a.row[x];
+        st[g] = rec_field(a, x, 0);
         e = light_end(a, x, du, t.lim);
       }
       deg[g] = sel[g] ? e - st[g] : 0;
       t.verts += sel[g];
       t.nsel += sel[g];
+      // the census: the bucket's first sweep counts the vertices it finds in
+      // the bucket, a light pass those flagged when they came below lim (the
+      // degrees share the record's line; vertices without out-arcs are never
+      // expanded or queued and stay out, which makes a pull look a little
+      // dearer)
+      if ((t.members || (t.census && join[g])) && sel[g]) {
+        const int2v dg = rec_degrees(a, x);
+        t.jout += (unsigned)dg.x;
+        t.jin += (unsigned)dg.y;
+      }
     }
     // long ranges go to the grid queue
 #pragma unroll
@@ -820,9 +845,9 @@ __device__ __forceinline__ void wave_pass(const DArgs& a, const DPhase& d, int p
       }
 #pragma unroll
       for (int q = 0; q < kDIlp; ++q) {
-        const int k = s_base[o[q]] + f[q];
-        vt[q] = stream_load(a.col + k);
-        alt[q] = s_du[o[q]] + stream_load(a.w + k);
+        const int2v e = stream_load(a.cw + s_base[o[q]] + f[q]);
+        vt[q] = e.x;
+        alt[q] = s_du[o[q]] + e.y;
       }
 #pragma unroll
       for (int q = 0; q < kDIlp; ++q) {
@@ -836,22 +861,11 @@ __device__ __forceinline__ void wave_pass(const DArgs& a, const DPhase& d, int p
       for (int q = 0; q < kDIlp; ++q) seen[q] = rel[q] ? a.dist[vt[q]] : 0;
 #pragma unroll
       for (int q = 0; q < kDIlp; ++q) want[q] = rel[q] && alt[q] < seen[q];
-      if (t.census) {
-        // as in relax_staged: a candidate to a vertex read at or beyond lim
-        // brings its degrees into the census
-        int2 dg[kDIlp];
+      // as in relax_staged: entries of vertices brought below lim are flagged
+      if (t.census)
 #pragma unroll
-        for (int q = 0; q < kDIlp; ++q) dg[q] = a.deg2[want[q] && seen[q] >= t.lim ? vt[q] : 0];
-        relax_packed_d<kDIlp>(a, vt, alt, want, false, t.label, t.s_pair, out, t.nbmin, t.nrem);
-#pragma unroll
-        for (int q = 0; q < kDIlp; ++q)
-          if (want[q] && seen[q] >= t.lim) {
-            t.jout += (unsigned)dg[q].x;
-            t.jin += (unsigned)dg[q].y;
-          }
-      } else {
-        relax_packed_d<kDIlp>(a, vt, alt, want, t.settle, t.label, t.s_pair, out, t.nbmin, t.nrem);
-      }
+        for (int q = 0; q < kDIlp; ++q) vt[q] |= want[q] && seen[q] >= t.lim ? kJoinFlag : 0;
+      relax_packed_d<kDIlp>(a, vt, alt, want, t.settle, t.label, t.s_pair, out, t.nbmin, t.nrem);
     }
     // the LDS area is written again by the next groups
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -898,8 +912,8 @@ __device__ __forceinline__ unsigned long long phase_chunks(const DArgs& a, const
         // range only grows, and the vertex is queued again for the rest (a
         // pulling vertex is written only by its own chunks, so du stays at
         // or beyond lim)
-        const int vend = t.pull ? a.rrow[c.v + 1]
-                                : (t.settle || t.tail ? a.row[c.v + 1] : light_end(a, c.v, du, t.lim));
+        const int vend = t.pull ? rec_field(a, c.v, kRecInEnd)
+                                : (t.settle || t.tail ? rec_field(a, c.v, kRecEnd) : light_end(a, c.v, du, t.lim));
         const int end = vend - c.first > a.chunk ? c.first + a.chunk : vend;
         len = end - c.first;
         st = c.first;
@@ -1055,13 +1069,16 @@ __device__ __forceinline__ int arc_group(int w, int delta, int bshift) { return 
 // Rows of the renumbered copy reordered by weight band: band j < kBands holds
 // the light arcs with j * width <= w < (j + 1) * width (width = 1 << bshift =
 // delta / kBands, at least 1), band kBands the heavy arcs (w >= delta); each
-// band keeps the arcs' original order.  band[u * kBands + j - 1] = first arc
-// of band j of row u.  One wave per row; lane j counts group j.
+// band keeps the arcs' original order.  Writes the reordered arcs twice (as
+// separate columns and weights for the unordered rounds, as {target, weight}
+// pairs for the bucketed passes) and the row's record: start, the first arc
+// of each band j >= 1, end and out-degree.  One wave per row; lane j counts
+// group j.
 __global__ __launch_bounds__(SMDP_WG) void k_band_rows(const int32_t* __restrict__ row,
                                                        const int32_t* __restrict__ col_in,
                                                        const int32_t* __restrict__ w_in, int32_t* col_out,
-                                                       int32_t* w_out, int32_t* band, int32_t n, int32_t delta,
-                                                       int32_t bshift) {
+                                                       int32_t* w_out, int2v* cw_out, int32_t* rec, int32_t n,
+                                                       int32_t delta, int32_t bshift) {
   static_assert(kBands < SMDP_WAVE, "one lane per group");
   const int lane = lane_id();
   const int64_t nw = (int64_t)gridDim.x * SMDP_WAVES_PER_WG;
@@ -1079,7 +1096,10 @@ __global__ __launch_bounds__(SMDP_WG) void k_band_rows(const int32_t* __restrict
     }
     const int mine = lane <= kBands ? cnt : 0;
     int base = s + wave_inclusive_scan(mine) - mine;  // lane j: first arc of group j
-    if (lane >= 1 && lane <= kBands) band[u * kBands + lane - 1] = base;
+    int32_t* r = rec + u * kRec;
+    if (lane <= kBands) r[lane] = base;  // lane 0: the row's start
+    if (lane == kRecEnd) r[lane] = e;
+    if (lane == kRecDeg) r[lane] = e - s;
     for (int b = s; b < e; b += SMDP_WAVE) {
       const int k = b + lane;
       const bool in = k < e;
@@ -1094,8 +1114,10 @@ __global__ __launch_bounds__(SMDP_WG) void k_band_rows(const int32_t* __restrict
         if (lane == j) base += __popcll(m);
       }
       if (in) {
-        col_out[pos] = col_in[k];
+        const int c = col_in[k];
+        col_out[pos] = c;
         w_out[pos] = wk;
+        cw_out[pos] = int2v{c, wk};
       }
     }
   }
@@ -1107,41 +1129,44 @@ __global__ __launch_bounds__(SMDP_WG) void k_band_rows(const int32_t* __restrict
 // the minimum over it, which does not depend on that order.
 __global__ __launch_bounds__(SMDP_WG) void k_in_arcs(const int32_t* __restrict__ row, const int32_t* __restrict__ col,
                                                      const int32_t* __restrict__ w, int32_t n, int32_t* cur,
-                                                     int32_t* rcol, int32_t* rw) {
+                                                     int2v* rcw) {
   const int lane = lane_id();
   const int64_t nw = (int64_t)gridDim.x * SMDP_WAVES_PER_WG;
   for (int64_t u = (int64_t)blockIdx.x * SMDP_WAVES_PER_WG + threadIdx.x / SMDP_WAVE; u < n; u += nw) {
     const int e = row[u + 1];
     for (int k = row[u] + lane; k < e; k += SMDP_WAVE) {
       const int pos = atomicAdd(&cur[col[k]], 1);
-      rcol[pos] = (int32_t)u;
-      rw[pos] = w[k];
+      rcw[pos] = int2v{(int32_t)u, w[k]};
     }
   }
 }
 
-// {out-degree, in-degree} of every vertex of the copy.
-__global__ void k_deg2(const int32_t* __restrict__ row, const int32_t* __restrict__ rrow, int32_t n, int2* deg2) {
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x)
-    deg2[v] = make_int2(row[v + 1] - row[v], rrow[v + 1] - rrow[v]);
+// In-row start, end and in-degree of every vertex into its record.
+__global__ void k_rec_in(const int32_t* __restrict__ rrow, int32_t n, int32_t* rec) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+    const int s = rrow[v], e = rrow[v + 1];
+    int32_t* r = rec + v * kRec;
+    r[kRecIn] = s;
+    r[kRecInEnd] = e;
+    r[kRecDeg + 1] = e - s;
+  }
 }
 
-// The copy's in-arcs, degree pairs and in-arc bits for the pull settle, built
-// once per resident graph from the in-row offsets the renumbering computed.
+// The copy's in-arcs, in-row fields of the records and in-arc bits for the
+// pull settle, built once per resident graph (after the records) from the
+// in-row offsets the renumbering computed.
 static int ensure_pull(smdp_graph* g, Relabel* rl) {
   if (rl->rrow) return SMDP_OK;
   const int64_t n = g->n, m = std::max<int64_t>(1, g->m);
-  if ((int64_t)rl->h_rrow.size() != n + 1 || !rl->w) return SMDP_ERR_ARG;
+  if ((int64_t)rl->h_rrow.size() != n + 1 || !rl->w || !rl->rec) return SMDP_ERR_ARG;
   hipStream_t s = g->ctx->stream;
-  int32_t *rrow = nullptr, *rcol = nullptr, *rw = nullptr, *deg2 = nullptr, *cur = nullptr;
+  int32_t *rrow = nullptr, *rcw = nullptr, *cur = nullptr;
   auto release = [&]() {
-    for (int32_t* q : {rrow, rcol, rw, deg2, cur})
+    for (int32_t* q : {rrow, rcw, cur})
       if (q) (void)hipFree(q);
   };
   if (hipMalloc((void**)&rrow, sizeof(int32_t) * (n + 1)) != hipSuccess ||
-      hipMalloc((void**)&rcol, sizeof(int32_t) * m) != hipSuccess ||
-      hipMalloc((void**)&rw, sizeof(int32_t) * m) != hipSuccess ||
-      hipMalloc((void**)&deg2, sizeof(int2) * std::max<int64_t>(1, n)) != hipSuccess ||
+      hipMalloc((void**)&rcw, sizeof(int2v) * m) != hipSuccess ||
       hipMalloc((void**)&cur, sizeof(int32_t) * (n + 1)) != hipSuccess) {
     release();
     return SMDP_ERR_NOMEM;
@@ -1152,12 +1177,12 @@ static int ensure_pull(smdp_graph* g, Relabel* rl) {
   if (ok && g->m > 0) {
     const int64_t blocks = std::min<int64_t>((n + SMDP_WAVES_PER_WG - 1) / SMDP_WAVES_PER_WG, 16384);
     hipLaunchKernelGGL(k_in_arcs, dim3((unsigned)std::max<int64_t>(1, blocks)), dim3(SMDP_WG), 0, s, rl->row,
-                       rl->col, rl->w, g->n, cur, rcol, rw);
+                       rl->col, rl->w, g->n, cur, reinterpret_cast<int2v*>(rcw));
     ok = hipGetLastError() == hipSuccess;
   }
   if (ok && n > 0) {
-    hipLaunchKernelGGL(k_deg2, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 4096)), dim3(256), 0, s, rl->row,
-                       rrow, g->n, reinterpret_cast<int2*>(deg2));
+    hipLaunchKernelGGL(k_rec_in, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 4096)), dim3(256), 0, s, rrow,
+                       g->n, rl->rec);
     ok = hipGetLastError() == hipSuccess;
   }
   if (!ok || hipStreamSynchronize(s) != hipSuccess) {
@@ -1172,9 +1197,7 @@ static int ensure_pull(smdp_graph* g, Relabel* rl) {
     return rc;
   }
   rl->rrow = rrow;
-  rl->rcol = rcol;
-  rl->rw = rw;
-  rl->deg2 = deg2;
+  rl->rcw = rcw;
   std::vector<int32_t>().swap(rl->h_rrow);
   return SMDP_OK;
 }
@@ -1189,7 +1212,9 @@ static int band_shift(int delta) {
 }
 
 // Reorders the renumbered copy's rows into weight bands for bucket width
-// delta (once per delta) and allocates the phase descriptors.
+// delta (once per delta), writes their records and {target, weight} pairs, and
+// allocates the phase descriptors.  The records' in-row fields, written once
+// by ensure_pull, do not depend on delta.
 static int ensure_buckets(smdp_graph* g, Relabel* rl, int delta) {
   hipStream_t s = g->ctx->stream;
   // 4 phase descriptors, the persistent launch's barrier lines and flags
@@ -1198,9 +1223,10 @@ static int ensure_buckets(smdp_graph* g, Relabel* rl, int delta) {
     rl->hctl = nullptr;
     return SMDP_ERR_NOMEM;
   }
-  if (rl->band && rl->band_delta == delta) return SMDP_OK;
-  SMDP_CHECK(alloc_once(&rl->band, std::max<int64_t>(1, (int64_t)g->n * kBands)));
+  if (rl->rec && rl->band_delta == delta) return SMDP_OK;
   const int64_t m = std::max<int64_t>(1, g->m);
+  SMDP_CHECK(alloc_once(&rl->rec, std::max<int64_t>(1, (int64_t)g->n * kRec)));
+  SMDP_CHECK(alloc_once(&rl->cw, 2 * m));
   int32_t *col2 = nullptr, *w2 = nullptr;
   if (hipMalloc((void**)&col2, sizeof(int32_t) * m) != hipSuccess) return SMDP_ERR_NOMEM;
   if (hipMalloc((void**)&w2, sizeof(int32_t) * m) != hipSuccess) {
@@ -1209,7 +1235,8 @@ static int ensure_buckets(smdp_graph* g, Relabel* rl, int delta) {
   }
   const int64_t blocks = std::min<int64_t>(((int64_t)g->n + SMDP_WAVES_PER_WG - 1) / SMDP_WAVES_PER_WG, 16384);
   hipLaunchKernelGGL(k_band_rows, dim3((unsigned)std::max<int64_t>(1, blocks)), dim3(SMDP_WG), 0, s, rl->row,
-                     rl->col, rl->w, col2, w2, rl->band, g->n, delta, band_shift(delta));
+                     rl->col, rl->w, col2, w2, reinterpret_cast<int2v*>(rl->cw), rl->rec, g->n, delta,
+                     band_shift(delta));
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
     (void)hipFree(col2);
     (void)hipFree(w2);
@@ -1251,9 +1278,8 @@ int run_delta(smdp_graph* g, Relabel* rl, int32_t src, const smdp_params& p, int
 
   DArgs a;
   a.row = rl->row;
-  a.col = rl->col;
-  a.w = rl->w;
-  a.band = rl->band;
+  a.rec = rl->rec;
+  a.cw = reinterpret_cast<const int2v*>(rl->cw);
   a.hasout = rl->hasout;
   a.dist = g->dist;
   a.stamp = g->stamp;
@@ -1264,10 +1290,7 @@ int run_delta(smdp_graph* g, Relabel* rl, int32_t src, const smdp_params& p, int
   a.ph = reinterpret_cast<DPhase*>(rl->phase);
   a.cnt = g->cnt;
   const bool pull = settle != kSettlePush;
-  a.rrow = pull ? rl->rrow : nullptr;
-  a.rcol = pull ? rl->rcol : nullptr;
-  a.rw = pull ? rl->rw : nullptr;
-  a.deg2 = pull ? reinterpret_cast<const int2*>(rl->deg2) : nullptr;
+  a.rcw = pull ? reinterpret_cast<const int2v*>(rl->rcw) : nullptr;
   a.hasin = pull ? rl->hasin : nullptr;
   a.m = g->m;
   a.settle = settle;

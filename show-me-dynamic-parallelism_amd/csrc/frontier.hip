@@ -638,8 +638,8 @@ __global__ void k_unpermute(const int32_t* __restrict__ d, const int32_t* __rest
 void relabel_free(smdp_graph* g) {
   Relabel* rl = static_cast<Relabel*>(g->relabel);
   if (!rl) return;
-  void* ptrs[] = {rl->row, rl->col, rl->w, rl->newid, rl->hasout, rl->band, rl->phase,
-                  rl->rrow, rl->rcol, rl->rw, rl->deg2, rl->hasin};
+  void* ptrs[] = {rl->row, rl->col, rl->w, rl->newid, rl->hasout, rl->rec, rl->cw, rl->phase,
+                  rl->rrow, rl->rcw, rl->hasin};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (rl->hctl) (void)hipHostFree(rl->hctl);

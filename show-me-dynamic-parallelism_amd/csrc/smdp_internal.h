@@ -327,16 +327,18 @@ struct Relabel {
   uint32_t* hasout = nullptr;        // bit v set when row v of the copy has arcs
   std::vector<int32_t> h_newid;      // the same on the host (source mapping)
   int32_t minw = 0;                  // smallest arc weight (bucketing needs minw >= 0)
-  int32_t* band = nullptr;           // bucketed SSSP: band starts, kBands per row
-  int32_t band_delta = 0;            // the bucket width band was built for (0: not built)
+  // bucketed SSSP: per row one kRec-int record (kRec* fields, delta.hip):
+  // the row's start, its kBands band starts, its end, its in-row and its
+  // degrees; and the rows' arcs as {target, weight} pairs in band order
+  int32_t* rec = nullptr;
+  int32_t* cw = nullptr;
+  int32_t band_delta = 0;            // the bucket width rec and cw were built for (0: not built)
   int32_t* phase = nullptr;          // bucketed SSSP: 4 phase descriptors (delta.hip)
-  // bucketed SSSP, pull settle: the copy's in-arcs (rrow[v] .. rrow[v + 1]:
-  // sources in rcol, weights in rw), per vertex {out-degree, in-degree}, and
-  // bit v set when v has in-arcs; built once per resident graph
+  // bucketed SSSP, pull settle: the copy's in-row offsets, its in-arcs as
+  // {source, weight} pairs, and bit v set when v has in-arcs; built once per
+  // resident graph
   int32_t* rrow = nullptr;
-  int32_t* rcol = nullptr;
-  int32_t* rw = nullptr;
-  int32_t* deg2 = nullptr;
+  int32_t* rcw = nullptr;
   uint32_t* hasin = nullptr;
   std::vector<int32_t> h_rrow;       // rrow on the host, kept from the renumbering until uploaded
   void* hctl = nullptr;              // bucketed SSSP: pinned host copy of one control slot
