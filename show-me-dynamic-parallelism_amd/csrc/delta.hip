@@ -724,6 +724,18 @@ __device__ __forceinline__ void pull_pass(const DArgs& a, const DPhase& d, DTile
   }
 }
 
+// Entries per group of a wave pass over L entries: 64 for sweeps; a short
+// light queue is spread over the grid's waves a few entries each (the next
+// power of two of L / waves), so that its arcs are not left to a few waves.
+__device__ __forceinline__ int wave_group(int L, bool queue) {
+  if (!queue) return SMDP_WAVE;
+  const int nw = (int)gridDim.x * SMDP_WAVES_PER_WG;
+  const int per = (L + nw - 1) / nw;
+  int e = 1;
+  while (e < per && e < SMDP_WAVE) e <<= 1;
+  return e;
+}
+
 // Pass 1 of a sweep, light or push-settle phase, each wave on its own (no
 // workgroup barrier, so a compute unit's waves wait on their memory steps
 // independently).  Entries are the vertices 0 .. n-1 (sweeps) or the light
@@ -744,7 +756,8 @@ __device__ __forceinline__ void wave_pass(const DArgs& a, const DPhase& d, int p
   const int L = d.nin;
   const int bucket = d.bucket;
   const int32_t* qin = d.mode == kLight ? ((p & 1) ? a.q1 : a.q0) : nullptr;  // queued vertices have arcs
-  const int ngroups = (L + SMDP_WAVE - 1) / SMDP_WAVE;
+  const int E = wave_group(L, qin != nullptr);  // entries per group (lanes E .. 63 idle while reading entries)
+  const int ngroups = (L + E - 1) / E;
   int32_t* s_pre = t.wl;         // inclusive arc counts of the wave's G * 64 entries
   int32_t* s_base = s_pre + GW;  // first arc minus the entry's first position in the wave's arc list
   int32_t* s_du = s_base + GW;   // the entry's distance
@@ -753,8 +766,8 @@ __device__ __forceinline__ void wave_pass(const DArgs& a, const DPhase& d, int p
     bool sel[G], join[G];
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-      const int i = (g0 + g * nw) * SMDP_WAVE + lane;
-      const bool in = g0 + g * nw < ngroups && i < L;
+      const int i = (g0 + g * nw) * E + lane;
+      const bool in = lane < E && g0 + g * nw < ngroups && i < L;
       const int raw = in ? (qin ? qin[i] : i) : -1;
       join[g] = in && qin && raw < 0;
       v[g] = in ? raw & ~kJoinFlag : -1;
@@ -984,7 +997,10 @@ __global__ __launch_bounds__(SMDP_WG) void k_delta_step(DArgs a, int p) {
     if (p > 0 && a.ph[(p - 1) & 3].mode != kDone) count_reached(a);
     return;
   }
-  if (d.mode == kLight && (int)blockIdx.x * SMDP_WG >= d.nin) return;  // more workgroups than light tiles
+  // more workgroups than a short light queue has groups
+  if (d.mode == kLight &&
+      (int)blockIdx.x * SMDP_WAVES_PER_WG >= (d.nin + wave_group(d.nin, true) - 1) / wave_group(d.nin, true))
+    return;
   __shared__ DShared sh;
   DTile t;
   tile_init(a, d, p, sh, t);
