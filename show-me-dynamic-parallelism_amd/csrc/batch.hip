@@ -234,7 +234,16 @@ __global__ __launch_bounds__(SMDP_WG) void k_ms_drain(MArgs a, int r) {
 // waiting; flags[1 + group] = the rounds run, or -1 when rmax was reached.
 // The launch needs no host check between rounds, so it can be captured in a
 // hipGraph with the initialisation and replayed.
-__global__ __launch_bounds__(SMDP_WG) void k_ms_run(MArgs a, uint32_t* bar, int32_t* flags, int group, int rmax) {
+#ifndef SMDP_MS_WAVES
+#define SMDP_MS_WAVES 0
+#endif
+#if SMDP_MS_WAVES > 0
+#define SMDP_MS_RUN_ATTR __attribute__((amdgpu_waves_per_eu(SMDP_MS_WAVES, SMDP_MS_WAVES)))
+#else
+#define SMDP_MS_RUN_ATTR
+#endif
+__global__ __launch_bounds__(SMDP_WG) SMDP_MS_RUN_ATTR void k_ms_run(MArgs a, uint32_t* bar, int32_t* flags, int group,
+                                                                     int rmax) {
   __shared__ int32_t s_out[SMDP_WAVES_PER_WG * kMsOut];
   __shared__ int s_ok;
   WaveOut<kMsOut> out;
