@@ -273,23 +273,23 @@ __device__ __forceinline__ DPhase phase_of(const DArgs& a, int p) {
     d.bout = 0;
     return d;
   }
+  // the descriptor and every counter the next phase may need are read at
+  // once (one round of loads, whatever prev's mode turns out to be)
   const DPhase prev = a.ph[(p - 1) & 3];
   const Ctl* c = a.ctl + (p & 3);
+  const int qlen = c->qlen;
   int nb = 0, rem = 0, sel = 0;
   int64_t jin = 0, jout = 0;
-  if (prev.mode == kSettle) {
-    for (int k = 0; k < kClaimParts; ++k) {
-      nb = max(nb, c->nbk[k * kClaimStride]);
-      rem += c->rem[k * kClaimStride];
-      sel += c->sel[k * kClaimStride];
-    }
-  } else if (a.settle == kSettleAuto) {
-    for (int k = 0; k < kClaimParts; ++k) {
-      jin += c->jin[k * kClaimStride];
-      jout += c->jout[k * kClaimStride];
-    }
+#pragma unroll
+  for (int k = 0; k < kClaimParts; ++k) {
+    nb = max(nb, c->nbk[k * kClaimStride]);
+    rem += c->rem[k * kClaimStride];
+    sel += c->sel[k * kClaimStride];
+    jin += c->jin[k * kClaimStride];
+    jout += c->jout[k * kClaimStride];
   }
-  return next_phase(prev, c->qlen, nb, rem, sel, jin, jout, a.n, a.m, a.settle);
+  if (prev.mode == kSettle) jin = jout = 0;
+  return next_phase(prev, qlen, nb, rem, sel, jin, jout, a.n, a.m, a.settle);
 }
 
 // Candidate relaxations of one step, packed to the low lanes through the
