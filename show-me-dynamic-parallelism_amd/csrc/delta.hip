@@ -978,11 +978,25 @@ __device__ __forceinline__ void phase_begin(const DArgs& a, const DPhase& d, int
   }
 }
 
-// Out-arcs of the reached vertices into the work counters (grid-stride).
+// Out-arcs of the reached vertices into the work counters (grid-stride, 8
+// vertices per thread with their loads issued at once).
 __device__ __forceinline__ void count_reached(const DArgs& a) {
+  constexpr int K = 8;
   unsigned long long acc = 0;
-  for (int64_t v = (int64_t)blockIdx.x * SMDP_WG + threadIdx.x; v < a.n; v += (int64_t)gridDim.x * SMDP_WG)
-    if (a.dist[v] != SMDP_SSSP_INF) acc += (unsigned long long)(a.row[v + 1] - a.row[v]);
+  const int64_t stride = (int64_t)gridDim.x * SMDP_WG;
+  for (int64_t v0 = (int64_t)blockIdx.x * SMDP_WG + threadIdx.x; v0 < a.n; v0 += K * stride) {
+    int d[K], r0[K], r1[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int64_t v = min(v0 + k * stride, (int64_t)a.n - 1);
+      d[k] = a.dist[v];
+      r0[k] = a.row[v];
+      r1[k] = a.row[v + 1];
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (v0 + k * stride < a.n && d[k] != SMDP_SSSP_INF) acc += (unsigned long long)(r1[k] - r0[k]);
+  }
   acc = wave_sum(acc);
   if (lane_id() == 0 && acc) atomicAdd(&cnt_of(a.cnt)->reached, acc);
 }

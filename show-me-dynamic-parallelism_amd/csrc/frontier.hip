@@ -558,9 +558,23 @@ __global__ __launch_bounds__(SMDP_WG) void k_drain(FArgs a, int r) {
 // Out-arcs of reached vertices (the GTEPS numerator).
 __global__ void k_reached(const int32_t* __restrict__ row, const int32_t* __restrict__ dist, int32_t n,
                           int32_t inf, Counters* cnt) {
+  // 8 vertices per thread with their loads issued at once
+  constexpr int K = 8;
   unsigned long long acc = 0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    if (dist[i] != inf) acc += (unsigned long long)(row[i + 1] - row[i]);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += K * stride) {
+    int d[K], r0[K], r1[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int64_t i = min(i0 + k * stride, (int64_t)n - 1);
+      d[k] = dist[i];
+      r0[k] = row[i];
+      r1[k] = row[i + 1];
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (i0 + k * stride < n && d[k] != inf) acc += (unsigned long long)(r1[k] - r0[k]);
+  }
   acc = wave_sum(acc);
   if (lane_id() == 0 && acc) atomicAdd(&cnt_of(cnt)->reached, acc);
 }
