@@ -310,6 +310,7 @@ void smdp_destroy(smdp_ctx* c) {
     if (c->stage_ev[i]) (void)hipEventDestroy(c->stage_ev[i]);
   }
   if (c->rlog) (void)hipFree(c->rlog);
+  if (c->arena) (void)hipFree(c->arena);
   if (c->ev_begin) (void)hipEventDestroy(c->ev_begin);
   if (c->ev_end) (void)hipEventDestroy(c->ev_end);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -384,30 +385,53 @@ int smdp_round_log(smdp_ctx* ctx, int64_t* qlen, int64_t* arcs, double* ms, int3
   return k;
 }
 
+// Validates and uploads h into g (buffers already set in g are used as they
+// are); on failure destroys g.
+static int graph_fill(smdp_ctx* ctx, const smdp_csr* h, smdp_graph* g, smdp_graph** out);
+
 int smdp_graph_create(smdp_ctx* ctx, const smdp_csr* h, smdp_graph** out) {
-  if (!ctx || !h || !out || h->n < 0 || h->m < 0 || !h->row || (h->m > 0 && !h->col)) return SMDP_ERR_ARG;
-  if (h->row[h->n] != h->m || h->row[0] != 0) return SMDP_ERR_ARG;
-  // Every kernel indexes row[u..u+1] and dist[col[e]] without bounds checks,
-  // so the CSR is validated here once (on all host threads).
-  // (columns of large graphs are checked while they are staged for upload, below)
-  {
-    std::atomic<bool> bad{false};
-    parallel_ranges(h->n, [&](int64_t b, int64_t e) {
-      for (int64_t v = b; v < e; ++v)
-        if (h->row[v + 1] < h->row[v]) bad = true;
-    });
-    if (h->m < kStagedMin)
-      parallel_ranges(h->m, [&](int64_t b, int64_t e) {
-        bool local = false;
-        for (int64_t k = b; k < e; ++k) local |= (uint32_t)h->col[k] >= (uint32_t)h->n;
-        if (local) bad = true;
-      });
-    if (bad) return SMDP_ERR_ARG;
-  }
+  if (!ctx || !h || !out) return SMDP_ERR_ARG;
   *out = nullptr;
   SMDP_HIP(hipSetDevice(ctx->device));
-  smdp_graph* g = new smdp_graph();
+  return graph_fill(ctx, h, new smdp_graph(), out);
+}
+
+
+static int graph_fill(smdp_ctx* ctx, const smdp_csr* h, smdp_graph* g, smdp_graph** out) {
   g->ctx = ctx;
+  if (h->n < 0 || h->m < 0 || !h->row || (h->m > 0 && !h->col) || h->row[h->n] != h->m || h->row[0] != 0) {
+    smdp_graph_destroy(g);
+    return SMDP_ERR_ARG;
+  }
+  // Every kernel indexes row[u..u+1] and dist[col[e]] without bounds checks,
+  // so the CSR is validated here once (on up to all host threads, 1M entries
+  // each at least: a check reads an entry in well under a nanosecond, and a
+  // small graph's drop-in call would otherwise spend its time starting them).
+  // (columns of large graphs are checked while they are staged for upload, below)
+  {
+    constexpr int64_t kCheckGrain = int64_t(1) << 20;
+    std::atomic<bool> bad{false};
+    parallel_ranges(
+        h->n,
+        [&](int64_t b, int64_t e) {
+          for (int64_t v = b; v < e; ++v)
+            if (h->row[v + 1] < h->row[v]) bad = true;
+        },
+        kCheckGrain);
+    if (h->m < kStagedMin)
+      parallel_ranges(
+          h->m,
+          [&](int64_t b, int64_t e) {
+            bool local = false;
+            for (int64_t k = b; k < e; ++k) local |= (uint32_t)h->col[k] >= (uint32_t)h->n;
+            if (local) bad = true;
+          },
+          kCheckGrain);
+    if (bad) {
+      smdp_graph_destroy(g);
+      return SMDP_ERR_ARG;
+    }
+  }
   g->n = h->n;
   g->m = h->m;
   int rc = SMDP_OK;
@@ -437,6 +461,13 @@ int smdp_graph_create(smdp_ctx* ctx, const smdp_csr* h, smdp_graph** out) {
 void smdp_graph_destroy(smdp_graph* g) {
   if (!g) return;
   (void)hipSetDevice(g->ctx->device);
+  if (g->arena) {
+    // these live in the context's arena
+    g->row = g->col = g->w = g->dist = g->stamp = g->q[0] = g->q[1] = nullptr;
+    g->ctl = nullptr;
+    g->cnt = nullptr;
+    g->hq = nullptr;
+  }
   void* bufs[] = {g->row,    g->col,      g->w,       g->dist,     g->stamp,  g->hasout, g->q[0],  g->q[1],
                   g->hq,     g->ctl,      g->cnt,     g->val,      g->x,      g->y,     g->outdeg,
                   g->rank[0], g->rank[1], g->contrib, g->partials, g->chunk_part, g->prio, g->resume,
@@ -529,3 +560,56 @@ int smdp_graph_read(smdp_graph* g, int what, void* out, int64_t count) {
 }
 
 }  // extern "C"
+
+namespace smdp {
+int graph_create_transient(smdp_ctx* ctx, const smdp_csr* h, smdp_graph** out) {
+  if (!ctx || !h || !out || h->n < 0 || h->m < 0) return SMDP_ERR_ARG;
+  *out = nullptr;
+  SMDP_HIP(hipSetDevice(ctx->device));
+  const int64_t n = std::max<int64_t>(1, h->n), m = std::max<int64_t>(1, h->m);
+  // the grid queue holds the most any solution can push (chunks are at
+  // least 64 arcs): its capacity never has to grow
+  const int64_t hq_cap = n + m / 64 + 64;
+  auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t sizes[] = {al(4 * (size_t)(n + 1)),           al(4 * (size_t)m),  al(h->w ? 4 * (size_t)m : 0),
+                          al(4 * sizeof(Ctl)),                al(kCntParts * sizeof(Counters)),
+                          al(4 * (size_t)n),                  al(4 * (size_t)n),  al(4 * (size_t)n),
+                          al(4 * (size_t)n),                  al(sizeof(Chunk) * (size_t)hq_cap)};
+  size_t total = 0;
+  for (size_t b : sizes) total += b;
+  if (total > ctx->arena_bytes) {
+    if (ctx->arena) (void)hipFree(ctx->arena);
+    ctx->arena = nullptr;
+    ctx->arena_bytes = 0;
+    const size_t want = total + total / 4;
+    if (hipMalloc(&ctx->arena, want) != hipSuccess) {
+      ctx->arena = nullptr;
+      return SMDP_ERR_NOMEM;
+    }
+    ctx->arena_bytes = want;
+  }
+  char* base = static_cast<char*>(ctx->arena);
+  size_t off = 0;
+  auto take = [&](size_t b) {
+    void* p = base + off;
+    off += b;
+    return p;
+  };
+  smdp_graph* g = new smdp_graph();
+  g->arena = true;
+  g->transient = true;
+  g->row = static_cast<int32_t*>(take(sizes[0]));
+  g->col = static_cast<int32_t*>(take(sizes[1]));
+  void* wp = take(sizes[2]);
+  if (h->w) g->w = static_cast<int32_t*>(wp);
+  g->ctl = static_cast<Ctl*>(take(sizes[3]));
+  g->cnt = static_cast<Counters*>(take(sizes[4]));
+  g->dist = static_cast<int32_t*>(take(sizes[5]));
+  g->stamp = static_cast<int32_t*>(take(sizes[6]));
+  g->q[0] = static_cast<int32_t*>(take(sizes[7]));
+  g->q[1] = static_cast<int32_t*>(take(sizes[8]));
+  g->hq = static_cast<Chunk*>(take(sizes[9]));
+  g->hq_cap = hq_cap;
+  return graph_fill(ctx, h, g, out);
+}
+}  // namespace smdp

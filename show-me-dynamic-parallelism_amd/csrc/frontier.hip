@@ -1069,8 +1069,9 @@ int frontier_dropin(smdp_ctx* ctx, const smdp_csr* h, int32_t src, int sol, bool
   if (!ctx || !h || !out) return SMDP_ERR_ARG;
   double t0 = now_ms();
   smdp_graph* g = nullptr;
-  SMDP_CHECK(smdp_graph_create(ctx, h, &g));
-  g->transient = true;  // one solve: a renumbered copy would not pay for itself
+  // one solve: a renumbered copy would not pay for itself, and the buffers
+  // come from the context's arena
+  SMDP_CHECK(graph_create_transient(ctx, h, &g));
   int rc = run_frontier(g, src, sol, bfs, st);
   if (rc == SMDP_OK) rc = smdp_graph_read(g, bfs ? SMDP_RES_LEVEL : SMDP_RES_DIST, out, g->n);
   smdp_graph_destroy(g);

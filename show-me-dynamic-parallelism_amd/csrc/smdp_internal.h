@@ -135,6 +135,10 @@ struct smdp_ctx {
   unsigned long long* rlog = nullptr;  // per-round (queue length, arcs) when profiling
   std::vector<int64_t> log_qlen, log_arcs;
   std::vector<double> log_ms;
+  // device memory of the one-solve graphs of drop-in calls, kept and grown
+  // as needed so that such a call allocates nothing once warm
+  void* arena = nullptr;
+  size_t arena_bytes = 0;
 };
 
 namespace smdp {
@@ -195,6 +199,7 @@ struct smdp_graph {
   void* relabel = nullptr;   // renumbered copy for the tuned frontier (frontier.hip)
   bool dist_perm = false;    // dist holds the renumbered copy's result
   bool transient = false;    // created by a drop-in call for one solve
+  bool arena = false;        // row, col, w, ctl, cnt, dist, stamp, q and hq live in the context's arena
   int8_t forest = -1;        // every vertex has at most one in-arc: 1 yes, 0 no, -1 not checked yet
   uint32_t* hasout = nullptr; // tuned frontier: bit v set when row v has arcs
   int32_t* tree_lo = nullptr; // tree: level bookkeeping and slots (tree.hip)
@@ -300,11 +305,16 @@ constexpr int kFeed = SMDP_FEED;
 constexpr int kFeedSlack = SMDP_FEED_SLACK;
 
 // Runs body(lo, hi) over [0, count) split into contiguous ranges, one per
-// host thread (at most 16).
+// host thread (at most 16), each of at least `grain` items; with one range
+// the calling thread runs it (starting threads costs tens of microseconds).
 template <typename F>
-void parallel_ranges(int64_t count, F body) {
+void parallel_ranges(int64_t count, F body, int64_t grain = 256) {
   const int64_t nt = std::max<int64_t>(1, std::min<int64_t>({(int64_t)std::thread::hardware_concurrency(), 16,
-                                                             (count + 255) / 256}));
+                                                             count / std::max<int64_t>(1, grain)}));
+  if (nt == 1) {
+    body(0, count);
+    return;
+  }
   std::vector<std::thread> th;
   for (int64_t t = 0; t < nt; ++t) th.emplace_back([=] { body(count * t / nt, count * (t + 1) / nt); });
   for (auto& x : th) x.join();
@@ -314,9 +324,8 @@ void parallel_ranges(int64_t count, F body) {
 // (frontier.hip builds it; delta.hip partitions its rows).  Vertices are
 // renumbered by falling in-degree; rows keep their arcs in order until a
 // bucketed SSSP partitions each row into kBands weight bands of its light
-// arcs (w < delta), lightest band first, then the heavy arcs, with
-// band[v * kBands + j - 1] the first arc of band j of row v (j = kBands: the
-// first heavy arc).
+// arcs (w < delta), lightest band first, then the heavy arcs, and gives each
+// row a record of its band starts.
 constexpr int kBands = 8;  // weight bands of a row's light arcs (bucketed SSSP)
 
 struct Relabel {
@@ -352,6 +361,9 @@ int relabel_unpermute(smdp_graph* g, int32_t* out_dev, int64_t count);
 int ensure_forest(smdp_graph* g);
 // Builds *out (once): bit v of word v / 32 set when row v of `row` has arcs.
 int build_hasout(smdp_graph* g, const int32_t* row, uint32_t** out);
+// A one-solve graph for a drop-in call: as smdp_graph_create, with its CSR
+// and frontier buffers in the context's arena (api.hip).
+int graph_create_transient(smdp_ctx* ctx, const smdp_csr* h, smdp_graph** out);
 int prof_collect(smdp_ctx* ctx, smdp_stats* st);
 
 void rowbufs_free(smdp_graph* g);
