@@ -291,7 +291,8 @@ __global__ __launch_bounds__(SMDP_WG) void k_color(CArgs a, int r) {
       }
     }
     (void)settled;
-    pushes += out.push(blocked, v, qout, &cout->qlen);
+    const int queued = out.push(blocked, v, qout, &cout->qlen);  // the same on every lane
+    if (lane == 0) pushes += queued;
   }
   out.flush(qout, &cout->qlen);
   edges = wave_sum(edges);
@@ -528,7 +529,8 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_seed(LArgs a) {
 // one after another, each waiting for its result (about half of the arcs
 // decrement).  Unconditional atomics that subtract 0 were measured 38x
 // slower, since the padded slots of a step's last pass all hit one vertex.
-// Returns the vertices this wave queued.
+// Returns the vertices this wave queued on lane 0, 0 on the other lanes (so a
+// wave_sum of the running totals counts each once).
 template <int K>
 __device__ __forceinline__ int notify_packed(const LArgs& a, const int* u, const uint32_t* pu, const int* v,
                                              const bool* ok, int newc, int32_t* s_pack, WaveOut<kCOut>& out,
@@ -570,7 +572,7 @@ __device__ __forceinline__ int notify_packed(const LArgs& a, const int* u, const
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  return pushes;
+  return lane_id() == 0 ? pushes : 0;
 }
 
 // Shape of round r's work, from its queue length L and the grid (the expand
@@ -606,6 +608,7 @@ constexpr int kShortMinWG = 128;
 // Expands short layer r (L <= kShortL queued vertices) on workgroups
 // 0 .. G-1 of the launch, G = min(workgroups, max(L / kShortPerWG,
 // kShortMinWG)); the others return at once.  s_raw: 3 * kShortL ints.
+template <bool COH>
 __device__ __forceinline__ void short_layer(const LArgs& a, int r, int L, int32_t* s_raw, int* s_scan,
                                             int32_t* s_pack, WaveOut<kCOut>& out, unsigned long long& edges,
                                             unsigned long long& verts, unsigned long long& pushes) {
@@ -621,7 +624,10 @@ __device__ __forceinline__ void short_layer(const LArgs& a, int r, int L, int32_
   constexpr int E = kShortL / SMDP_WG;  // consecutive entries staged per thread
   int vv[E], st[E], en[E];
 #pragma unroll
-  for (int k = 0; k < E; ++k) vv[k] = qin[min((int)threadIdx.x * E + k, L - 1)];
+  for (int k = 0; k < E; ++k) {
+    const int i = min((int)threadIdx.x * E + k, L - 1);
+    vv[k] = COH ? __hip_atomic_load(qin + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : qin[i];
+  }
 #pragma unroll
   for (int k = 0; k < E; ++k) {
     st[k] = a.trow[vv[k]];
@@ -702,7 +708,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_expand(LArgs a) {
   unsigned long long edges = 0, verts = 0, pushes = 0, pg = 0;
 
   if (L <= kShortL) {
-    short_layer(a, r, L, s_raw, s_scan, s_pack, out, edges, verts, pushes);
+    short_layer<false>(a, r, L, s_raw, s_scan, s_pack, out, edges, verts, pushes);
   } else {
   const LayerShape sh = layer_shape(a, L, a.grid);  // launched with a.grid workgroups
   const int ntiles = (L + sh.tile - 1) / sh.tile;
@@ -800,27 +806,91 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_expand(LArgs a) {
 // has pushed).  Then rd[0] = the first layer not run, for k_layer_expand.
 // RMAT-23 has ~2300 such layers; one expand and one drain launch per layer
 // cost about 18 us, mostly launch and ramp.
+constexpr int kFlagGrid = 64;  // workgroups the flag barrier serves (one flag per lane of a wave)
+static_assert(2 * kFlagGrid * 2 <= kBarLines * 32, "two parities of 64-bit flags fit the barrier lines");
+
+// Flag barrier between the short layers (at most kFlagGrid workgroups).
+// Workgroup b stores (epoch << 32 | entries it queued in this layer) in its
+// flag of the epoch's parity; wave 0 reads every flag, one per lane, until
+// all carry the epoch, and the sum of their entries is the next layer's
+// length, returned to the whole workgroup (-1 after about a second of
+// polling: not every workgroup is resident).  A workgroup writes the flags
+// of one parity again only at epoch e + 2, after every workgroup stored its
+// epoch e + 1 flag, which each does after it read all of epoch e: no read of
+// epoch e sees a later count.  queued: this wave's entries (read on lane 0).
+//
+// What crosses workgroups between layers is stored and read device-coherent
+// (queue entries, the cleared control slot, the flags: global_store /
+// global_load with sc1), and each wave waits for its stores to complete
+// before the flag is written, so no agent-scope release (an L2 write-back)
+// or acquire (an L2 invalidate) is needed.  Colours are read only after the
+// kernel; pend[] changes only by atomics, which are performed beyond L2.
+// Against the two-level atomic barrier with agent-scope fences, RMAT-23's
+// colouring went from 35.4 to 33.9 ms (flags with the fences kept: 34.6).
+__device__ __forceinline__ int flag_barrier(uint64_t* flags, uint32_t epoch, int queued, int* s_cnt, int* s_len) {
+  if (lane_id() == 0 && queued) atomicAdd(s_cnt, queued);
+  __builtin_amdgcn_s_waitcnt(0);  // this wave's coherent stores are complete
+  __syncthreads();
+  uint64_t* f = flags + (epoch & 1) * kFlagGrid;
+  if (threadIdx.x < SMDP_WAVE) {
+    const int lane = threadIdx.x;
+    const int G = gridDim.x;
+    if (lane == 0) {
+      const uint64_t mine = ((uint64_t)epoch << 32) | (uint32_t)*s_cnt;
+      *s_cnt = 0;  // the next adds come after the __syncthreads below
+      __hip_atomic_store(f + blockIdx.x, mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    uint64_t v = 0;
+    bool ok = true;
+    for (int spins = 0;;) {
+      v = lane < G ? __hip_atomic_load(f + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                   : (uint64_t)epoch << 32;
+      if (ballot((uint32_t)(v >> 32) < epoch) == 0) break;
+      __builtin_amdgcn_s_sleep(2);
+      if (++spins > kBarSpinLimit) {
+        ok = false;
+        break;
+      }
+    }
+    const unsigned long long tot = wave_sum((unsigned long long)(lane < G ? (uint32_t)v : 0u));
+    if (lane == 0) *s_len = ok ? (int)tot : -1;
+  }
+  __syncthreads();
+  return *s_len;
+}
+
+// ctl_clear with device-coherent stores (no L2 write-back publishes them).
+__device__ __forceinline__ void ctl_clear_coherent(Ctl* ctl, int slot) {
+  if (blockIdx.x != 0) return;
+  int32_t* c = reinterpret_cast<int32_t*>(ctl + slot);
+  for (int i = threadIdx.x; i < kCtlInts; i += blockDim.x)
+    __hip_atomic_store(c + i, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __global__ __launch_bounds__(SMDP_WG) void k_layer_short(LArgs a) {
   __shared__ int32_t s_raw[3 * kShortL];
   __shared__ int s_scan[SMDP_WAVES_PER_WG];
   __shared__ int32_t s_out[SMDP_WAVES_PER_WG * kCOut];
   __shared__ int32_t s_pack[SMDP_WAVES_PER_WG * kLayerIlp * SMDP_WAVE];  // notify_packed
-  __shared__ int s_ok;
+  __shared__ int s_cnt, s_len;
   const int r0 = a.rd[1];
   if (a.ctl[r0 & 3].qlen == 0) return;  // the colouring is complete (every workgroup sees it)
+  if (threadIdx.x == 0) s_cnt = 0;
   WaveOut<kCOut> out;
-  out.init(s_out + (threadIdx.x / SMDP_WAVE) * kCOut);
+  out.init(s_out + (threadIdx.x / SMDP_WAVE) * kCOut, INT32_MAX, true);
   unsigned long long edges = 0, verts = 0, pushes = 0;
+  uint64_t* flags = reinterpret_cast<uint64_t*>(a.bar);
   int r = r0 + 1;
+  int L = a.ctl[r & 3].qlen;  // written by the kernels before this one
   for (uint32_t epoch = 1;; ++epoch) {
-    // read after the previous barrier: the same in every workgroup
-    const int L = a.ctl[r & 3].qlen;
     if (L == 0 || L > kShortL) break;
-    ctl_clear(a.ctl, (r + 2) & 3);  // workgroup 0; no one uses that slot until layer r + 1
+    ctl_clear_coherent(a.ctl, (r + 2) & 3);  // workgroup 0; no one uses that slot until layer r + 1
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&cnt_of(a.cnt)->rounds, (unsigned long long)(r + 1));
-    short_layer(a, r, L, s_raw, s_scan, s_pack, out, edges, verts, pushes);
+    const unsigned long long before = pushes;
+    short_layer<true>(a, r, L, s_raw, s_scan, s_pack, out, edges, verts, pushes);
     out.flush((r & 1) ? a.q0 : a.q1, &a.ctl[(r + 1) & 3].qlen);
-    if (!grid_barrier(a.bar, epoch, &s_ok)) {
+    L = flag_barrier(flags, epoch, (int)(pushes - before), &s_cnt, &s_len);
+    if (L < 0) {
       if (threadIdx.x == 0) a.rd[3] = 1;
       break;
     }
@@ -1034,7 +1104,7 @@ static int run_color_layers(smdp_graph* g, int use_prio, smdp_stats* st) {
   SMDP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_layer_short),
                                                           SMDP_WG, 0));
   if (per_cu < 1) return SMDP_ERR_DEVICE;
-  const int short_grid = std::min(std::max(8, ctx->cus / 4), per_cu * ctx->cus);
+  const int short_grid = std::min({std::max(8, ctx->cus / 4), per_cu * ctx->cus, kFlagGrid});
   hipStream_t s = ctx->stream;
   prof_reset(ctx);
   int launches = 0;

@@ -55,11 +55,13 @@ struct WaveOut {
   int32_t* buf;  // this wave's CAP-entry LDS region
   int n;
   int32_t qcap;  // entries the global queue holds: later ones are counted, not stored
+  bool coherent;  // entries stored device-coherent (visible to other XCDs without an L2 write-back)
 
-  __device__ __forceinline__ void init(int32_t* region, int32_t queue_cap = INT32_MAX) {
+  __device__ __forceinline__ void init(int32_t* region, int32_t queue_cap = INT32_MAX, bool coh = false) {
     buf = region;
     n = 0;
     qcap = queue_cap;
+    coherent = coh;
   }
 
   __device__ __forceinline__ void flush(int32_t* __restrict__ q, int32_t* counter) {
@@ -74,8 +76,13 @@ struct WaveOut {
     int base = 0;
     if (lane_id() == leader) base = atomicAdd(counter, n);
     base = __shfl(base, leader);
-    for (int j = me; j < n; j += stride)
-      if (base + j < qcap) q[base + j] = buf[j];
+    if (coherent) {
+      for (int j = me; j < n; j += stride)
+        if (base + j < qcap) __hip_atomic_store(q + base + j, buf[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      for (int j = me; j < n; j += stride)
+        if (base + j < qcap) q[base + j] = buf[j];
+    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

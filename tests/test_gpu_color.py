@@ -40,6 +40,9 @@ def test_color_parity(ctx, name, use_prio, sol):
     c, st = ctx.color(g, prio, sol)
     assert np.array_equal(c, ref)
     assert st.rounds == k
+    if sol == smdp.SOL_TUNED:
+        # layering queues every vertex of colour >= 2 once, when its last preceding neighbour is coloured
+        assert st.q_push[0] == int((ref >= 2).sum())
 
 
 @pytest.mark.parametrize("sol", SOLS)
@@ -95,3 +98,4 @@ def test_color_layers_long_and_short(ctx, use_prio):
     c, st = ctx.color(g, prio, smdp.SOL_TUNED)
     assert np.array_equal(c, ref)
     assert st.rounds == k
+    assert st.q_push[0] == int((ref >= 2).sum())
