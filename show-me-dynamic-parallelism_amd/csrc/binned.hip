@@ -57,6 +57,8 @@ constexpr int kScatterBlocks = kScatterArcs / kScatterBlock + 1;
 constexpr int kWidePiece = 16384;             // arcs per piece of a wide row
 constexpr int kQuads = 8;                     // 4-arc groups each thread loads at once
 constexpr int kPad = 8;                       // entries past m in vals and dst: 4-arc loads may touch them
+constexpr int kFront = 4;                     // entries before vals and dst: a bin without arcs that starts
+                                              // at arc 0 loads the group of 4 just below it (never used)
 constexpr int kSrcPad = kScatterBlock + 8;    // entries past m in src: a scatter block may read that far
 constexpr int64_t kMaxSegments = int64_t(1) << 27;  // bound on nchunks x nbins (host plan memory)
 
@@ -248,10 +250,11 @@ __global__ __launch_bounds__(kBinNT, 2) void k_bin_reduce(RArgs a, BArgs b, int 
     uint16_t perm[kRowsPerThread];
     float rold[kRowsPerThread];
     int od[kRowsPerThread];
-    // Loads go to clamped positions (a bin has at least one row): a load under
-    // a per-lane condition made the compiler wait for it before the next one,
-    // which held back the bin's value loads by three round trips.  Values
-    // loaded past nr are never used.
+    // Loads go to clamped positions (a bin has at least one row; a bin without
+    // arcs loads the group of 4 below its start, kFront entries of padding at
+    // arc 0): a load under a per-lane condition made the compiler wait for it
+    // before the next one, which held back the bin's value loads by three
+    // round trips.  Values loaded past nr or outside [beg, end) are never used.
 #pragma unroll
     for (int q = 0; q < kRowLoads; ++q) roff[q] = a.row[r0 + min((int)threadIdx.x + q * kBinNT, nr)];
 #pragma unroll
@@ -481,7 +484,7 @@ int binned_ensure(smdp_graph* g, void** slot, BArgs* out) {
       at += cnt[(size_t)(c * B + b)];
     }
   }
-  std::vector<uint16_t> src((size_t)(m + kSrcPad), 0), dst((size_t)(m + kPad), 0);
+  std::vector<uint16_t> src((size_t)(m + kSrcPad), 0), dst((size_t)(kFront + m + kPad), 0);
   parallel_ranges(B, [&](int64_t b0, int64_t b1) {
     std::vector<int32_t> cs((size_t)K), bs((size_t)K);
     for (int64_t b = b0; b < b1; ++b) {
@@ -494,7 +497,7 @@ int binned_ensure(smdp_graph* g, void** slot, BArgs* out) {
         const int32_t u = col[e];
         const int32_t c = u >> kSrcChunkLog;
         src[cs[(size_t)c]++] = (uint16_t)(u & (kSrcChunk - 1));
-        dst[bs[(size_t)c]++] = wide[b] ? 0 : (uint16_t)(e - e0);
+        dst[(size_t)kFront + bs[(size_t)c]++] = wide[b] ? 0 : (uint16_t)(e - e0);
       }
     }
   });
@@ -587,7 +590,7 @@ int binned_ensure(smdp_graph* g, void** slot, BArgs* out) {
   step(upload(&p->r_item, r_item));
   step(upload(&p->wfirst, wfirst));
   step(upload(&p->wrow, wrow));
-  if (rc == SMDP_OK) step(alloc_once(&p->vals, m + kPad));
+  if (rc == SMDP_OK) step(alloc_once(&p->vals, kFront + m + kPad));
   if (rc == SMDP_OK) step(alloc_once(&p->wpart, std::max<int64_t>(1, npieces)));
   if (rc == SMDP_OK) step(alloc_once(&p->bpart, std::max<int64_t>(1, nparts)));
   if (rc != SMDP_OK) {
@@ -598,8 +601,8 @@ int binned_ensure(smdp_graph* g, void** slot, BArgs* out) {
   a.cm = p->cm;
   a.bm = p->bm;
   a.src = p->src;
-  a.dst = p->dst;
-  a.vals = p->vals;
+  a.dst = p->dst + kFront;
+  a.vals = p->vals + kFront;
   a.rperm = p->rperm;
   a.s_item = p->s_item;
   a.s_group = p->s_group;

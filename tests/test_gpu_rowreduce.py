@@ -277,7 +277,22 @@ def _wide_graph():
     return smdp.HostGraph(np.cumsum(row).astype(np.int32), dst[order].astype(np.int32))
 
 
-PR_BIN = {"wide": _wide_graph(), **{k: MATS[k] for k in ("rmat12", "random", "rmat10_noperm", "uniform")}}
+def _empty_head_graph():
+    """Every arc points into the last 100 vertices: the first 4900 rows of the
+    in-CSR are empty, so the first bins hold rows but no arcs, the first of
+    them starting at arc 0."""
+    rng = np.random.default_rng(12)
+    n = 5000
+    src = rng.integers(0, n, 20000)
+    dst = rng.integers(n - 100, n, 20000)
+    order = np.argsort(src, kind="stable")
+    row = np.zeros(n + 1, dtype=np.int32)
+    np.add.at(row, src + 1, 1)
+    return smdp.HostGraph(np.cumsum(row).astype(np.int32), dst[order].astype(np.int32))
+
+
+PR_BIN = {"wide": _wide_graph(), "empty_head": _empty_head_graph(),
+          **{k: MATS[k] for k in ("rmat12", "random", "rmat10_noperm", "uniform")}}
 
 
 @pytest.fixture
