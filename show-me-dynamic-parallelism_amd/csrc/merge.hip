@@ -96,7 +96,7 @@ __global__ __launch_bounds__(kMpNT) void k_spmv_merge(MArgs a) {
   // The tile's nr row ends and nz nonzeros fill its kMpItems merge items, so
   // one array holds both: slot k < nr the end of row i0 + k (in tile nonzero
   // positions, as int bits in .x), slot nr + j the product of nonzero j and
-  // its rounding error.  19 KB of LDS per workgroup: 8 workgroups per CU.
+  // its rounding error.  At 16 items per thread, 34 KB of LDS per workgroup: 4 workgroups per CU.
   __shared__ float2 s_item[kMpItems];
   __shared__ float2 s_cs[kMpNT];       // per thread: partial sum of its open row, scanned over the row's threads
   __shared__ int32_t s_hk[kMpNT / SMDP_WAVE], s_tk[kMpNT / SMDP_WAVE];  // per wave: open row of lanes 0 and 63
