@@ -411,8 +411,8 @@ def spmv_section(ctx, smdp, orc, np):
     t_par = time.perf_counter() - t0
     return {"workload": "spmv_rmat20", "rows": g.n, "nnz": g.m, "ms_device_median": round(ms, 4),
             "gflops": round(2 * g.m / (ms * 1e6), 2), "gteps": round(g.m / (ms * 1e6), 3),
-            "roofline": dict(_roof(12 * g.m + 8 * g.n + 4, ms), kernel="k_spmv_merge + k_merge_fixup",
-                             traffic=_sum_traffic("k_spmv_merge", "k_merge_fixup")),
+            "roofline": dict(_roof(12 * g.m + 8 * g.n + 4, ms), kernel="k_permute_x + k_spmv_merge + k_merge_fixup",
+                             traffic=_sum_traffic("k_permute_x", "k_spmv_merge", "k_merge_fixup")),
             "linf_vs_fp64": float(np.abs(y.astype(np.float64) - y64).max()),
             "cpu": _cpu_entry(t_cpu, t_par, nt, g.m, "one thread-per-row FP64 product (SpMV_kernel.cu:81-96 "
                               "restated, orc_spmv_f64 / orc_spmv_f64_par)")}
