@@ -504,6 +504,7 @@ int smdp_graph_set_outdeg(smdp_graph* g, const int32_t* outdeg) {
 int smdp_graph_set_prio(smdp_graph* g, const uint32_t* prio) {
   if (!g || !prio) return SMDP_ERR_ARG;
   SMDP_HIP(hipSetDevice(g->ctx->device));
+  ++g->prio_version;  // the per-arc priority copies of the colouring must be redone
   return upload4(g->ctx, &g->prio, prio, g->n);
 }
 

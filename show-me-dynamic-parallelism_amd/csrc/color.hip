@@ -368,6 +368,11 @@ struct LArgs {
   const int32_t* __restrict__ trow;
   const int32_t* __restrict__ tcol;
   const uint32_t* __restrict__ prio;  // nullptr: vertex-id priority
+  // with priorities: prio of each arc's vertex, aligned with col and tcol
+  // (built once per graph and priority upload), so an arc's priority comes
+  // with the arc instead of a dependent gather
+  const uint32_t* __restrict__ cprio;
+  const uint32_t* __restrict__ tprio;
   int32_t* color;
   int32_t* pend;
   int32_t* q0;
@@ -391,13 +396,6 @@ struct LArgs {
 constexpr int kLayerIlp = 4;
 constexpr int kLayerCountSlot = 3;  // control slot of the counting chunks (cleared when layer r = 1 starts)
 
-// (prio[u], u) < (prio[v], v) with pu = prio[u] already loaded.
-__device__ __forceinline__ bool lprec(const LArgs& a, int u, uint32_t pu, int v) {
-  if (!a.prio) return u < v;
-  const uint32_t pv = a.prio[v];
-  return pu < pv || (pu == pv && u < v);
-}
-
 __global__ void k_layer_init(Ctl* ctl, Counters* cnt, int32_t* rd) {
   if (threadIdx.x < 4) rd[threadIdx.x] = 0;
   int32_t* c = reinterpret_cast<int32_t*>(ctl);
@@ -415,7 +413,7 @@ __device__ __forceinline__ int count_preceding(const LArgs& a, int v, uint32_t p
     if (!a.prio) {
       p = u < v;
     } else {
-      const uint32_t pu = a.prio[u];
+      const uint32_t pu = a.cprio[q];
       p = pu < pv || (pu == pv && u < v);
     }
     c += p;
@@ -523,7 +521,8 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_seed(LArgs a) {
 // Arcs u[t] -> v[t] of the transposed CSR, K per lane (ok[t]: the arc exists;
 // v[t] is a valid vertex for every t).  An arc whose source precedes its
 // target decrements pend[v]; the lane that brings it to zero colours v with
-// newc and queues it.  Priorities are gathered unconditionally, K at a time.
+// newc and queues it.  pv[t]: the priority of v[t], loaded with the arc from
+// tprio (unused without priorities).
 // The decrements are packed to the low lanes through the wave's LDS area
 // (s_pack: K * 64 ints) and issued one per lane: K conditional atomics ran
 // one after another, each waiting for its result (about half of the arcs
@@ -533,13 +532,10 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_seed(LArgs a) {
 // wave_sum of the running totals counts each once).
 template <int K>
 __device__ __forceinline__ int notify_packed(const LArgs& a, const int* u, const uint32_t* pu, const int* v,
-                                             const bool* ok, int newc, int32_t* s_pack, WaveOut<kCOut>& out,
-                                             int32_t* qout, int32_t* qlen) {
+                                             const uint32_t* pv, const bool* ok, int newc, int32_t* s_pack,
+                                             WaveOut<kCOut>& out, int32_t* qout, int32_t* qlen) {
   bool want[K];
   if (a.prio) {
-    uint32_t pv[K];
-#pragma unroll
-    for (int t = 0; t < K; ++t) pv[t] = a.prio[v[t]];
 #pragma unroll
     for (int t = 0; t < K; ++t) want[t] = ok[t] && (pu[t] < pv[t] || (pu[t] == pv[t] && u[t] < v[t]));
   } else {
@@ -654,8 +650,8 @@ __device__ __forceinline__ void short_layer(const LArgs& a, int r, int L, int32_
   if (blockIdx.x == 0 && threadIdx.x == 0) verts += L;
   const int a0 = (int)((int64_t)A * blockIdx.x / G), a1 = (int)((int64_t)A * (blockIdx.x + 1) / G);
   for (int jb = a0; jb < a1; jb += kLayerIlp * SMDP_WG) {
-    int j[kLayerIlp], own[kLayerIlp], v[kLayerIlp], uu[kLayerIlp];
-    uint32_t pu[kLayerIlp];
+    int j[kLayerIlp], own[kLayerIlp], v[kLayerIlp], uu[kLayerIlp], k[kLayerIlp];
+    uint32_t pu[kLayerIlp], pv[kLayerIlp];
     bool ok[kLayerIlp];
 #pragma unroll
     for (int t = 0; t < kLayerIlp; ++t) j[t] = min(jb + (int)threadIdx.x + t * SMDP_WG, a1 - 1);
@@ -663,18 +659,22 @@ __device__ __forceinline__ void short_layer(const LArgs& a, int r, int L, int32_
 #pragma unroll
     for (int t = 0; t < kLayerIlp; ++t) {
       uu[t] = s_v[own[t]];
-      v[t] = stream_load(a.tcol + s_st[own[t]] + (j[t] - s_of[own[t]]));
+      k[t] = s_st[own[t]] + (j[t] - s_of[own[t]]);
+      v[t] = stream_load(a.tcol + k[t]);
       ok[t] = jb + (int)threadIdx.x + t * SMDP_WG < a1 && v[t] != uu[t];
       edges += jb + (int)threadIdx.x + t * SMDP_WG < a1;
     }
     if (a.prio) {
 #pragma unroll
-      for (int t = 0; t < kLayerIlp; ++t) pu[t] = a.prio[uu[t]];
+      for (int t = 0; t < kLayerIlp; ++t) {
+        pv[t] = stream_load(a.tprio + k[t]);
+        pu[t] = a.prio[uu[t]];
+      }
     } else {
 #pragma unroll
-      for (int t = 0; t < kLayerIlp; ++t) pu[t] = 0u;
+      for (int t = 0; t < kLayerIlp; ++t) pu[t] = pv[t] = 0u;
     }
-    pushes += notify_packed<kLayerIlp>(a, uu, pu, v, ok, newc,
+    pushes += notify_packed<kLayerIlp>(a, uu, pu, v, pv, ok, newc,
                                        s_pack + (threadIdx.x / SMDP_WAVE) * kLayerIlp * SMDP_WAVE, out, qout,
                                        &cout->qlen);
   }
@@ -766,21 +766,29 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_expand(LArgs a) {
     s_off[threadIdx.x] = ex;
     __syncthreads();
     for (int jb = 0; jb < tot; jb += kLayerIlp * SMDP_WG) {
-      int j[kLayerIlp], own[kLayerIlp], v[kLayerIlp], uu[kLayerIlp];
-      uint32_t pu[kLayerIlp];
+      int j[kLayerIlp], own[kLayerIlp], v[kLayerIlp], uu[kLayerIlp], k[kLayerIlp];
+      uint32_t pu[kLayerIlp], pv[kLayerIlp];
       bool ok[kLayerIlp];
 #pragma unroll
       for (int t = 0; t < kLayerIlp; ++t) j[t] = min(jb + (int)threadIdx.x + t * SMDP_WG, tot - 1);
       tile_owners<kLayerIlp>(s_off, j, own);
 #pragma unroll
       for (int t = 0; t < kLayerIlp; ++t) {
-        v[t] = stream_load(a.tcol + s_start[own[t]] + (j[t] - s_off[own[t]]));
+        k[t] = s_start[own[t]] + (j[t] - s_off[own[t]]);
+        v[t] = stream_load(a.tcol + k[t]);
         uu[t] = s_u[own[t]];
         pu[t] = s_pu[own[t]];
         ok[t] = jb + (int)threadIdx.x + t * SMDP_WG < tot && v[t] != uu[t];
         edges += jb + (int)threadIdx.x + t * SMDP_WG < tot;
       }
-      pushes += notify_packed<kLayerIlp>(a, uu, pu, v, ok, newc,
+      if (a.prio) {
+#pragma unroll
+        for (int t = 0; t < kLayerIlp; ++t) pv[t] = stream_load(a.tprio + k[t]);
+      } else {
+#pragma unroll
+        for (int t = 0; t < kLayerIlp; ++t) pv[t] = 0u;
+      }
+      pushes += notify_packed<kLayerIlp>(a, uu, pu, v, pv, ok, newc,
                                          s_pack + (threadIdx.x / SMDP_WAVE) * kLayerIlp * SMDP_WAVE, out, qout,
                                          &cout->qlen);
     }
@@ -944,18 +952,19 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_drain(LArgs a) {
     const uint32_t pu1 = a.prio ? a.prio[ch.v] : 0u;
     for (int b = ch.first; b < e; b += kLayerIlp * SMDP_WG) {
       int v[kLayerIlp], uu[kLayerIlp];
-      uint32_t pu[kLayerIlp];
+      uint32_t pu[kLayerIlp], pv[kLayerIlp];
       bool ok[kLayerIlp];
 #pragma unroll
       for (int t = 0; t < kLayerIlp; ++t) {
         const int k = b + (int)threadIdx.x + t * SMDP_WG;
         v[t] = stream_load(a.tcol + min(k, e - 1));
+        pv[t] = a.prio ? stream_load(a.tprio + min(k, e - 1)) : 0u;
         uu[t] = ch.v;
         pu[t] = pu1;
         ok[t] = k < e && v[t] != ch.v;
         edges += k < e;
       }
-      pushes += notify_packed<kLayerIlp>(a, uu, pu, v, ok, newc,
+      pushes += notify_packed<kLayerIlp>(a, uu, pu, v, pv, ok, newc,
                                          s_pack + (threadIdx.x / SMDP_WAVE) * kLayerIlp * SMDP_WAVE, out, qout,
                                          &cout->qlen);
     }
@@ -993,6 +1002,13 @@ __global__ void k_tfill(const int32_t* __restrict__ row, const int32_t* __restri
   }
 }
 
+// prio of each arc's vertex: out[e] = prio[col[e]].
+__global__ void k_arc_prio(const int32_t* __restrict__ col, int64_t m, const uint32_t* __restrict__ prio,
+                           uint32_t* __restrict__ out) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += (int64_t)gridDim.x * blockDim.x)
+    out[e] = prio[col[e]];
+}
+
 }  // namespace smdp
 
 namespace {
@@ -1003,7 +1019,10 @@ struct ColorBufs {
   int32_t* trow = nullptr;   // tuned: transposed CSR, built once per graph
   int32_t* tcol = nullptr;
   int32_t* rd = nullptr;     // tuned: device layer indices (LArgs::rd)
-  uint32_t* bar = nullptr;   // tuned: grid barrier lines of k_layer_short
+  uint32_t* bar = nullptr;   // tuned: barrier flags of k_layer_short
+  uint32_t* cprio = nullptr;  // tuned, with priorities: prio[col[e]] per arc
+  uint32_t* tprio = nullptr;  // tuned, with priorities: prio[tcol[e]] per arc
+  uint32_t prio_version = 0;  // the graph's priority upload cprio/tprio hold (0: none)
 };
 }  // namespace
 
@@ -1019,6 +1038,8 @@ void colorbufs_free(smdp_graph* g) {
   if (b->tcol) (void)hipFree(b->tcol);
   if (b->rd) (void)hipFree(b->rd);
   if (b->bar) (void)hipFree(b->bar);
+  if (b->cprio) (void)hipFree(b->cprio);
+  if (b->tprio) (void)hipFree(b->tprio);
   delete b;
   g->colorbufs = nullptr;
 }
@@ -1068,12 +1089,27 @@ static int run_color_layers(smdp_graph* g, int use_prio, smdp_stats* st) {
   SMDP_CHECK(ensure_transpose(g, b));
   SMDP_CHECK(alloc_once(&b->rd, 4));
   SMDP_CHECK(alloc_once(&b->bar, kBarLines * 32));
+  if (use_prio && b->prio_version != g->prio_version) {
+    // the arcs' priorities, aligned with col and tcol: once per priority upload
+    const int64_t m1 = std::max<int64_t>(1, g->m);
+    SMDP_CHECK(alloc_once(&b->cprio, m1));
+    SMDP_CHECK(alloc_once(&b->tprio, m1));
+    const int pg = std::max(1, ctx->cus * 8);
+    if (g->m) {
+      hipLaunchKernelGGL(k_arc_prio, dim3(pg), dim3(256), 0, ctx->stream, g->col, g->m, g->prio, b->cprio);
+      hipLaunchKernelGGL(k_arc_prio, dim3(pg), dim3(256), 0, ctx->stream, b->tcol, g->m, g->prio, b->tprio);
+      SMDP_HIP(hipGetLastError());
+    }
+    b->prio_version = g->prio_version;
+  }
   LArgs a;
   a.row = g->row;
   a.col = g->col;
   a.trow = b->trow;
   a.tcol = b->tcol;
   a.prio = use_prio ? g->prio : nullptr;
+  a.cprio = use_prio ? b->cprio : nullptr;
+  a.tprio = use_prio ? b->tprio : nullptr;
   a.color = g->dist;
   a.pend = b->pend;
   a.q0 = g->q[0];
@@ -1090,8 +1126,8 @@ static int run_color_layers(smdp_graph* g, int use_prio, smdp_stats* st) {
   a.grid = grid;
   a.rd = b->rd;
   a.bar = reinterpret_cast<uint32_t*>(b->bar);
-  // k_layer_short: its workgroups meet at grid barriers, so all must be
-  // resident at once.  The grid is a quarter of the CUs (its barrier costs
+  // k_layer_short: its workgroups meet at barriers between layers, so all
+  // must be resident at once.  The grid is a quarter of the CUs (its barrier costs
   // more per workgroup than a short layer gains from more of them: on
   // RMAT-23, 256/128/64/32/16/8 workgroups took 48.6/43.0/39.1/42.6/56.0/
   // 83.8 ms per colouring), capped at what the device holds, and launched

@@ -184,6 +184,7 @@ struct smdp_graph {
   int rank_cur = 0;             // which rank buffer holds the result
   // coloring
   uint32_t* prio = nullptr;
+  uint32_t prio_version = 0;    // bumped at every priority upload (per-arc priority copies follow it)
   int32_t* resume = nullptr;
   int32_t* rmax = nullptr;
   // multi-source SSSP

@@ -99,3 +99,17 @@ def test_color_layers_long_and_short(ctx, use_prio):
     assert np.array_equal(c, ref)
     assert st.rounds == k
     assert st.q_push[0] == int((ref >= 2).sum())
+
+
+def test_color_new_priorities_on_a_resident_graph(ctx):
+    """The tuned solution keeps each arc's priority beside the arc, built once per
+    priority upload: every new upload must be seen, with id order in between."""
+    g = smdp.gen_rmat(13, 8, symmetrize=True, seed=7)
+    with smdp.Graph(ctx, g) as dg:
+        for seed in (1, 2, 1):
+            prio = smdp.gen_prio(g.n, seed)
+            dg.set_prio(prio)
+            for use_prio in (True, False, True):
+                dg.color(use_prio, smdp.SOL_TUNED)
+                ref, _ = orc.color_closed(g.row, g.col, prio if use_prio else None)
+                assert np.array_equal(dg.read(smdp.RES_COLOR), ref), (seed, use_prio)
