@@ -70,6 +70,15 @@ def test_tree_chain_star_and_kary(ctx):
     _check(ctx, _tree_graph(np.array([-1] + [(v - 1) // k for v in range(1, m)]), 3), 0)
 
 
+@pytest.mark.parametrize("n", [1, 2, 6, 7, 8, 9, 17, 24, 25])
+def test_tree_depths_around_the_batches(ctx, n):
+    """Chains of n levels: the down launches go out in batches (8, then 16, ...)
+    with a host check of the depth between batches, so the empty level that
+    ends the down pass falls inside, at the end of or just past a batch."""
+    print(f"tree chain n={n}", flush=True)
+    _check(ctx, _tree_graph(np.arange(-1, n - 1), 1), 0)
+
+
 def test_tree_unreached_part_and_cycle_through_root(ctx):
     print("tree unreached and cycle", flush=True)
     parent, root = _random_parent(3000, 7)
