@@ -1046,14 +1046,14 @@ void colorbufs_free(smdp_graph* g) {
 
 // The transposed CSR: trow from a count of each vertex's appearances in col
 // (prefix sum on the host), tcol filled by one atomic cursor per vertex.
-static int ensure_transpose(smdp_graph* g, ColorBufs* b) {
-  if (b->trow) return SMDP_OK;
+static int build_transpose(smdp_graph* g, ColorBufs* b) {
   const int64_t n = g->n, m = g->m;
   hipStream_t s = g->ctx->stream;
   SMDP_CHECK(alloc_once(&b->trow, n + 1));
   SMDP_CHECK(alloc_once(&b->tcol, std::max<int64_t>(1, m)));
-  int32_t* tcur = nullptr;
-  SMDP_CHECK(alloc_once(&tcur, std::max<int64_t>(1, n)));
+  DeviceScratch<int32_t> cur;
+  SMDP_CHECK(alloc_once(&cur.p, std::max<int64_t>(1, n)));
+  int32_t* tcur = cur.p;
   SMDP_HIP(hipMemsetAsync(tcur, 0, sizeof(int32_t) * n, s));
   const int grid = std::max(1, g->ctx->cus * 8);
   if (m) hipLaunchKernelGGL(k_tdeg, dim3(grid), dim3(256), 0, s, g->col, m, tcur);
@@ -1068,8 +1068,21 @@ static int ensure_transpose(smdp_graph* g, ColorBufs* b) {
   if (m) hipLaunchKernelGGL(k_tfill, dim3(grid), dim3(256), 0, s, g->row, g->col, g->n, m, tcur, b->tcol);
   SMDP_HIP(hipGetLastError());
   SMDP_HIP(hipStreamSynchronize(s));
-  (void)hipFree(tcur);
   return SMDP_OK;
+}
+
+// A build that failed part of the way leaves nothing behind, so that a later
+// call does not take half-filled arrays for the transposed CSR.
+static int ensure_transpose(smdp_graph* g, ColorBufs* b) {
+  if (b->trow) return SMDP_OK;
+  const int rc = build_transpose(g, b);
+  if (rc != SMDP_OK) {
+    if (b->trow) (void)hipFree(b->trow);
+    if (b->tcol) (void)hipFree(b->tcol);
+    b->trow = nullptr;
+    b->tcol = nullptr;
+  }
+  return rc;
 }
 
 // Tuned solution: topological layering (see the kernels above).

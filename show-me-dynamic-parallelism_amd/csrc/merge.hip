@@ -324,8 +324,9 @@ void merge_free(void* p) {
 static int renumber_columns(smdp_graph* g, MergePlan* mp, hipStream_t s) {
   const int64_t n = g->n, m = g->m;
   const int grid = std::max(1, g->ctx->cus * 8);
-  int32_t* deg = nullptr;
-  SMDP_CHECK(alloc_once(&deg, n));
+  DeviceScratch<int32_t> tmp;
+  SMDP_CHECK(alloc_once(&tmp.p, n));
+  int32_t* deg = tmp.p;
   SMDP_HIP(hipMemsetAsync(deg, 0, sizeof(int32_t) * n, s));
   hipLaunchKernelGGL(k_col_degree, dim3(grid), dim3(256), 0, s, g->col, m, deg);
   std::vector<int32_t> hd((size_t)n);
@@ -354,7 +355,6 @@ static int renumber_columns(smdp_graph* g, MergePlan* mp, hipStream_t s) {
   hipLaunchKernelGGL(k_renumber_cols, dim3(grid), dim3(256), 0, s, g->col, m, deg, mp->pcol);
   SMDP_HIP(hipGetLastError());
   SMDP_HIP(hipStreamSynchronize(s));
-  (void)hipFree(deg);
   mp->ncols = (int32_t)ncols;
   return SMDP_OK;
 }

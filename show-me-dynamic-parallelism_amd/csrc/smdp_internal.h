@@ -224,6 +224,18 @@ int hip_fail(hipError_t e, const char* what, int line);
     if (_rc != SMDP_OK) return _rc; \
   } while (0)
 
+// A temporary device buffer of a set-up step, freed on every return from it.
+template <typename T>
+struct DeviceScratch {
+  T* p = nullptr;
+  DeviceScratch() = default;
+  DeviceScratch(const DeviceScratch&) = delete;
+  DeviceScratch& operator=(const DeviceScratch&) = delete;
+  ~DeviceScratch() {
+    if (p) (void)hipFree(p);
+  }
+};
+
 // Allocation helper: (re)allocates *p to hold `count` elements when too small.
 template <typename T>
 int ensure(T** p, int64_t* cap, int64_t count) {
