@@ -120,6 +120,28 @@ def _merge_mats():
     return mats
 
 
+@pytest.mark.parametrize("sol", SOLS)
+def test_spmv_columns_that_never_occur(ctx, sol):
+    """Columns are renumbered by how often they occur; x entries of columns no
+    nonzero uses must never be read: NaN there leaves y finite and exact."""
+    rng = np.random.default_rng(21)
+    n = 6000
+    deg = rng.integers(0, 9, n)
+    row = np.concatenate([[0], np.cumsum(deg)]).astype(np.int32)
+    hot = rng.integers(0, 40, int(row[-1]))            # a few frequent columns
+    cold = rng.integers(40, n // 2, int(row[-1]))      # columns >= n/2 never occur
+    col = np.where(rng.random(int(row[-1])) < 0.5, hot, cold).astype(np.int32)
+    g = smdp.HostGraph(row, col)
+    val = ((np.arange(g.m) % 7) - 3).astype(np.float32)
+    x = ((np.arange(n) % 5) + 1).astype(np.float32)
+    x[n // 2:] = np.nan
+    exact = np.zeros(n, np.int64)
+    np.add.at(exact, np.repeat(np.arange(n), deg), val.astype(np.int64) * x[:n // 2].astype(np.int64)[col])
+    y, _ = ctx.spmv(g, val, x, sol)
+    assert np.isfinite(y).all()
+    assert np.array_equal(y.astype(np.int64), exact)
+
+
 def test_spmv_tuned_tile_boundaries_exact(ctx):
     """Small integer values make every sum exact: a partial dropped or added twice changes y."""
     for i, g in enumerate(_merge_mats()):
