@@ -355,12 +355,18 @@ __global__ __launch_bounds__(SMDP_WG) void k_mg_reset(int32_t* next, Counters* c
 // Barrier of the k workgroups of one group (all resident: they are the
 // first workgroups of the launch).  Arrivals only grow (epoch e is complete
 // at e * k); the last publishes the epoch.  Returns false in a workgroup that
-// gave up waiting.
+// gave up waiting.  What a group's workgroups exchange between rounds is
+// stored and read device-coherent (queue entries, the initial distances and
+// stamps) or changed by atomics, which are performed beyond L2, and each wave
+// waits for its stores to complete before arriving: no agent-scope release
+// (an L2 write-back) or acquire (an L2 invalidate) is needed.  The plain
+// reads of dist[] and stamp[] are filters that a stale value cannot mislead
+// (distances only fall, stamps only rise).  256 small graphs: 5.44 -> 5.17 ms.
 __device__ __forceinline__ bool group_barrier(int32_t* gq, int k, int epoch, int* s_ok) {
+  __builtin_amdgcn_s_waitcnt(0);  // this wave's device-coherent stores are complete
   __syncthreads();
   if (threadIdx.x == 0) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this workgroup's queue writes
-    if (atomicAdd(&gq[3], 1) + 1 == epoch * k) __hip_atomic_store(&gq[4], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (atomicAdd(&gq[3], 1) + 1 == epoch * k) __hip_atomic_store(&gq[4], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     int ok = 1;
     for (int spins = 0; __hip_atomic_load(&gq[4], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch;) {
       __builtin_amdgcn_s_sleep(2);
@@ -369,7 +375,6 @@ __device__ __forceinline__ bool group_barrier(int32_t* gq, int k, int epoch, int
         break;
       }
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the others' queue writes
     *s_ok = ok;
   }
   __syncthreads();
@@ -414,12 +419,19 @@ __device__ void mg_solve(const GArgs& a, int g, int rank, int k, int32_t* gq, Mg
   int32_t* stamp = a.stamp + a.vbase[g];
   int32_t* q[2] = {a.qa + a.vbase[g], a.qb + a.vbase[g]};
   const int src = a.src[g];
+  constexpr bool COH = GROUP;  // stores other workgroups read go device-coherent
   for (int i = rank * NT + (int)threadIdx.x; i < n; i += k * NT) {
-    dist[i] = i == src ? 0 : SMDP_SSSP_INF;
-    stamp[i] = i == src ? 0 : -1;
+    if (COH) {
+      __hip_atomic_store(&dist[i], i == src ? 0 : SMDP_SSSP_INF, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&stamp[i], i == src ? 0 : -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      dist[i] = i == src ? 0 : SMDP_SSSP_INF;
+      stamp[i] = i == src ? 0 : -1;
+    }
   }
   if (rank == 0 && threadIdx.x == 0) {
-    q[0][0] = src;
+    if (COH) __hip_atomic_store(&q[0][0], src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else q[0][0] = src;
     if (GROUP) {
       __hip_atomic_store(&gq[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
@@ -449,13 +461,13 @@ __device__ void mg_solve(const GArgs& a, int g, int rank, int k, int32_t* gq, Mg
     int32_t* qlen_out = GROUP ? &gq[(r + 1) % 3] : &sh.qlen[cur ^ 1];
     const int label = r + 1;
     WaveOut<kMgOut> out;
-    out.init(sh.out + wave * kMgOut);
+    out.init(sh.out + wave * kMgOut, INT32_MAX, COH);
     int32_t* qout = q[cur ^ 1];
     for (int base = rank * NT; base < L; base += k * NT) {
       const int i = base + (int)threadIdx.x;
       int s = 0, deg = 0, du = 0;
       if (i < L) {
-        const int u = q[cur][i];
+        const int u = COH ? __hip_atomic_load(&q[cur][i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : q[cur][i];
         s = row[u];
         deg = row[u + 1] - s;
         // Rounds do not cross a kernel boundary here, so this CU's L1 may
