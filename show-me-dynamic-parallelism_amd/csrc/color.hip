@@ -388,9 +388,9 @@ struct LArgs {
   // Layer indices kept on the device, so that a launch can run several layers:
   // rd[0] the layer k_layer_expand / k_layer_drain run next, rd[1] the layer
   // the last drain ran, rd[2] set once a layer is empty (done), rd[3] set
-  // when a grid barrier gave up waiting.
+  // when a barrier between short layers gave up waiting.
   int32_t* rd;
-  uint32_t* bar;  // grid barrier of k_layer_short: kBarLines lines of 32 words
+  uint32_t* bar;  // flags of k_layer_short's barrier (two parities of 64-bit flags), zeroed before each launch
 };
 
 constexpr int kLayerIlp = 4;
@@ -809,8 +809,8 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_expand(LArgs a) {
 }
 
 // Runs short layers one after another in one launch: layers rd[1] + 1,
-// rd[1] + 2, ... while they hold at most kShortL vertices, with a grid
-// barrier after each (a layer's queue is complete only when every workgroup
+// rd[1] + 2, ... while they hold at most kShortL vertices, with a flag
+// barrier (below) after each (a layer's queue is complete only when every workgroup
 // has pushed).  Then rd[0] = the first layer not run, for k_layer_expand.
 // RMAT-23 has ~2300 such layers; one expand and one drain launch per layer
 // cost about 18 us, mostly launch and ramp.
@@ -922,7 +922,7 @@ __global__ __launch_bounds__(SMDP_WG) void k_layer_drain(LArgs a) {
   Ctl* cout = a.ctl + ((r + 1) & 3);
   if (blockIdx.x == 0) {
     // for k_layer_short, which runs next: the layer done here, and fresh
-    // barrier lines (no other kernel is using them now)
+    // barrier flags (no other kernel is using them now)
     if (threadIdx.x == 0) a.rd[1] = r;
     for (int i = threadIdx.x; i < kBarLines * 32; i += blockDim.x) a.bar[i] = 0;
   }
@@ -1191,7 +1191,7 @@ static int run_color_layers(smdp_graph* g, int use_prio, smdp_stats* st) {
     SMDP_HIP(hipGetLastError());
     SMDP_HIP(hipMemcpyAsync(ctx->pinned, a.rd, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     SMDP_HIP(hipStreamSynchronize(s));
-    if (ctx->pinned[3]) return SMDP_ERR_DEVICE;  // a grid barrier gave up waiting
+    if (ctx->pinned[3]) return SMDP_ERR_DEVICE;  // a barrier between short layers gave up waiting
     if (ctx->pinned[2]) break;
     if (ctx->pinned[0] >= p.max_rounds || ctx->pinned[0] >= g->n + 1) return SMDP_ERR_OVERFLOW;
     batch = std::min(batch * 2, 8);
