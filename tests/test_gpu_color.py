@@ -113,3 +113,30 @@ def test_color_new_priorities_on_a_resident_graph(ctx):
                 dg.color(use_prio, smdp.SOL_TUNED)
                 ref, _ = orc.color_closed(g.row, g.col, prio if use_prio else None)
                 assert np.array_equal(dg.read(smdp.RES_COLOR), ref), (seed, use_prio)
+
+
+def test_color_short_long_short_layers(ctx):
+    """Vertex-id priority on a symmetric graph whose layers go short (a path of
+    300 vertices, one vertex per layer), long (3000 vertices hanging off the
+    path's end: one layer above the 2048-vertex limit of the short-layer run),
+    and short again (a second path after them): the short-layer launch must hand
+    the long layer back to the tile kernels and resume after it."""
+    p1, fan, p2 = 300, 3000, 200
+    n = p1 + fan + p2
+    src, dst = [], []
+    src += list(range(p1 - 1)); dst += list(range(1, p1))            # first path
+    src += [p1 - 1] * fan; dst += list(range(p1, p1 + fan))           # the fan
+    src += list(range(p1, p1 + fan)); dst += [p1 + fan] * fan         # all join the second path
+    src += list(range(p1 + fan, n - 1)); dst += list(range(p1 + fan + 1, n))
+    src, dst = np.array(src), np.array(dst)
+    order = np.argsort(src, kind="stable")
+    row = np.zeros(n + 1, np.int64)
+    np.add.at(row, src + 1, 1)
+    srow, scol = symmetrize(np.cumsum(row).astype(np.int32), dst[order].astype(np.int32))
+    g = smdp.HostGraph(srow, scol)
+    ref, k = orc.color_closed(g.row, g.col, None)
+    assert int(np.bincount(ref).max()) > 2048 and k > 400
+    c, st = ctx.color(g, None, smdp.SOL_TUNED)
+    assert np.array_equal(c, ref)
+    assert st.rounds == k
+    assert st.q_push[0] == int((ref >= 2).sum())

@@ -285,3 +285,20 @@ def test_round_batches_follow_queue_decay(ctx):
     # queue decay (and the batch sizes) vary a little between runs: the bound
     # is the 28 rounds that plain doubling would enqueue
     assert rounds <= enqueued < 28, (rounds, enqueued)
+
+
+def test_dropin_arena_grows_and_is_reused(ctx):
+    """One-call SSSP and BFS carve their graph from a device arena kept by the
+    context: calls alternating between small and larger graphs (the arena grows,
+    then a smaller graph reuses it) stay exact, and so does a graph without arcs."""
+    sizes = [(2000, 8), (60000, 16), (300, 3), (120000, 8), (5000, 4), (1, 0)]
+    for i, (n, deg) in enumerate(sizes):
+        print(f"arena case {i}: n={n} deg={deg}", flush=True)
+        if deg:
+            g = smdp.gen_uniform(n, deg, 1, 100, 10 + i)
+        else:
+            g = smdp.HostGraph(np.zeros(n + 1, np.int32), np.zeros(0, np.int32), np.zeros(0, np.int32), 0)
+        d, _ = ctx.sssp(g, g.source, smdp.SOL_TUNED)
+        assert np.array_equal(d, orc.dijkstra(g.row, g.col, g.w, g.source))
+        lv, _ = ctx.bfs(g, g.source, smdp.SOL_TUNED)
+        assert np.array_equal(lv, orc.bfs(g.row, g.col, g.source))
