@@ -142,6 +142,24 @@ def test_spmv_columns_that_never_occur(ctx, sol):
     assert np.array_equal(y.astype(np.int64), exact)
 
 
+@pytest.mark.parametrize("sol", SOLS)
+def test_spmv_one_column(ctx, sol):
+    """Every nonzero in column 7 (the renumbering's degree range is then the
+    whole nonzero count, and one x value is read): exact integer sums."""
+    rng = np.random.default_rng(23)
+    n = 3000
+    deg = rng.integers(0, 30, n)
+    row = np.concatenate([[0], np.cumsum(deg)]).astype(np.int32)
+    g = smdp.HostGraph(row, np.full(int(row[-1]), 7, np.int32))
+    val = ((np.arange(g.m) % 5) - 2).astype(np.float32)
+    x = np.full(n, np.nan, np.float32)
+    x[7] = 3.0
+    exact = np.zeros(n, np.int64)
+    np.add.at(exact, np.repeat(np.arange(n), deg), val.astype(np.int64) * 3)
+    y, _ = ctx.spmv(g, val, x, sol)
+    assert np.array_equal(y.astype(np.int64), exact)
+
+
 def test_spmv_tuned_tile_boundaries_exact(ctx):
     """Small integer values make every sum exact: a partial dropped or added twice changes y."""
     for i, g in enumerate(_merge_mats()):
